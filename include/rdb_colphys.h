@@ -1,0 +1,256 @@
+/*
+ * rdb_colphys.h -- C ABI of the MI355X column-physics engine (vertical mixing).
+ *
+ * This library replaces, for one call site, the reference's per-column
+ * vertical-mixing chain of the Roundabout ocean model:
+ *
+ *   vmix_build_coefficients   src/core/ocean/dynamics/split_rk2/rdb_ocean_dyn.F90:1473-1634
+ *     vmix_compute_pp81       src/parameterizations/vertical/rdb_ocean_vmix.F90:1007-1074
+ *     vmix_apply_kpp_overlay  src/parameterizations/vertical/rdb_ocean_vmix.F90:1076-1516
+ *     vmix_split_kd_heat_salt src/parameterizations/vertical/rdb_ocean_vmix.F90:1850-1934
+ *     vmix_assemble           src/parameterizations/vertical/rdb_ocean_vmix.F90:2183-2385
+ *   ocean_surface_flux_apply_tracers
+ *                             src/parameterizations/vertical/rdb_ocean_surface_flux.F90:959-1142
+ *   vdiff_apply_tracers       src/parameterizations/vertical/rdb_ocean_vdiff.F90:1107-1414
+ *   vmix_apply_nonlocal_tendencies
+ *                             src/parameterizations/vertical/rdb_ocean_vmix.F90:1605-1714
+ *   eos_wright_impl           src/equation_of_state/rdb_eos.F90:909-959
+ *
+ * Conventions kept from the reference ABI (include/rdb_ocean.h):
+ *   - every entry point returns an int status; 0 is success and the codes
+ *     below carry the same numeric values as rdb_ocean.h:62-98;
+ *   - the specific reason for a failure goes to an error ring read with
+ *     rdb_colphys_last_error (snprintf semantics, rdb_ocean.h:302-314);
+ *   - the handle is opaque, NULL on failure, and destroy is idempotent and
+ *     NULL-safe (rdb_ocean.h:113-119);
+ *   - at most one live handle per process (rdb_ocean_api.F90:23-29);
+ *   - host arrays are Fortran column-major with i fastest, k = 1 at the bed
+ *     and k = nz at the surface, interfaces k = 1..nz+1 (k = 1 bed,
+ *     k = nz+1 surface), exactly the reference's multilayer_state layout
+ *     (src/core/rdb_multilayer_state.F90:105-339);
+ *   - the caller owns host arrays (copied from / into); the library owns the
+ *     device buffers.  One HIP stream per handle, single host thread, no
+ *     re-entrancy.  Calls that return host-visible data synchronise first.
+ *
+ * No torch types, no HIP types: plain pointers and sizes only.
+ */
+#ifndef RDB_COLPHYS_H
+#define RDB_COLPHYS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: identical numeric values to include/rdb_ocean.h:82-97. */
+#define RDB_COLPHYS_OK 0
+#define RDB_COLPHYS_ERR_CONFIG_PARSE 1
+#define RDB_COLPHYS_ERR_CONFIG_VALIDATE 2
+#define RDB_COLPHYS_ERR_SETUP 3
+#define RDB_COLPHYS_ERR_IO 5
+#define RDB_COLPHYS_ERR_BAD_HANDLE 10
+#define RDB_COLPHYS_ERR_ALREADY_EXISTS 11
+#define RDB_COLPHYS_ERR_NOT_INITIALISED 12
+#define RDB_COLPHYS_ERR_NOT_FOUND 13
+#define RDB_COLPHYS_ERR_BAD_SHAPE 14
+/* vmix_assemble's guard status (rdb_ocean_vmix.F90:2248-2251, 2265-2277). */
+#define RDB_COLPHYS_GUARD_TRIPPED 1
+
+/* Precision of the diffusivity assembly (BASELINE split-precision config). */
+#define RDB_COLPHYS_PREC_FP64 0  /* every stage in FP64                       */
+#define RDB_COLPHYS_PREC_SPLIT 1 /* EOS..clip in FP32, Thomas + totals in FP64 */
+
+/* Closure used for kv/kt/ks. */
+#define RDB_COLPHYS_CLOSURE_PLAIN 0     /* PP81 + KPP (the reference)              */
+#define RDB_COLPHYS_CLOSURE_SURROGATE 1 /* MLP with per-column plain fallback      */
+
+/*
+ * Knob set of the path.  Field names follow ocean_vmix_t
+ * (rdb_ocean_vmix.F90:198-611), eos_t (rdb_eos.F90:537-554),
+ * ocean_surface_flux_t (rdb_ocean_surface_flux.F90:203-215) and
+ * ocean_vdiff_t.  rdb_colphys_config_default() fills the reference defaults
+ * (src/core/rdb_config.F90:1743-1827).
+ */
+typedef struct rdb_colphys_config {
+  /* PP81 interior closure (rdb_ocean_vmix.F90:231-262) */
+  double pp81_nu0;      /* 1e-2  m^2/s */
+  double pp81_nu_bg;    /* 1e-4  m^2/s */
+  double pp81_kappa_bg; /* 1e-5  m^2/s */
+  double pp81_alpha;    /* 5     */
+  double rho0;          /* 1035  kg/m^3 (vmix, eos and surface flux share it) */
+  double shear2_floor;  /* 1e-10 s^-2 */
+  /* KPP overlay (rdb_ocean_vmix.F90:311-348) */
+  int use_kpp;          /* 1 */
+  int pad0;
+  double ri_crit;       /* 0.3 */
+  double c_vt2;         /* 1.8 */
+  double cs_nonlocal;   /* 6.3 */
+  /* EOS handle: dimensional linear sensitivities used by KPP's B0, and the
+   * scalar reference pressure of the Wright EOS (rdb_eos.F90:537-554). */
+  double alpha_T;       /* 1.7e-4 kg/m^3/degC */
+  double beta_S;        /* 7.6e-4 kg/m^3/psu  */
+  double p_ref;         /* 0 Pa */
+  /* surface flux slot (rdb_ocean_surface_flux.F90:203-215) */
+  double cp;            /* 3992 J/kg/K */
+  int has_heat;         /* 1: deposit Q_heat */
+  int has_salt;         /* 1: deposit Q_salt */
+  /* vmix_assemble gate (rdb_ocean_vmix.F90:352-380) */
+  double kv_bg;         /* 1e-4 */
+  double kt_bg;         /* 1e-5 */
+  double ks_bg;         /* 1e-5 */
+  double kv_max;        /* huge */
+  double kd_max;        /* huge */
+  int vmix_guard;       /* 0 */
+  /* contributors that are no-ops at the defaults; non-default values are
+   * refused at create (RDB_COLPHYS_ERR_CONFIG_VALIDATE) */
+  int conv_enable;      /* 0 */
+  double kv_ml_invz2;   /* 0 */
+  int ddiff_enable;     /* 0 */
+  int kd_smooth_iterations; /* 0 */
+  /* vdiff (rdb_ocean_vdiff.F90 face_thick :1928-1962) */
+  int use_harmonic;     /* 0: arithmetic face thickness */
+  int precision;        /* RDB_COLPHYS_PREC_* */
+} rdb_colphys_config;
+
+typedef struct rdb_colphys rdb_colphys;
+
+/* Per-handle measurement counters (filled by rdb_colphys_stats). */
+typedef struct rdb_colphys_counters {
+  long long steps;             /* column steps are steps * wet_columns     */
+  long long wet_columns;       /* packed wet columns                       */
+  long long land_columns;      /* columns outside the wet mask             */
+  double last_kernel_ms;       /* HIP-event time of the last step launch   */
+  double sum_kernel_ms;        /* sum of timed step launches               */
+  long long timed_launches;    /* number of launches in sum_kernel_ms      */
+  double algorithmic_bytes_per_step; /* B_col * wet_columns                */
+  double fallback_fraction;    /* surrogate closure: plain-path share      */
+  int block_threads;           /* autotuned launch shape                   */
+  int order_mode;              /* 0 natural order, 1 sorted by depth       */
+} rdb_colphys_counters;
+
+/* ---- configuration ------------------------------------------------------ */
+
+/* Fill *cfg with the reference defaults. */
+int rdb_colphys_config_default(rdb_colphys_config *cfg);
+
+/* Bytes per working-precision float of the state arrays (always 8);
+ * mirrors rdb_working_precision (rdb_ocean.h:300). */
+int rdb_colphys_working_precision(void);
+
+/* ---- lifecycle (mirrors rdb_ocean_create_from_string / destroy,
+ *      include/rdb_ocean.h:108-119) ---------------------------------------- */
+
+/*
+ * Build a plan for an nx_tot x ny_tot grid of columns with nz layers and
+ * n_passive passive tracers (diffused with ks, like MOM6's Kd_salt rule,
+ * rdb_ocean_vdiff.F90:1114-1131).
+ *   wet      (nx_tot, ny_tot)  cell wet mask (1 wet, 0 land), as ms%wet_mask;
+ *   k_top    (nx_tot, ny_tot)  static first-live scan start (ms%k_top), or
+ *                              NULL for nz everywhere;
+ *   sort_key (nx_tot, ny_tot)  per-column depth key (e.g. live-layer count)
+ *                              used only to order wet columns so that a
+ *                              wavefront holds columns of similar depth, or
+ *                              NULL for natural order.
+ * Columns with wet <= 0.5 are "land": the library keeps them out of the
+ * per-step launch and refreshes them whenever one of their inputs changes.
+ */
+int rdb_colphys_create(const rdb_colphys_config *cfg, int nx_tot, int ny_tot,
+                       int nz, int n_passive, const double *wet,
+                       const int *k_top, const int *sort_key,
+                       rdb_colphys **handle_out);
+
+/* Idempotent: NULL or an already-destroyed handle is a no-op success. */
+int rdb_colphys_destroy(rdb_colphys **handle);
+
+/* ---- state transfer (mirrors the rdb_ocean_set_* / get_*_ptr pairs,
+ *      include/rdb_ocean.h:338-446, with explicit copies) -------------------
+ *
+ * Field names and host shapes (n1, n2, n3):
+ *   "h_layer"        (nx, ny, nz)     layer thickness, m
+ *   "u_face_x_layer" (nx+1, ny, nz)   west-face x velocity, m/s
+ *   "v_face_y_layer" (nx, ny+1, nz)   south-face y velocity, m/s
+ *   "temperature"    (nx, ny, nz)     degC
+ *   "salinity"       (nx, ny, nz)     psu
+ *   "tracer_<n>"     (nx, ny, nz)     passive tracer n = 1..n_passive
+ *   "tau_x"          (nx+1, ny, 1)    east-face wind stress, N/m^2
+ *   "tau_y"          (nx, ny+1, 1)    north-face wind stress, N/m^2
+ *   "stress_shelf"   (nx, ny, 1)      ice-shelf stress magnitude (0 default)
+ *   "Q_heat"         (nx, ny, 1)      W/m^2
+ *   "Q_salt"         (nx, ny, 1)      salt flux
+ *   "bl_depth"       (nx, ny, 1)      KPP boundary-layer depth (lagged seed)
+ *   "kv" "kt" "ks"   (nx, ny, nz+1)   get only (set allowed for restart kv)
+ *   "gamma_t" "gamma_s" (nx, ny, nz+1) get only
+ *   "rho_layer"      (nx, ny, nz)     get; set overrides the EOS output
+ *   "stress_mag"     (nx, ny, 1)      get only
+ *   "b0"             (nx, ny, 1)      get only
+ *   "kbl"            (nx, ny, 1)      get only; KPP crossing layer (0 none),
+ *                                     stored as double
+ *   "surrogate_used" (nx, ny, 1)      get only; 1 where the MLP K was used
+ * A shape mismatch returns RDB_COLPHYS_ERR_BAD_SHAPE, an unknown name
+ * RDB_COLPHYS_ERR_NOT_FOUND.
+ */
+int rdb_colphys_set_field(rdb_colphys *h, const char *name, const double *data,
+                          int n1, int n2, int n3);
+int rdb_colphys_get_field(rdb_colphys *h, const char *name, double *out,
+                          int n1, int n2, int n3);
+
+/* ---- compute ------------------------------------------------------------- */
+
+/* A1: rho_layer from (T, S, h) with the Wright EOS (rdb_eos.F90:909-959). */
+int rdb_colphys_eos(rdb_colphys *h);
+
+/* A3-A10: PP81 -> KPP overlay -> split -> assemble on the device rho_layer;
+ * writes kv, kt, ks, gamma_t/s, bl_depth, b0, kbl
+ * (vmix_build_coefficients, rdb_ocean_dyn.F90:1473-1634).  Returns
+ * RDB_COLPHYS_GUARD_TRIPPED when cfg.vmix_guard is on and a negative or NaN
+ * diffusivity reached the gate. */
+int rdb_colphys_coefficients(rdb_colphys *h, double dt);
+
+/* A2: surface heat/salt deposit into the first live layer
+ * (ocean_surface_flux_apply_tracers, rdb_ocean_surface_flux.F90:959-1070). */
+int rdb_colphys_surface_flux(rdb_colphys *h, double dt);
+
+/* A11-A13: backward-Euler Thomas solve of T (kt), S and passives (ks)
+ * (vdiff_apply_tracers, rdb_ocean_vdiff.F90:1107-1226). */
+int rdb_colphys_diffuse_tracers(rdb_colphys *h, double dt);
+
+/* A14: KPP non-local tendencies on T and S
+ * (vmix_apply_nonlocal_tendencies, rdb_ocean_vmix.F90:1605-1692). */
+int rdb_colphys_nonlocal(rdb_colphys *h, double dt);
+
+/* A1-A14 fused: n_steps column steps in the order of apply_deferred_thermo
+ * (rdb_ocean_dyn.F90:3836-3856): EOS -> coefficients -> surface flux ->
+ * tracer diffusion -> non-local.  One launch per step over all wet columns,
+ * replayed from a captured hipGraph. */
+int rdb_colphys_step(rdb_colphys *h, double dt, int n_steps);
+
+/* Block until the handle's stream is idle. */
+int rdb_colphys_synchronize(rdb_colphys *h);
+
+/* Choose the launch shape and column order by timing candidates on a copy
+ * of the current state (the state is restored afterwards). */
+int rdb_colphys_autotune(rdb_colphys *h, double dt);
+
+/* RDB_COLPHYS_PREC_FP64 or RDB_COLPHYS_PREC_SPLIT. */
+int rdb_colphys_set_precision(rdb_colphys *h, int mode);
+
+/* RDB_COLPHYS_CLOSURE_PLAIN or RDB_COLPHYS_CLOSURE_SURROGATE. */
+int rdb_colphys_set_closure(rdb_colphys *h, int mode);
+
+/* A17: order-invariant column integrals sum(h*T), sum(h*S) over wet columns
+ * (integer-bin reproducible sum, after compute_total_tracer_efp,
+ * src/core/ocean/diag/rdb_ocean_console_stats.F90:1120). */
+int rdb_colphys_heat_salt_totals(rdb_colphys *h, double *heat, double *salt);
+
+/* Counters and timings; enable_timing brackets every step launch with HIP
+ * events on the handle's stream. */
+int rdb_colphys_set_timing(rdb_colphys *h, int enable_timing);
+int rdb_colphys_stats(rdb_colphys *h, rdb_colphys_counters *out);
+
+/* ---- error ring (mirrors rdb_ocean_last_error, rdb_ocean.h:302-314) ---- */
+int rdb_colphys_last_error(int idx, char *buf, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RDB_COLPHYS_H */

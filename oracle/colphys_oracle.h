@@ -1,0 +1,95 @@
+/*
+ * colphys_oracle.h -- CPU FP64 restatement of the reference vertical-mixing
+ * path.  TEST INFRASTRUCTURE ONLY: the checker for the HIP library and the
+ * timed CPU baseline.  Nothing in the product path links or calls it.
+ *
+ * Arrays use the reference layout: Fortran column-major, i fastest,
+ * 0-based here: IDX(i, j, k) = i + nx*(j + ny*k), k = 0 is the bed layer.
+ * Interface arrays have nz+1 levels (k = 0 bed, k = nz surface).
+ */
+#ifndef COLPHYS_ORACLE_H
+#define COLPHYS_ORACLE_H
+
+#include "../include/rdb_colphys.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_GRAVITY 9.80665
+#define ORC_H_VANISHED 1.5e-4
+#define ORC_H_DIV_EPS 1.0e-20
+
+double orc_cbrt(double x);
+
+void orc_eos_wright(int nx, int ny, int nz, const double *h, const double *S,
+                    const double *T, double *rho, double rho0, double p_ref);
+
+void orc_stress_mag(int nx, int ny, const double *tau_x, const double *tau_y,
+                    double *stress_mag);
+
+void orc_pp81(const rdb_colphys_config *cfg, int nx, int ny, int nz,
+              const double *u, const double *v, const double *h,
+              const double *rho, double *kv, double *kt);
+
+void orc_kpp_overlay(const rdb_colphys_config *cfg, int nx, int ny, int nz,
+                     const double *u, const double *v, const double *h,
+                     const double *rho, const double *stress_mag,
+                     const double *stress_shelf, const double *Q_heat,
+                     const double *Q_salt, double *bl_depth, double *kv,
+                     double *kt, double *gamma_t, double *gamma_s, double *b0,
+                     int *kbl);
+
+void orc_split(int nx, int ny, int nz, const double *kt, double *ks);
+
+int orc_assemble(const rdb_colphys_config *cfg, int nx, int ny, int nz,
+                 double *kv, double *kt, double *ks);
+
+void orc_surface_flux(const rdb_colphys_config *cfg, int nx, int ny, int nz,
+                      double dt, double *T, double *S, const double *h,
+                      const double *wet, const double *Q_heat,
+                      const double *Q_salt, const int *k_top);
+
+void orc_build_factorize(int nx, int ny, int nz, int use_harmonic, double dt,
+                         const double *K, const double *h, double *a,
+                         double *b, double *c);
+
+void orc_apply_factored(int nx, int ny, int nz, double *x, const double *a,
+                        const double *b, const double *c, double *rhs);
+
+void orc_vdiff_tracers(int nx, int ny, int nz, int use_harmonic, double dt,
+                       const double *kt, const double *ks, const double *h,
+                       double *T, double *S, int n_passive, double **passive);
+
+void orc_nonlocal(int nx, int ny, int nz, double dt, const double *gamma_t,
+                  const double *gamma_s, const double *h, double *T,
+                  double *S);
+
+/* Whole grid state for one handle-equivalent. */
+typedef struct orc_state {
+  int nx, ny, nz, n_passive;
+  double *h, *u, *v, *T, *S;
+  double **passive;
+  double *stress_mag, *stress_shelf, *Q_heat, *Q_salt, *wet;
+  int *k_top;
+  double *rho, *kv, *kt, *ks, *gamma_t, *gamma_s, *bl_depth, *b0;
+  int *kbl;
+  /* scratch for the tracer solve */
+  double *a, *b, *c, *rhs;
+} orc_state;
+
+/* One column step in the order of apply_deferred_thermo
+ * (rdb_ocean_dyn.F90:3836-3856).  Returns the assemble guard status. */
+int orc_column_step(const rdb_colphys_config *cfg, orc_state *st, double dt);
+
+/* Column integrals sum(h*c) over wet cells in long double (checker only). */
+void orc_totals(int nx, int ny, int nz, const double *h, const double *c,
+                const double *wet, double *total);
+
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,886 @@
+// colphys_api.hip -- the C ABI (include/rdb_colphys.h): handle lifecycle,
+// state transfer, launches, hipGraph capture of the step loop, launch-shape
+// autotuning, reproducible totals and the error ring.
+//
+// Conventions follow the reference ABI (include/rdb_ocean.h): int status on
+// every entry point, never abort, a ring of error messages, an opaque handle
+// that is NULL on failure, one live handle per process
+// (src/api/rdb_ocean_api.F90:9-45).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/rdb_colphys.h"
+#include "colphys_kernels.h"
+
+using colphys::Params;
+using colphys::StateView;
+
+namespace {
+
+std::deque<std::string> g_ring;  // index 0 = most recent
+bool g_live = false;
+
+int push_error(int status, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_ring.push_front(buf);
+  while (g_ring.size() > 32) g_ring.pop_back();
+  return status;
+}
+
+#define HIP_TRY(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t _e = (expr);                                                                    \
+    if (_e != hipSuccess)                                                                      \
+      return push_error(RDB_COLPHYS_ERR_SETUP, "%s failed: %s (%s:%d)", #expr,                 \
+                        hipGetErrorString(_e), __FILE__, __LINE__);                            \
+  } while (0)
+
+enum FieldKind { F_LAYER, F_IFACE, F_FACE_X, F_FACE_Y, F_TAU_X, F_TAU_Y, F_COL, F_COL_INT };
+
+struct FieldSpec {
+  FieldKind kind;
+  bool settable;
+  bool gettable;
+};
+
+}  // namespace
+
+struct rdb_colphys {
+  rdb_colphys_config cfg;
+  Params pr;
+  int nx, ny, nz, n_passive;
+  int n_all, n_wet, P;
+  std::vector<int> col_host;
+  std::vector<unsigned char> dry_host;  // per full-grid column
+  int* d_col = nullptr;
+  StateView s{};
+  double* d_stage = nullptr;
+  size_t stage_elems = 0;
+  double *d_txc = nullptr, *d_tyc = nullptr, *d_smag = nullptr, *d_shelf = nullptr;
+  unsigned int* d_guard = nullptr;
+  unsigned long long* d_acc = nullptr;
+  hipStream_t stream = nullptr;
+  bool static_dirty = true;
+  std::vector<double> h_u, h_v, h_taux, h_tauy;
+  // graph cache
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  double graph_dt = 0.0;
+  int graph_n = 0;
+  int graph_block = 0;
+  int graph_prec = -1;
+  bool graph_timing = false;
+  std::vector<hipEvent_t> events;
+  // measurement
+  bool timing = false;
+  double last_ms = 0.0, sum_ms = 0.0;
+  long long timed = 0, steps = 0;
+  int block = 256;
+  int order_mode = 0;
+  int precision = RDB_COLPHYS_PREC_FP64;
+  int closure = RDB_COLPHYS_CLOSURE_PLAIN;
+  std::vector<void*> allocations;
+};
+
+namespace {
+
+Params make_params(const rdb_colphys_config& c) {
+  Params p;
+  p.nu0 = c.pp81_nu0;
+  p.nu_bg = c.pp81_nu_bg;
+  p.kappa_bg = c.pp81_kappa_bg;
+  p.pp_alpha = c.pp81_alpha;
+  p.rho0 = c.rho0;
+  p.shear2_floor = c.shear2_floor;
+  p.ri_crit = c.ri_crit;
+  p.c_vt2 = c.c_vt2;
+  p.cs_nonlocal = c.cs_nonlocal;
+  p.alpha_T = c.alpha_T;
+  p.beta_S = c.beta_S;
+  p.p_ref = c.p_ref;
+  p.cp = c.cp;
+  p.kv_bg = c.kv_bg;
+  p.kt_bg = c.kt_bg;
+  p.ks_bg = c.ks_bg;
+  p.kv_max = c.kv_max;
+  p.kd_max = c.kd_max;
+  p.use_kpp = c.use_kpp;
+  p.has_heat = c.has_heat;
+  p.has_salt = c.has_salt;
+  p.use_harmonic = c.use_harmonic;
+  p.guard = c.vmix_guard;
+  return p;
+}
+
+bool lookup_field(const char* name, int n_passive, FieldSpec& spec, int& tracer) {
+  tracer = -1;
+  struct E {
+    const char* n;
+    FieldSpec s;
+  };
+  static const E table[] = {
+      {"h_layer", {F_LAYER, true, true}},
+      {"u_face_x_layer", {F_FACE_X, true, true}},
+      {"v_face_y_layer", {F_FACE_Y, true, true}},
+      {"temperature", {F_LAYER, true, true}},
+      {"salinity", {F_LAYER, true, true}},
+      {"tau_x", {F_TAU_X, true, true}},
+      {"tau_y", {F_TAU_Y, true, true}},
+      {"stress_shelf", {F_COL, true, true}},
+      {"Q_heat", {F_COL, true, true}},
+      {"Q_salt", {F_COL, true, true}},
+      {"bl_depth", {F_COL, true, true}},
+      {"kv", {F_IFACE, true, true}},
+      {"kt", {F_IFACE, true, true}},
+      {"ks", {F_IFACE, true, true}},
+      {"gamma_t", {F_IFACE, false, true}},
+      {"gamma_s", {F_IFACE, false, true}},
+      {"rho_layer", {F_LAYER, true, true}},
+      {"stress_mag", {F_COL, false, true}},
+      {"b0", {F_COL, false, true}},
+      {"kbl", {F_COL_INT, false, true}},
+  };
+  for (const E& e : table)
+    if (std::strcmp(e.n, name) == 0) {
+      spec = e.s;
+      return true;
+    }
+  if (std::strncmp(name, "tracer_", 7) == 0) {
+    int t = std::atoi(name + 7);
+    if (t >= 1 && t <= n_passive) {
+      spec = {F_LAYER, true, true};
+      tracer = t - 1;
+      return true;
+    }
+  }
+  return false;
+}
+
+bool valid(rdb_colphys* h) { return h != nullptr && g_live; }
+
+double* layer_ptr(rdb_colphys* h, const char* name, int tracer) {
+  StateView& s = h->s;
+  if (tracer >= 0) return s.tr + (size_t)tracer * (size_t)s.nz * (size_t)s.P;
+  if (!std::strcmp(name, "h_layer")) return s.h;
+  if (!std::strcmp(name, "temperature")) return s.T;
+  if (!std::strcmp(name, "salinity")) return s.S;
+  if (!std::strcmp(name, "rho_layer")) return s.rho;
+  if (!std::strcmp(name, "kv")) return s.kv;
+  if (!std::strcmp(name, "kt")) return s.kt;
+  if (!std::strcmp(name, "ks")) return s.ks;
+  if (!std::strcmp(name, "gamma_t")) return s.gamma_t;
+  if (!std::strcmp(name, "gamma_s")) return s.gamma_s;
+  if (!std::strcmp(name, "stress_shelf")) return h->d_shelf;
+  if (!std::strcmp(name, "Q_heat")) return s.Qh;
+  if (!std::strcmp(name, "Q_salt")) return s.Qs;
+  if (!std::strcmp(name, "bl_depth")) return s.bl;
+  if (!std::strcmp(name, "stress_mag")) return h->d_smag;
+  if (!std::strcmp(name, "b0")) return s.b0;
+  return nullptr;
+}
+
+int dalloc(rdb_colphys* h, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+  if (e != hipSuccess)
+    return push_error(RDB_COLPHYS_ERR_SETUP, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+  h->allocations.push_back(*p);
+  e = hipMemsetAsync(*p, 0, bytes ? bytes : 8, h->stream);
+  if (e != hipSuccess) return push_error(RDB_COLPHYS_ERR_SETUP, "hipMemsetAsync failed: %s", hipGetErrorString(e));
+  return RDB_COLPHYS_OK;
+}
+
+void release(rdb_colphys* h) {
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  for (hipEvent_t e : h->events) (void)hipEventDestroy(e);
+  h->events.clear();
+  for (void* p : h->allocations) (void)hipFree(p);
+  h->allocations.clear();
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  h->stream = nullptr;
+}
+
+inline dim3 grid1(int n, int block) { return dim3((unsigned)((n + block - 1) / block)); }
+
+int launch_check(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return push_error(RDB_COLPHYS_ERR_SETUP, "launch of %s failed: %s", what, hipGetErrorString(e));
+  return RDB_COLPHYS_OK;
+}
+
+// Launch the fused step over packed columns [p0, p1).
+int launch_step(rdb_colphys* h, double dt, int p0, int p1) {
+  if (p1 <= p0) return RDB_COLPHYS_OK;
+  const bool two = (h->cfg.kt_bg != h->cfg.ks_bg);
+  const bool split = (h->precision == RDB_COLPHYS_PREC_SPLIT);
+  const int ntr = h->n_passive;
+  dim3 g = grid1(p1 - p0, h->block), b(h->block);
+#define LAUNCH(R, N, TF) \
+  hipLaunchKernelGGL((colphys::k_column_step<R, N, TF>), g, b, 0, h->stream, h->s, h->pr, dt, p0, p1, h->d_guard)
+#define BY_NTR(R, TF)               \
+  switch (ntr) {                    \
+    case 0: LAUNCH(R, 0, TF); break; \
+    case 1: LAUNCH(R, 1, TF); break; \
+    case 2: LAUNCH(R, 2, TF); break; \
+    case 3: LAUNCH(R, 3, TF); break; \
+    default: LAUNCH(R, 4, TF); break; \
+  }
+  if (split) {
+    if (two) { BY_NTR(float, true) } else { BY_NTR(float, false) }
+  } else {
+    if (two) { BY_NTR(double, true) } else { BY_NTR(double, false) }
+  }
+#undef BY_NTR
+#undef LAUNCH
+  return launch_check("k_column_step");
+}
+
+// Staged sequence (one kernel per reference routine) over [p0, p1).
+int staged_coefficients(rdb_colphys* h, int p0, int p1) {
+  if (p1 <= p0) return RDB_COLPHYS_OK;
+  dim3 g = grid1(p1 - p0, h->block), b(h->block);
+  hipLaunchKernelGGL(colphys::k_pp81, g, b, 0, h->stream, h->s, h->pr, p0, p1);
+  if (h->cfg.use_kpp) hipLaunchKernelGGL(colphys::k_kpp, g, b, 0, h->stream, h->s, h->pr, p0, p1);
+  dim3 g2(g.x, (unsigned)(h->nz + 1));
+  hipLaunchKernelGGL(colphys::k_split_assemble, g2, b, 0, h->stream, h->s, h->pr, p0, p1, h->d_guard);
+  return launch_check("coefficients");
+}
+
+int read_guard(rdb_colphys* h, unsigned int* out) {
+  HIP_TRY(hipMemcpyAsync(out, h->d_guard, sizeof(unsigned int), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipMemsetAsync(h->d_guard, 0, sizeof(unsigned int), h->stream));
+  return RDB_COLPHYS_OK;
+}
+
+// Dry columns (no live layer) have outputs that depend only on h, u, v, tau
+// and Q: refresh them with one launch whenever one of those changed.
+int refresh_static(rdb_colphys* h, double dt) {
+  if (!h->static_dirty) return RDB_COLPHYS_OK;
+  int st = launch_step(h, dt, h->n_wet, h->n_all);
+  if (st) return st;
+  h->static_dirty = false;
+  return RDB_COLPHYS_OK;
+}
+
+void drop_graph(rdb_colphys* h) {
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->graph_exec = nullptr;
+  h->graph = nullptr;
+  h->graph_n = 0;
+}
+
+int ensure_events(rdb_colphys* h, int n) {
+  while ((int)h->events.size() < n) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    h->events.push_back(e);
+  }
+  return RDB_COLPHYS_OK;
+}
+
+// Capture n step launches (with event records between them when timing)
+// into a graph.
+int build_graph(rdb_colphys* h, double dt, int n) {
+  drop_graph(h);
+  if (h->timing) {
+    int st = ensure_events(h, n + 1);
+    if (st) return st;
+  }
+  HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int st = RDB_COLPHYS_OK;
+  if (h->timing) (void)hipEventRecord(h->events[0], h->stream);
+  for (int i = 0; i < n && st == RDB_COLPHYS_OK; ++i) {
+    st = launch_step(h, dt, 0, h->n_wet);
+    if (h->timing) (void)hipEventRecord(h->events[i + 1], h->stream);
+  }
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (st) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess)
+    return push_error(RDB_COLPHYS_ERR_SETUP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+  h->graph = g;
+  e = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    drop_graph(h);
+    return push_error(RDB_COLPHYS_ERR_SETUP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+  }
+  h->graph_dt = dt;
+  h->graph_n = n;
+  h->graph_block = h->block;
+  h->graph_prec = h->precision;
+  h->graph_timing = h->timing;
+  return RDB_COLPHYS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rdb_colphys_config_default(rdb_colphys_config* c) {
+  if (!c) return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "config_default: NULL config");
+  std::memset(c, 0, sizeof *c);
+  c->pp81_nu0 = 1.0e-2;
+  c->pp81_nu_bg = 1.0e-4;
+  c->pp81_kappa_bg = 1.0e-5;
+  c->pp81_alpha = 5.0;
+  c->rho0 = 1035.0;
+  c->shear2_floor = 1.0e-10;
+  c->use_kpp = 1;
+  c->ri_crit = 0.3;
+  c->c_vt2 = 1.8;
+  c->cs_nonlocal = 6.3;
+  c->alpha_T = 1.7e-4;
+  c->beta_S = 7.6e-4;
+  c->p_ref = 0.0;
+  c->cp = 3992.0;
+  c->has_heat = 1;
+  c->has_salt = 1;
+  c->kv_bg = 1.0e-4;
+  c->kt_bg = 1.0e-5;
+  c->ks_bg = 1.0e-5;
+  c->kv_max = 1.7976931348623157e308;
+  c->kd_max = 1.7976931348623157e308;
+  c->precision = RDB_COLPHYS_PREC_FP64;
+  return RDB_COLPHYS_OK;
+}
+
+int rdb_colphys_working_precision(void) { return 8; }
+
+int rdb_colphys_create(const rdb_colphys_config* cfg, int nx, int ny, int nz, int n_passive, const double* wet,
+                       const int* k_top, const int* sort_key, rdb_colphys** out) {
+  if (!out) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "create: NULL handle_out");
+  *out = nullptr;
+  if (g_live)
+    return push_error(RDB_COLPHYS_ERR_ALREADY_EXISTS, "create: a column-physics handle is already live");
+  if (!cfg || !wet) return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "create: NULL config or wet mask");
+  if (nx < 1 || ny < 1 || nz < 1 || n_passive < 0 || n_passive > 16)
+    return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "create: bad extents nx=%d ny=%d nz=%d n_passive=%d", nx, ny, nz,
+                      n_passive);
+  if ((long long)nx * ny > 0x7fffffffLL / 2)
+    return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "create: %d x %d columns exceed the index range", nx, ny);
+  if (cfg->conv_enable)
+    return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "convective adjustment (conv_enable) is not supported by the column engine");
+  if (cfg->kv_ml_invz2 != 0.0)
+    return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "kv_ml_invz2 != 0 is not supported by the column engine");
+  if (cfg->ddiff_enable)
+    return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "double diffusion (ddiff_enable) is not supported by the column engine");
+  if (cfg->kd_smooth_iterations != 0)
+    return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "kd_smooth_iterations > 0 needs a horizontal halo; not supported");
+  if (cfg->use_harmonic != 0 && cfg->use_harmonic != 1)
+    return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "use_harmonic must be 0 or 1");
+  if (cfg->precision != RDB_COLPHYS_PREC_FP64 && cfg->precision != RDB_COLPHYS_PREC_SPLIT)
+    return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "precision must be 0 (fp64) or 1 (split)");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return push_error(RDB_COLPHYS_ERR_SETUP, "create: no HIP device available");
+
+  rdb_colphys* h = new rdb_colphys();
+  h->cfg = *cfg;
+  h->pr = make_params(*cfg);
+  h->nx = nx;
+  h->ny = ny;
+  h->nz = nz;
+  h->n_passive = n_passive;
+  h->precision = cfg->precision;
+  const int ncol = nx * ny;
+  std::vector<int> wet_idx, dry_idx;
+  h->dry_host.assign((size_t)ncol, 0);
+  for (int c = 0; c < ncol; ++c) {
+    if (wet[c] > 0.5)
+      wet_idx.push_back(c);
+    else {
+      dry_idx.push_back(c);
+      h->dry_host[(size_t)c] = 1;
+    }
+  }
+  if (sort_key) {
+    std::stable_sort(wet_idx.begin(), wet_idx.end(), [&](int a, int b) { return sort_key[a] > sort_key[b]; });
+    h->order_mode = 1;
+  }
+  h->n_wet = (int)wet_idx.size();
+  h->n_all = ncol;
+  h->P = ((ncol + 63) / 64) * 64;
+  h->col_host = wet_idx;
+  h->col_host.insert(h->col_host.end(), dry_idx.begin(), dry_idx.end());
+
+  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete h;
+    return push_error(RDB_COLPHYS_ERR_SETUP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+  }
+  const size_t P = (size_t)h->P;
+  const size_t L = (size_t)nz * P, I = (size_t)(nz + 1) * P;
+  StateView& s = h->s;
+  s.P = h->P;
+  s.nz = nz;
+  s.n_passive = n_passive;
+  int st = RDB_COLPHYS_OK;
+  auto A = [&](void* pp, size_t bytes) {
+    if (st == RDB_COLPHYS_OK) st = dalloc(h, (void**)pp, bytes);
+  };
+  A(&h->d_col, sizeof(int) * P);
+  A(&s.h, 8 * L);
+  A(&s.ub, 8 * L);
+  A(&s.vb, 8 * L);
+  A(&s.T, 8 * L);
+  A(&s.S, 8 * L);
+  A(&s.tr, 8 * L * (size_t)std::max(n_passive, 1));
+  A(&s.rho, 8 * L);
+  A(&s.kv, 8 * I);
+  A(&s.kt, 8 * I);
+  A(&s.ks, 8 * I);
+  A(&s.gamma_t, 8 * I);
+  A(&s.gamma_s, 8 * I);
+  A(&s.fa, 8 * L);
+  A(&s.fb, 8 * L);
+  A(&s.fc, 8 * L);
+  A(&s.tau_tot, 8 * P);
+  A(&s.Qh, 8 * P);
+  A(&s.Qs, 8 * P);
+  A(&s.wet, 8 * P);
+  A(&s.bl, 8 * P);
+  A(&s.b0, 8 * P);
+  A(&s.kbl, 4 * P);
+  A(&s.ktop, 4 * P);
+  A(&h->d_txc, 8 * P);
+  A(&h->d_tyc, 8 * P);
+  A(&h->d_smag, 8 * P);
+  A(&h->d_shelf, 8 * P);
+  A(&h->d_guard, 64);
+  A(&h->d_acc, 64);
+  h->stage_elems = (size_t)(nx + 1) * (size_t)(ny + 1) * (size_t)(nz + 1);
+  A(&h->d_stage, 8 * h->stage_elems);
+  if (st) {
+    release(h);
+    delete h;
+    return st;
+  }
+  // column map, wet mask and k_top (0-based) in packed order
+  std::vector<int> col_pad(P, 0), ktop(P, nz - 1);
+  std::vector<double> wetp(P, 0.0);
+  for (int p = 0; p < ncol; ++p) {
+    int c = h->col_host[(size_t)p];
+    col_pad[(size_t)p] = c;
+    wetp[(size_t)p] = wet[c];
+    ktop[(size_t)p] = k_top ? std::min(std::max(k_top[c], 1), nz) - 1 : nz - 1;
+  }
+  if (hipMemcpy(h->d_col, col_pad.data(), sizeof(int) * P, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s.wet, wetp.data(), 8 * P, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s.ktop, ktop.data(), 4 * P, hipMemcpyHostToDevice) != hipSuccess) {
+    release(h);
+    delete h;
+    return push_error(RDB_COLPHYS_ERR_SETUP, "create: initial upload failed");
+  }
+  // a fresh state: every layer vanished (the land contract) until set
+  std::vector<double> hv(L, colphys::H_VANISHED);
+  if (hipMemcpy(s.h, hv.data(), 8 * L, hipMemcpyHostToDevice) != hipSuccess) {
+    release(h);
+    delete h;
+    return push_error(RDB_COLPHYS_ERR_SETUP, "create: initial upload failed");
+  }
+  h->h_u.assign((size_t)(nx + 1) * ny * nz, 0.0);
+  h->h_v.assign((size_t)nx * (ny + 1) * nz, 0.0);
+  h->h_taux.assign((size_t)(nx + 1) * ny, 0.0);
+  h->h_tauy.assign((size_t)nx * (ny + 1), 0.0);
+  if (hipStreamSynchronize(h->stream) != hipSuccess) {
+    release(h);
+    delete h;
+    return push_error(RDB_COLPHYS_ERR_SETUP, "create: stream synchronize failed");
+  }
+  g_live = true;
+  *out = h;
+  return RDB_COLPHYS_OK;
+}
+
+int rdb_colphys_destroy(rdb_colphys** hp) {
+  if (!hp || !*hp) return RDB_COLPHYS_OK;
+  rdb_colphys* h = *hp;
+  if (g_live) {
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    release(h);
+    delete h;
+    g_live = false;
+  }
+  *hp = nullptr;
+  return RDB_COLPHYS_OK;
+}
+
+int rdb_colphys_set_field(rdb_colphys* h, const char* name, const double* data, int n1, int n2, int n3) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "set_field: bad handle");
+  if (!name || !data) return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "set_field: NULL name or data");
+  FieldSpec spec;
+  int tracer;
+  if (!lookup_field(name, h->n_passive, spec, tracer))
+    return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "set_field: unknown field '%s'", name);
+  if (!spec.settable) return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "set_field: '%s' is read-only", name);
+  const int nx = h->nx, ny = h->ny, nz = h->nz;
+  int e1 = nx, e2 = ny, e3 = 1;
+  switch (spec.kind) {
+    case F_LAYER: e3 = nz; break;
+    case F_IFACE: e3 = nz + 1; break;
+    case F_FACE_X: e1 = nx + 1; e3 = nz; break;
+    case F_FACE_Y: e2 = ny + 1; e3 = nz; break;
+    case F_TAU_X: e1 = nx + 1; break;
+    case F_TAU_Y: e2 = ny + 1; break;
+    default: break;
+  }
+  if (n1 != e1 || n2 != e2 || n3 != e3)
+    return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "set_field '%s': shape (%d,%d,%d), expected (%d,%d,%d)", name, n1,
+                      n2, n3, e1, e2, e3);
+  const size_t n = (size_t)e1 * e2 * e3;
+  if (!std::strcmp(name, "h_layer")) {
+    // land contract (rdb_ocean_state_seed.F90:1842-1880): a dry column holds
+    // no live layer; its outputs are refreshed only when inputs change.
+    for (int k = 0; k < nz; ++k)
+      for (int c = 0; c < nx * ny; ++c)
+        if (h->dry_host[(size_t)c] && data[(size_t)k * nx * ny + c] > colphys::H_VANISHED)
+          return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE,
+                            "set_field h_layer: dry column (i=%d, j=%d) has a live layer k=%d (h=%g > H_VANISHED)",
+                            c % nx + 1, c / nx + 1, k + 1, data[(size_t)k * nx * ny + c]);
+  }
+  HIP_TRY(hipMemcpyAsync(h->d_stage, data, 8 * n, hipMemcpyHostToDevice, h->stream));
+  const int ncol = h->n_all, P = h->P, B = 256;
+  dim3 b(B);
+  switch (spec.kind) {
+    case F_LAYER:
+    case F_IFACE: {
+      double* dst = layer_ptr(h, name, tracer);
+      hipLaunchKernelGGL(colphys::k_pack_layer, dim3(grid1(ncol, B).x, e3), b, 0, h->stream, h->d_stage, dst, h->d_col,
+                         ncol, P, e3, (long long)nx * ny);
+      break;
+    }
+    case F_COL: {
+      double* dst = layer_ptr(h, name, tracer);
+      hipLaunchKernelGGL(colphys::k_pack_layer, dim3(grid1(ncol, B).x, 1), b, 0, h->stream, h->d_stage, dst, h->d_col,
+                         ncol, P, 1, (long long)nx * ny);
+      break;
+    }
+    case F_FACE_X:
+      std::copy(data, data + n, h->h_u.begin());
+      hipLaunchKernelGGL(colphys::k_pack_centre_x, dim3(grid1(ncol, B).x, nz), b, 0, h->stream, h->d_stage, h->s.ub,
+                         h->d_col, ncol, P, nz, nx, ny);
+      break;
+    case F_FACE_Y:
+      std::copy(data, data + n, h->h_v.begin());
+      hipLaunchKernelGGL(colphys::k_pack_centre_y, dim3(grid1(ncol, B).x, nz), b, 0, h->stream, h->d_stage, h->s.vb,
+                         h->d_col, ncol, P, nz, nx, ny);
+      break;
+    case F_TAU_X:
+      std::copy(data, data + n, h->h_taux.begin());
+      hipLaunchKernelGGL(colphys::k_pack_centre_x, dim3(grid1(ncol, B).x, 1), b, 0, h->stream, h->d_stage, h->d_txc,
+                         h->d_col, ncol, P, 1, nx, ny);
+      break;
+    case F_TAU_Y:
+      std::copy(data, data + n, h->h_tauy.begin());
+      hipLaunchKernelGGL(colphys::k_pack_centre_y, dim3(grid1(ncol, B).x, 1), b, 0, h->stream, h->d_stage, h->d_tyc,
+                         h->d_col, ncol, P, 1, nx, ny);
+      break;
+    default:
+      break;
+  }
+  int st = launch_check("pack");
+  if (st) return st;
+  if (spec.kind == F_TAU_X || spec.kind == F_TAU_Y || !std::strcmp(name, "stress_shelf")) {
+    hipLaunchKernelGGL(colphys::k_stress_mag, grid1(ncol, B), b, 0, h->stream, h->d_txc, h->d_tyc, h->d_smag,
+                       h->d_shelf, h->s.tau_tot, ncol);
+    st = launch_check("k_stress_mag");
+    if (st) return st;
+  }
+  h->static_dirty = true;
+  HIP_TRY(hipStreamSynchronize(h->stream));  // the caller may reuse its buffer
+  return RDB_COLPHYS_OK;
+}
+
+int rdb_colphys_get_field(rdb_colphys* h, const char* name, double* out, int n1, int n2, int n3) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "get_field: bad handle");
+  if (!name || !out) return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "get_field: NULL name or buffer");
+  FieldSpec spec;
+  int tracer;
+  if (!lookup_field(name, h->n_passive, spec, tracer))
+    return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "get_field: unknown field '%s'", name);
+  const int nx = h->nx, ny = h->ny, nz = h->nz;
+  int e1 = nx, e2 = ny, e3 = 1;
+  switch (spec.kind) {
+    case F_LAYER: e3 = nz; break;
+    case F_IFACE: e3 = nz + 1; break;
+    case F_FACE_X: e1 = nx + 1; e3 = nz; break;
+    case F_FACE_Y: e2 = ny + 1; e3 = nz; break;
+    case F_TAU_X: e1 = nx + 1; break;
+    case F_TAU_Y: e2 = ny + 1; break;
+    default: break;
+  }
+  if (n1 != e1 || n2 != e2 || n3 != e3)
+    return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "get_field '%s': shape (%d,%d,%d), expected (%d,%d,%d)", name, n1,
+                      n2, n3, e1, e2, e3);
+  const size_t n = (size_t)e1 * e2 * e3;
+  switch (spec.kind) {
+    case F_FACE_X: std::copy(h->h_u.begin(), h->h_u.end(), out); return RDB_COLPHYS_OK;
+    case F_FACE_Y: std::copy(h->h_v.begin(), h->h_v.end(), out); return RDB_COLPHYS_OK;
+    case F_TAU_X: std::copy(h->h_taux.begin(), h->h_taux.end(), out); return RDB_COLPHYS_OK;
+    case F_TAU_Y: std::copy(h->h_tauy.begin(), h->h_tauy.end(), out); return RDB_COLPHYS_OK;
+    default: break;
+  }
+  const int ncol = h->n_all, P = h->P, B = 256;
+  if (spec.kind == F_COL_INT) {
+    hipLaunchKernelGGL(colphys::k_unpack_int, grid1(ncol, B), dim3(B), 0, h->stream, h->s.kbl, h->d_stage, h->d_col,
+                       ncol);
+  } else {
+    double* src = layer_ptr(h, name, tracer);
+    hipLaunchKernelGGL(colphys::k_unpack_layer, dim3(grid1(ncol, B).x, e3), dim3(B), 0, h->stream, src, h->d_stage,
+                       h->d_col, ncol, P, e3, (long long)nx * ny);
+  }
+  int st = launch_check("unpack");
+  if (st) return st;
+  HIP_TRY(hipMemcpyAsync(out, h->d_stage, 8 * n, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return RDB_COLPHYS_OK;
+}
+
+int rdb_colphys_eos(rdb_colphys* h) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "eos: bad handle");
+  dim3 g(grid1(h->n_all, h->block).x, (unsigned)h->nz);
+  hipLaunchKernelGGL(colphys::k_eos, g, dim3(h->block), 0, h->stream, h->s, h->pr, 0, h->n_all);
+  return launch_check("k_eos");
+}
+
+int rdb_colphys_coefficients(rdb_colphys* h, double dt) {
+  (void)dt;
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "coefficients: bad handle");
+  int st = staged_coefficients(h, 0, h->n_all);
+  if (st) return st;
+  if (h->cfg.vmix_guard) {
+    unsigned int bad = 0;
+    st = read_guard(h, &bad);
+    if (st) return st;
+    if (bad) {
+      push_error(RDB_COLPHYS_GUARD_TRIPPED, "vmix_assemble: negative or NaN diffusivity detected (%u values)", bad);
+      return RDB_COLPHYS_GUARD_TRIPPED;
+    }
+  }
+  return RDB_COLPHYS_OK;
+}
+
+int rdb_colphys_surface_flux(rdb_colphys* h, double dt) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "surface_flux: bad handle");
+  if (!h->cfg.has_heat && !h->cfg.has_salt) return RDB_COLPHYS_OK;
+  hipLaunchKernelGGL(colphys::k_surface_flux, grid1(h->n_all, h->block), dim3(h->block), 0, h->stream, h->s, h->pr,
+                     dt, 0, h->n_all);
+  return launch_check("k_surface_flux");
+}
+
+int rdb_colphys_diffuse_tracers(rdb_colphys* h, double dt) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "diffuse_tracers: bad handle");
+  hipLaunchKernelGGL(colphys::k_vdiff, grid1(h->n_all, h->block), dim3(h->block), 0, h->stream, h->s, h->pr, dt, 0,
+                     h->n_all);
+  return launch_check("k_vdiff");
+}
+
+int rdb_colphys_nonlocal(rdb_colphys* h, double dt) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "nonlocal: bad handle");
+  if (!h->cfg.use_kpp) return RDB_COLPHYS_OK;
+  dim3 g(grid1(h->n_all, h->block).x, (unsigned)h->nz);
+  hipLaunchKernelGGL(colphys::k_nonlocal, g, dim3(h->block), 0, h->stream, h->s, dt, 0, h->n_all);
+  return launch_check("k_nonlocal");
+}
+
+int rdb_colphys_step(rdb_colphys* h, double dt, int n_steps) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "step: bad handle");
+  if (n_steps <= 0) return RDB_COLPHYS_OK;
+  if (h->closure != RDB_COLPHYS_CLOSURE_PLAIN)
+    return push_error(RDB_COLPHYS_ERR_SETUP, "step: the surrogate closure is not loaded");
+  int st = refresh_static(h, dt);
+  if (st) return st;
+  if (h->n_passive > 4) {
+    // beyond the fused kernel's tracer slots: the staged sequence, per step
+    for (int i = 0; i < n_steps; ++i) {
+      if ((st = rdb_colphys_eos(h)) || (st = staged_coefficients(h, 0, h->n_wet)) ||
+          (st = rdb_colphys_surface_flux(h, dt)) || (st = rdb_colphys_diffuse_tracers(h, dt)) ||
+          (st = rdb_colphys_nonlocal(h, dt)))
+        return st;
+    }
+  } else {
+    if (!h->graph_exec || h->graph_dt != dt || h->graph_n != n_steps || h->graph_block != h->block ||
+        h->graph_prec != h->precision || h->graph_timing != h->timing) {
+      st = build_graph(h, dt, n_steps);
+      if (st) return st;
+    }
+    HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
+    if (h->timing) {
+      HIP_TRY(hipStreamSynchronize(h->stream));
+      for (int i = 0; i < n_steps; ++i) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, h->events[(size_t)i], h->events[(size_t)i + 1]));
+        h->last_ms = ms;
+        h->sum_ms += ms;
+        h->timed += 1;
+      }
+    }
+  }
+  h->steps += n_steps;
+  if (h->cfg.vmix_guard) {
+    unsigned int bad = 0;
+    st = read_guard(h, &bad);
+    if (st) return st;
+    if (bad) {
+      push_error(RDB_COLPHYS_GUARD_TRIPPED, "vmix_assemble: negative or NaN diffusivity detected (%u values)", bad);
+      return RDB_COLPHYS_GUARD_TRIPPED;
+    }
+  }
+  return RDB_COLPHYS_OK;
+}
+
+int rdb_colphys_synchronize(rdb_colphys* h) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "synchronize: bad handle");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return RDB_COLPHYS_OK;
+}
+
+int rdb_colphys_autotune(rdb_colphys* h, double dt) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "autotune: bad handle");
+  int st = refresh_static(h, dt);
+  if (st) return st;
+  // back up the evolving state (T, S, passives, bl_depth)
+  const size_t L = (size_t)h->nz * (size_t)h->P;
+  const size_t nstate = L * (size_t)(2 + h->n_passive) + (size_t)h->P;
+  double* bak = nullptr;
+  HIP_TRY(hipMalloc((void**)&bak, 8 * nstate));
+  auto save = [&](bool restore) {
+    double* parts[3] = {h->s.T, h->s.S, h->s.tr};
+    size_t sizes[3] = {L, L, L * (size_t)h->n_passive};
+    size_t off = 0;
+    for (int q = 0; q < 3; ++q) {
+      if (sizes[q])
+        (void)hipMemcpyAsync(restore ? parts[q] : bak + off, restore ? bak + off : parts[q], 8 * sizes[q],
+                       hipMemcpyDeviceToDevice, h->stream);
+      off += sizes[q];
+    }
+    (void)hipMemcpyAsync(restore ? h->s.bl : bak + off, restore ? bak + off : h->s.bl, 8 * (size_t)h->P,
+                   hipMemcpyDeviceToDevice, h->stream);
+  };
+  save(false);
+  const int cands[3] = {64, 128, 256};
+  double best = 1e300;
+  int best_b = h->block;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  for (int b : cands) {
+    h->block = b;
+    st = launch_step(h, dt, 0, h->n_wet);  // warm-up
+    if (st) break;
+    (void)hipEventRecord(e0, h->stream);
+    for (int r = 0; r < 3 && !st; ++r) st = launch_step(h, dt, 0, h->n_wet);
+    (void)hipEventRecord(e1, h->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best) {
+      best = ms;
+      best_b = b;
+    }
+    save(true);
+  }
+  (void)hipStreamSynchronize(h->stream);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(bak);
+  h->block = best_b;
+  if (st) return st;
+  return RDB_COLPHYS_OK;
+}
+
+int rdb_colphys_set_precision(rdb_colphys* h, int mode) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "set_precision: bad handle");
+  if (mode != RDB_COLPHYS_PREC_FP64 && mode != RDB_COLPHYS_PREC_SPLIT)
+    return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "set_precision: unknown mode %d", mode);
+  if (mode != h->precision) h->static_dirty = true;
+  h->precision = mode;
+  return RDB_COLPHYS_OK;
+}
+
+int rdb_colphys_set_closure(rdb_colphys* h, int mode) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "set_closure: bad handle");
+  if (mode == RDB_COLPHYS_CLOSURE_PLAIN) {
+    h->closure = mode;
+    return RDB_COLPHYS_OK;
+  }
+  if (mode == RDB_COLPHYS_CLOSURE_SURROGATE)
+    return push_error(RDB_COLPHYS_ERR_SETUP, "set_closure: no surrogate network is loaded into this handle");
+  return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "set_closure: unknown mode %d", mode);
+}
+
+int rdb_colphys_heat_salt_totals(rdb_colphys* h, double* heat, double* salt) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "heat_salt_totals: bad handle");
+  if (!heat || !salt) return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "heat_salt_totals: NULL output");
+  HIP_TRY(hipMemsetAsync(h->d_acc, 0, 4 * sizeof(unsigned long long), h->stream));
+  int nb = std::min(2048, std::max(1, (h->n_wet + 255) / 256));
+  hipLaunchKernelGGL(colphys::k_totals, dim3(nb), dim3(256), 0, h->stream, h->s, 0, h->n_wet, h->d_acc);
+  int st = launch_check("k_totals");
+  if (st) return st;
+  unsigned long long acc[4];
+  HIP_TRY(hipMemcpyAsync(acc, h->d_acc, sizeof acc, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  auto combine = [](unsigned long long hi, unsigned long long lo) {
+    __int128 v = (__int128)(long long)hi * ((__int128)1 << 32) + (__int128)(long long)lo;
+    return (double)((long double)v / 17592186044416.0L);
+  };
+  *heat = combine(acc[0], acc[1]);
+  *salt = combine(acc[2], acc[3]);
+  return RDB_COLPHYS_OK;
+}
+
+int rdb_colphys_set_timing(rdb_colphys* h, int enable) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "set_timing: bad handle");
+  h->timing = enable != 0;
+  if (!h->timing) {
+    h->sum_ms = 0.0;
+    h->timed = 0;
+  }
+  return RDB_COLPHYS_OK;
+}
+
+int rdb_colphys_stats(rdb_colphys* h, rdb_colphys_counters* out) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "stats: bad handle");
+  if (!out) return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "stats: NULL output");
+  std::memset(out, 0, sizeof *out);
+  out->steps = h->steps;
+  out->wet_columns = h->n_wet;
+  out->land_columns = h->n_all - h->n_wet;
+  out->last_kernel_ms = h->last_ms;
+  out->sum_kernel_ms = h->sum_ms;
+  out->timed_launches = h->timed;
+  const double L = (double)h->nz;
+  out->algorithmic_bytes_per_step = (8.0 * (11.0 * L + 3.0 * (L + 1.0)) + 44.0) * (double)h->n_wet;
+  out->fallback_fraction = h->closure == RDB_COLPHYS_CLOSURE_PLAIN ? 1.0 : 0.0;
+  out->block_threads = h->block;
+  out->order_mode = h->order_mode;
+  return RDB_COLPHYS_OK;
+}
+
+int rdb_colphys_last_error(int idx, char* buf, int cap) {
+  if (idx < 0 || idx >= (int)g_ring.size() || cap <= 0 || !buf) return 0;
+  const std::string& m = g_ring[(size_t)idx];
+  std::snprintf(buf, (size_t)cap, "%s", m.c_str());
+  return (int)m.size();
+}
+
+}  // extern "C"

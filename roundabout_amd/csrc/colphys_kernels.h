@@ -1,0 +1,58 @@
+// colphys_kernels.h -- device state view and kernel declarations.
+#pragma once
+
+#include "colphys_device.h"
+
+namespace colphys {
+
+// Device pointers of one handle's packed state (pitch P, level-major).
+struct StateView {
+  int P, nz, n_passive, pad;
+  double* h;        // [nz][P]   layer thickness
+  double* ub;       // [nz][P]   centred x velocity
+  double* vb;       // [nz][P]   centred y velocity
+  double* T;        // [nz][P]
+  double* S;        // [nz][P]
+  double* tr;       // [n_passive][nz][P]
+  double* rho;      // [nz][P]   (staged path)
+  double* kv;       // [nz+1][P]
+  double* kt;       // [nz+1][P]
+  double* ks;       // [nz+1][P]
+  double* gamma_t;  // [nz+1][P] (staged path)
+  double* gamma_s;  // [nz+1][P] (staged path)
+  double* fa;       // [nz][P]   tridiagonal scratch
+  double* fb;       // [nz][P]
+  double* fc;       // [nz][P]
+  double* tau_tot;  // [P] stress_mag + stress_shelf
+  double* Qh;       // [P]
+  double* Qs;       // [P]
+  double* wet;      // [P]
+  double* bl;       // [P] bl_depth
+  double* b0;       // [P]
+  int* kbl;         // [P] KPP crossing layer (1-based, 0 none)
+  int* ktop;        // [P] 0-based first-live scan start
+};
+
+__global__ void k_pack_layer(const double* src, double* dst, const int* col, int ncol, int P, int nlev,
+                             long long plane);
+__global__ void k_unpack_layer(const double* src, double* dst, const int* col, int ncol, int P, int nlev,
+                               long long plane);
+__global__ void k_unpack_int(const int* src, double* dst, const int* col, int ncol);
+__global__ void k_pack_centre_x(const double* uf, double* dst, const int* col, int ncol, int P, int nlev, int nx,
+                                int ny);
+__global__ void k_pack_centre_y(const double* vf, double* dst, const int* col, int ncol, int P, int nlev, int nx,
+                                int ny);
+__global__ void k_stress_mag(const double* txc, const double* tyc, double* smag, const double* shelf,
+                             double* tau_tot, int ncol);
+__global__ void k_eos(StateView s, Params pr, int p0, int p1);
+__global__ void k_pp81(StateView s, Params pr, int p0, int p1);
+__global__ void k_kpp(StateView s, Params pr, int p0, int p1);
+__global__ void k_split_assemble(StateView s, Params pr, int p0, int p1, unsigned int* guard_count);
+__global__ void k_surface_flux(StateView s, Params pr, double dt, int p0, int p1);
+__global__ void k_vdiff(StateView s, Params pr, double dt, int p0, int p1);
+__global__ void k_nonlocal(StateView s, double dt, int p0, int p1);
+template <typename R, int NTR, bool TWO_FACTOR>
+__global__ void k_column_step(StateView s, Params pr, double dt, int p0, int p1, unsigned int* guard_count);
+__global__ void k_totals(StateView s, int p0, int p1, unsigned long long* acc);
+
+}  // namespace colphys

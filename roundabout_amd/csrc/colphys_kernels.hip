@@ -1,0 +1,656 @@
+// colphys_kernels.hip -- gfx950 kernels of the vertical-mixing column path.
+//
+// Data layout in HBM (all kernels): columns are packed, wet columns first
+// (ordered by a depth key so that a wavefront holds columns of similar
+// depth), then the dry columns.  Every per-level field is stored level-major
+// with a pitch P that is a multiple of 64: X[k * P + p].  One lane owns one
+// column, so the 64 lanes of a wavefront read 64 consecutive doubles (512 B)
+// of one level: fully coalesced, and each column's serial k-sweeps (KPP
+// bulk-Ri search, Thomas elimination) run down a register pipeline.
+#include "colphys_kernels.h"
+
+namespace colphys {
+
+#define LV(arr, k) (arr)[(size_t)(k) * (size_t)P + (size_t)p]
+
+// ---------------------------------------------------------------------------
+// Pack / unpack between the reference's full-grid layout (i fastest) and the
+// packed level-major layout.  col[p] = i + nx * j.
+// ---------------------------------------------------------------------------
+__global__ void k_pack_layer(const double* __restrict__ src, double* __restrict__ dst,
+                             const int* __restrict__ col, int ncol, int P, int nlev, long long plane) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int k = blockIdx.y;
+  if (p >= ncol || k >= nlev) return;
+  LV(dst, k) = src[(size_t)k * (size_t)plane + (size_t)col[p]];
+}
+
+__global__ void k_unpack_layer(const double* __restrict__ src, double* __restrict__ dst,
+                               const int* __restrict__ col, int ncol, int P, int nlev, long long plane) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int k = blockIdx.y;
+  if (p >= ncol || k >= nlev) return;
+  dst[(size_t)k * (size_t)plane + (size_t)col[p]] = LV(src, k);
+}
+
+__global__ void k_unpack_int(const int* __restrict__ src, double* __restrict__ dst,
+                             const int* __restrict__ col, int ncol) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ncol) return;
+  dst[col[p]] = (double)src[p];
+}
+
+// Centred velocity from faces: 0.5 * (u(i) + u(i+1)) (x faces) or
+// 0.5 * (v(j) + v(j+1)) (y faces), the face average PP81 and KPP form
+// (rdb_ocean_vmix.F90:1045-1048, 1302-1303).
+__global__ void k_pack_centre_x(const double* __restrict__ uf, double* __restrict__ dst,
+                                const int* __restrict__ col, int ncol, int P, int nlev, int nx, int ny) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int k = blockIdx.y;
+  if (p >= ncol || k >= nlev) return;
+  int c = col[p];
+  int i = c % nx, j = c / nx;
+  size_t base = (size_t)k * (size_t)(nx + 1) * (size_t)ny + (size_t)j * (size_t)(nx + 1) + (size_t)i;
+  LV(dst, k) = 0.5 * (uf[base] + uf[base + 1]);
+}
+
+__global__ void k_pack_centre_y(const double* __restrict__ vf, double* __restrict__ dst,
+                                const int* __restrict__ col, int ncol, int P, int nlev, int nx, int ny) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int k = blockIdx.y;
+  if (p >= ncol || k >= nlev) return;
+  int c = col[p];
+  int i = c % nx, j = c / nx;
+  size_t base = (size_t)k * (size_t)nx * (size_t)(ny + 1) + (size_t)j * (size_t)nx + (size_t)i;
+  LV(dst, k) = 0.5 * (vf[base] + vf[base + (size_t)nx]);
+}
+
+// A16 |tau| from centred stress (rdb_ocean_surface_stress.F90:867-874).
+__global__ void k_stress_mag(const double* __restrict__ txc, const double* __restrict__ tyc,
+                             double* __restrict__ smag, const double* __restrict__ shelf,
+                             double* __restrict__ tau_tot, int ncol) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ncol) return;
+  double tx = txc[p], ty = tyc[p];
+  double s = sqrt(tx * tx + ty * ty);
+  smag[p] = s;
+  tau_tot[p] = s + shelf[p];  // stress_mag + stress_shelf (rdb_ocean_vmix.F90:1318)
+}
+
+// ---------------------------------------------------------------------------
+// Staged kernels: one per reference routine, over packed columns [p0, p1).
+// ---------------------------------------------------------------------------
+__global__ void k_eos(StateView s, Params pr, int p0, int p1) {
+  int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  int k = blockIdx.y;
+  if (p >= p1 || k >= s.nz) return;
+  const int P = s.P;
+  LV(s.rho, k) = eos_wright<double>(LV(s.T, k), LV(s.S, k), LV(s.h, k), pr.rho0, pr.p_ref);
+}
+
+__global__ void k_pp81(StateView s, Params pr, int p0, int p1) {
+  int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= p1) return;
+  const int P = s.P, nz = s.nz;
+  LV(s.kv, 0) = 0.0;
+  LV(s.kt, 0) = 0.0;
+  LV(s.kv, nz) = 0.0;
+  LV(s.kt, nz) = 0.0;
+  double ub0 = LV(s.ub, 0), vb0 = LV(s.vb, 0), h0 = LV(s.h, 0), r0 = LV(s.rho, 0);
+  for (int k = 1; k < nz; ++k) {
+    double ub1 = LV(s.ub, k), vb1 = LV(s.vb, k), h1 = LV(s.h, k), r1 = LV(s.rho, k);
+    double kv, kt;
+    pp81_iface<double>(pr, ub0, vb0, ub1, vb1, h0, h1, r0, r1, kv, kt);
+    LV(s.kv, k) = kv;
+    LV(s.kt, k) = kt;
+    ub0 = ub1; vb0 = vb1; h0 = h1; r0 = r1;
+  }
+}
+
+// A4/A5 KPP overlay, both passes for one column (rdb_ocean_vmix.F90:1282-1515).
+__global__ void k_kpp(StateView s, Params pr, int p0, int p1) {
+  int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= p1) return;
+  const int P = s.P, nz = s.nz;
+  for (int k = 0; k <= nz; ++k) {
+    LV(s.gamma_t, k) = 0.0;
+    LV(s.gamma_s, k) = 0.0;
+  }
+  int k_sfc = nz - 1;
+  for (int k = nz - 1; k >= 0; --k)
+    if (LV(s.h, k) > H_VANISHED) { k_sfc = k; break; }
+  const double rho0 = pr.rho0;
+  double u_ref = LV(s.ub, k_sfc), v_ref = LV(s.vb, k_sfc);
+  double b_ref = -(GRAVITY * LV(s.rho, k_sfc) / rho0);
+  double d_centre_ref = 0.5 * LV(s.h, k_sfc);
+  double tau_mag = s.tau_tot[p];
+  double u_star = sqrt(tau_mag / rho0);
+  double qT = s.Qh[p] / (rho0 * pr.cp);
+  double qS = s.Qs[p] / rho0;
+  double B0 = kpp_b0<double>(pr, qT, qS);
+  double h_b_lagged = s.bl[p];
+  double wstar3_lagged = fmx(0.0, -B0) * h_b_lagged;
+  double cb = cbrt_kahan(wstar3_lagged);
+  double w_s_col = sqrt(u_star * u_star + cb * cb);
+  double ri_prev = 0.0, d_prev = d_centre_ref, h_b = 0.0, d_centre_k = d_centre_ref;
+  int kcross = 0;
+  double h_up = LV(s.h, k_sfc);
+  for (int k = k_sfc - 1; k >= 0; --k) {
+    double hk = LV(s.h, k);
+    d_centre_k = d_centre_k + 0.5 * (h_up + hk);
+    h_up = hk;
+    double u_k = LV(s.ub, k), v_k = LV(s.vb, k);
+    double b_k = -(GRAVITY * LV(s.rho, k) / rho0);
+    double delta_d = d_centre_k - d_centre_ref;
+    double delta_b = b_ref - b_k;
+    double n_brunt = sqrt(fmx(0.0, delta_b / delta_d));
+    double vt2 = (pr.c_vt2 / pr.ri_crit) * d_centre_k * n_brunt * w_s_col;
+    double du = u_k - u_ref, dv = v_k - v_ref;
+    double shear2 = fmx(du * du + dv * dv + vt2, pr.shear2_floor);
+    double ri_bulk = delta_b * delta_d / shear2;
+    if (ri_bulk >= pr.ri_crit) {
+      double denom = ri_bulk - ri_prev;
+      double frac = (fabs(denom) > 1.0e-12) ? (pr.ri_crit - ri_prev) / denom : 0.5;
+      h_b = d_prev + frac * (d_centre_k - d_prev);
+      kcross = k + 1;
+      break;
+    }
+    ri_prev = ri_bulk;
+    d_prev = d_centre_k;
+  }
+  if (kcross == 0) {
+    h_b = 0.0;
+    for (int k = 0; k <= k_sfc; ++k) h_b = h_b + LV(s.h, k);
+  }
+  s.bl[p] = h_b;
+  s.kbl[p] = kcross;
+  // pass 2
+  s.b0[p] = B0;
+  bool destab = (B0 < 0.0);
+  double wstar3 = fmx(0.0, -B0) * h_b;
+  double w_star = cbrt_kahan(wstar3);
+  double w_s = sqrt(u_star * u_star + w_star * w_star);
+  if (h_b > 0.0 && w_s > 0.0) {
+    double d_running = 0.0;
+    for (int k = k_sfc; k >= 1; --k) {
+      d_running = d_running + LV(s.h, k);
+      if (!(d_running < h_b)) break;  // d is non-decreasing: no later interface qualifies
+      double sigma = d_running / h_b;
+      double g_shape = sigma * (1.0 - sigma) * (1.0 - sigma);
+      double kv_kpp = h_b * w_s * g_shape;
+      LV(s.kv, k) = fmx(LV(s.kv, k), kv_kpp);
+      LV(s.kt, k) = fmx(LV(s.kt, k), kv_kpp);
+      if (destab) {
+        double gf = pr.cs_nonlocal * g_shape;
+        LV(s.gamma_t, k) = gf * qT;
+        LV(s.gamma_s, k) = gf * qS;
+      }
+    }
+  }
+}
+
+// A9 split + A10 assemble: guard count (before the clip), then floors and
+// ceilings on interior interfaces (rdb_ocean_vmix.F90:1921-1934, 2265-2385).
+__global__ void k_split_assemble(StateView s, Params pr, int p0, int p1, unsigned int* guard_count) {
+  int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  int k = blockIdx.y;
+  if (p >= p1 || k > s.nz) return;
+  const int P = s.P;
+  double kt = LV(s.kt, k);
+  double kv = LV(s.kv, k);
+  double ks = kt;
+  if (pr.guard) {
+    if (isnan(kv) || isnan(kt) || kv < 0.0 || kt < 0.0) atomicAdd(guard_count, 1u);
+  }
+  if (k >= 1 && k < s.nz) {
+    kv = fmn(fmx(kv, pr.kv_bg), pr.kv_max);
+    kt = fmn(fmx(kt, pr.kt_bg), pr.kd_max);
+    ks = fmn(fmx(ks, pr.ks_bg), pr.kd_max);
+  }
+  LV(s.kv, k) = kv;
+  LV(s.kt, k) = kt;
+  LV(s.ks, k) = ks;
+}
+
+// A2 surface deposit into the first live layer scanning down from k_top
+// (apply_surface_src_2d_impl, rdb_ocean_surface_flux.F90:1093-1142).
+__global__ void k_surface_flux(StateView s, Params pr, double dt, int p0, int p1) {
+  int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= p1) return;
+  const int P = s.P;
+  int kstart = s.ktop[p];
+  int kt = kstart;
+  for (int k = kstart; k >= 0; --k)
+    if (LV(s.h, k) > H_VANISHED) { kt = k; break; }
+  double hk = LV(s.h, kt);
+  double wet = s.wet[p];
+  if (pr.has_heat) {
+    double cell = dt / (pr.rho0 * pr.cp) * s.Qh[p] * wet;
+    LV(s.T, kt) = conc_add_source(LV(s.T, kt), hk, cell);
+  }
+  if (pr.has_salt) {
+    double cell = dt / pr.rho0 * s.Qs[p] * wet;
+    LV(s.S, kt) = conc_add_source(LV(s.S, kt), hk, cell);
+  }
+}
+
+// A12 factorise one diffusivity field into (a, b', c') scratch.
+__device__ void factorize_column(const StateView& s, const double* __restrict__ K, double dt, int use_harmonic,
+                                 int p) {
+  const int P = s.P, nz = s.nz;
+  double h_km1 = 0.0, h_k = LV(s.h, 0);
+  double cprev = 0.0;
+  for (int k = 0; k < nz; ++k) {
+    double h_kp1 = (k + 1 < nz) ? LV(s.h, k + 1) : 0.0;
+    double a, b, c;
+    tracer_row(k, nz, dt, use_harmonic, h_km1, h_k, h_kp1, LV(K, k), LV(K, k + 1), a, b, c);
+    double piv;
+    if (k == 0) {
+      cprev = c / b;
+      piv = b;
+    } else {
+      piv = b - a * cprev;
+      cprev = c / piv;
+    }
+    LV(s.fa, k) = a;
+    LV(s.fb, k) = piv;
+    LV(s.fc, k) = cprev;
+    h_km1 = h_k;
+    h_k = h_kp1;
+  }
+}
+
+// A13 forward sweep + back substitution of one tracer against stored pivots.
+__device__ void apply_column(const StateView& s, double* __restrict__ x, int p) {
+  const int P = s.P, nz = s.nz;
+  double r = LV(x, 0) / LV(s.fb, 0);
+  LV(x, 0) = r;
+  for (int k = 1; k < nz; ++k) {
+    r = (LV(x, k) - LV(s.fa, k) * r) / LV(s.fb, k);
+    LV(x, k) = r;
+  }
+  for (int k = nz - 2; k >= 0; --k) {
+    r = LV(x, k) - LV(s.fc, k) * r;
+    LV(x, k) = r;
+  }
+}
+
+// A11 two-pass dispatch: kt -> T; ks -> S and passives (rdb_ocean_vdiff.F90:1107-1226).
+__global__ void k_vdiff(StateView s, Params pr, double dt, int p0, int p1) {
+  int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= p1) return;
+  factorize_column(s, s.kt, dt, pr.use_harmonic, p);
+  apply_column(s, s.T, p);
+  factorize_column(s, s.ks, dt, pr.use_harmonic, p);
+  apply_column(s, s.S, p);
+  for (int t = 0; t < s.n_passive; ++t) apply_column(s, s.tr + (size_t)t * (size_t)s.nz * (size_t)s.P, p);
+}
+
+// A14 non-local tendencies on T (gamma_t) and S (gamma_s)
+// (rdb_ocean_vmix.F90:1652-1691).
+__global__ void k_nonlocal(StateView s, double dt, int p0, int p1) {
+  int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  int k = blockIdx.y;
+  if (p >= p1 || k >= s.nz) return;
+  const int P = s.P, nz = s.nz;
+  double hk = LV(s.h, k);
+  double hkp1 = (k < nz - 1) ? LV(s.h, k + 1) : 0.0;
+  double hkm1 = (k > 0) ? LV(s.h, k - 1) : 0.0;
+  for (int pass = 0; pass < 2; ++pass) {
+    const double* g = pass == 0 ? s.gamma_t : s.gamma_s;
+    double* x = pass == 0 ? s.T : s.S;
+    double f_up = 0.0, f_dn = 0.0;
+    if (k < nz - 1) f_up = nl_face(LV(g, k + 1), hk, hkp1);
+    if (k > 0) f_dn = nl_face(LV(g, k), hkm1, hk);
+    LV(x, k) = conc_add_source(LV(x, k), hk, dt * (f_up - f_dn));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused column step: A1-A14 for one column per lane in the order
+// EOS -> PP81 -> KPP -> split -> assemble -> surface flux -> Thomas -> non-local
+// (apply_deferred_thermo, rdb_ocean_dyn.F90:3836-3856).  Three sweeps:
+//   A  (surface -> down, boundary layer only): KPP bulk-Ri search with the
+//      EOS evaluated on the fly, then the KPP shape-function values K_k of the
+//      boundary-layer interfaces, parked in kv[] until sweep B reads them;
+//   B  (bed -> surface): EOS + PP81 + KPP max + split + clip at every
+//      interface (kv/kt/ks written once), the tracer row, the Thomas
+//      elimination (pivot-scaled super-diagonal c' to scratch, forward
+//      residuals r in place in T/S/tracers), surface deposit folded into the
+//      right-hand side of the first live layer;
+//   C  (surface -> bed): back substitution, then the non-local tendency with
+//      gamma recomputed from the same top-down running depth as the KPP pass.
+// Each value is produced by the same floating-point operations, in the same
+// order, as the staged kernels and the reference, so results are bitwise
+// equal.  TWO_FACTOR is needed only when kt_bg != ks_bg (kt != ks after the
+// clip); otherwise one factorisation of kt serves every tracer, which is the
+// identical arithmetic the reference repeats on its bitwise copy ks = kt.
+// ---------------------------------------------------------------------------
+template <typename R, int NTR, bool TWO_FACTOR>
+__global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, double dt, int p0, int p1,
+                                                     unsigned int* guard_count) {
+  int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= p1) return;
+  const int P = s.P, nz = s.nz;
+  const double HV = H_VANISHED;
+  const R rho0 = R(pr.rho0);
+  const R p_ref = R(pr.p_ref);
+
+  // ---------------- sweep A: KPP pass 1 + pass 2 values -------------------
+  int k_sfc = nz - 1;
+  for (int k = nz - 1; k >= 0; --k)
+    if (LV(s.h, k) > HV) { k_sfc = k; break; }
+
+  const double tau_mag = s.tau_tot[p];
+  const double qh = s.Qh[p], qsv = s.Qs[p];
+  int k_lo = nz + 1;  // lowest interface holding a KPP value (none: > nz)
+  R qT_r = R(0), qS_r = R(0), h_b_r = R(0), w_s_r = R(0);
+  bool destab = false, kpp_on = false;
+  double h_b_out = s.bl[p];
+  int kcross = 0;
+  if (pr.use_kpp) {
+    const R u_ref = R(LV(s.ub, k_sfc)), v_ref = R(LV(s.vb, k_sfc));
+    double h_sfc = LV(s.h, k_sfc);
+    R rho_sfc = eos_wright<R>(R(LV(s.T, k_sfc)), R(LV(s.S, k_sfc)), h_sfc, rho0, p_ref);
+    const R b_ref = -(R(GRAVITY) * rho_sfc / rho0);
+    const R d_centre_ref = R(0.5) * R(h_sfc);
+    R u_star = sqrt(R(tau_mag) / rho0);
+    qT_r = R(qh) / (rho0 * R(pr.cp));
+    qS_r = R(qsv) / rho0;
+    R B0 = kpp_b0<R>(pr, qT_r, qS_r);
+    R wstar3_lagged = fmx(R(0), -B0) * R(s.bl[p]);
+    R cb = cbrt_kahan(wstar3_lagged);
+    R w_s_col = sqrt(u_star * u_star + cb * cb);
+    const R vt2_coef = R(pr.c_vt2) / R(pr.ri_crit);
+    R ri_prev = R(0), d_prev = d_centre_ref, h_b = R(0), d_centre_k = d_centre_ref;
+    R h_up = R(h_sfc);
+    for (int k = k_sfc - 1; k >= 0; --k) {
+      double hk_d = LV(s.h, k);
+      R hk = R(hk_d);
+      d_centre_k = d_centre_k + R(0.5) * (h_up + hk);
+      h_up = hk;
+      R u_k = R(LV(s.ub, k)), v_k = R(LV(s.vb, k));
+      R rho_k = eos_wright<R>(R(LV(s.T, k)), R(LV(s.S, k)), hk_d, rho0, p_ref);
+      R b_k = -(R(GRAVITY) * rho_k / rho0);
+      R delta_d = d_centre_k - d_centre_ref;
+      R delta_b = b_ref - b_k;
+      R n_brunt = sqrt(fmx(R(0), delta_b / delta_d));
+      R vt2 = vt2_coef * d_centre_k * n_brunt * w_s_col;
+      R du = u_k - u_ref, dv = v_k - v_ref;
+      R shear2 = fmx(du * du + dv * dv + vt2, R(pr.shear2_floor));
+      R ri_bulk = delta_b * delta_d / shear2;
+      if (ri_bulk >= R(pr.ri_crit)) {
+        R denom = ri_bulk - ri_prev;
+        R frac = (fabs(denom) > R(1.0e-12)) ? (R(pr.ri_crit) - ri_prev) / denom : R(0.5);
+        h_b = d_prev + frac * (d_centre_k - d_prev);
+        kcross = k + 1;
+        break;
+      }
+      ri_prev = ri_bulk;
+      d_prev = d_centre_k;
+    }
+    if (kcross == 0) {
+      h_b = R(0);
+      for (int k = 0; k <= k_sfc; ++k) h_b = h_b + R(LV(s.h, k));
+    }
+    h_b_out = (double)h_b;
+    s.b0[p] = (double)B0;
+    destab = (B0 < R(0));
+    R wstar3 = fmx(R(0), -B0) * h_b;
+    R w_star = cbrt_kahan(wstar3);
+    R w_s = sqrt(u_star * u_star + w_star * w_star);
+    kpp_on = (h_b > R(0) && w_s > R(0));
+    h_b_r = h_b;
+    w_s_r = w_s;
+    if (kpp_on) {
+      R d_running = R(0);
+      for (int k = k_sfc; k >= 1; --k) {
+        d_running = d_running + R(LV(s.h, k));
+        if (!(d_running < h_b)) break;
+        R sigma = d_running / h_b;
+        R g_shape = sigma * (R(1) - sigma) * (R(1) - sigma);
+        R kv_kpp = h_b * w_s * g_shape;
+        LV(s.kv, k) = (double)kv_kpp;  // parked until sweep B
+        k_lo = k;
+      }
+    }
+    s.bl[p] = h_b_out;
+    s.kbl[p] = kcross;
+  }
+
+  // ---------------- sweep B: coefficients + Thomas elimination ------------
+  // first live layer for the surface deposit, scanning down from k_top
+  int k_fl = s.ktop[p];
+  for (int k = k_fl; k >= 0; --k)
+    if (LV(s.h, k) > HV) { k_fl = k; break; }
+  const double wetv = s.wet[p];
+  const double cellT = pr.has_heat ? dt / (pr.rho0 * pr.cp) * qh * wetv : 0.0;
+  const double cellS = pr.has_salt ? dt / pr.rho0 * qsv * wetv : 0.0;
+
+  double* const trb = s.tr;
+  const size_t tstride = (size_t)nz * (size_t)P;
+
+  // rolling state of the layer below (m1), current (0) and above (p1)
+  double h0 = LV(s.h, 0);
+  double T0 = LV(s.T, 0), S0 = LV(s.S, 0);
+  R ub0 = R(LV(s.ub, 0)), vb0 = R(LV(s.vb, 0));
+  R rho_0 = eos_wright<R>(R(T0), R(S0), h0, rho0, p_ref);
+  double hm1 = 0.0;
+  double Kt_below = 0.0, Ks_below = 0.0;  // interface 0 (bed)
+  LV(s.kv, 0) = 0.0;
+  LV(s.kt, 0) = 0.0;
+  LV(s.ks, 0) = 0.0;
+  double cp_t = 0.0, cp_s = 0.0;  // c' of the row below
+  double rT = 0.0, rS = 0.0, rtr[NTR > 0 ? NTR : 1];
+#pragma unroll
+  for (int t = 0; t < NTR; ++t) rtr[t] = 0.0;
+  unsigned int bad = 0;
+
+  for (int k = 0; k < nz; ++k) {
+    // interface k+1 (top of layer k): coefficients
+    double h1 = 0.0, T1 = 0.0, S1 = 0.0;
+    R ub1 = R(0), vb1 = R(0), rho_1 = R(0);
+    double Kt_above = 0.0, Ks_above = 0.0;
+    if (k + 1 < nz) {
+      h1 = LV(s.h, k + 1);
+      T1 = LV(s.T, k + 1);
+      S1 = LV(s.S, k + 1);
+      ub1 = R(LV(s.ub, k + 1));
+      vb1 = R(LV(s.vb, k + 1));
+      rho_1 = eos_wright<R>(R(T1), R(S1), h1, rho0, p_ref);
+      R kv_r, kt_r;
+      pp81_iface<R>(pr, ub0, vb0, ub1, vb1, R(h0), R(h1), rho_0, rho_1, kv_r, kt_r);
+      double kvv = (double)kv_r, ktv = (double)kt_r;
+      if (k + 1 >= k_lo && k + 1 <= k_sfc) {
+        double kk = LV(s.kv, k + 1);  // KPP value parked by sweep A
+        kvv = fmx(kvv, kk);
+        ktv = fmx(ktv, kk);
+      }
+      if (pr.guard) {
+        if (isnan(kvv) || isnan(ktv) || kvv < 0.0 || ktv < 0.0) ++bad;
+      }
+      double ksv = ktv;
+      kvv = fmn(fmx(kvv, pr.kv_bg), pr.kv_max);
+      ktv = fmn(fmx(ktv, pr.kt_bg), pr.kd_max);
+      ksv = fmn(fmx(ksv, pr.ks_bg), pr.kd_max);
+      LV(s.kv, k + 1) = kvv;
+      LV(s.kt, k + 1) = ktv;
+      LV(s.ks, k + 1) = ksv;
+      Kt_above = ktv;
+      Ks_above = ksv;
+    } else {
+      LV(s.kv, nz) = 0.0;
+      LV(s.kt, nz) = 0.0;
+      LV(s.ks, nz) = 0.0;
+    }
+
+    // Thomas row k and elimination
+    double a, b, c;
+    tracer_row(k, nz, dt, pr.use_harmonic, hm1, h0, h1, Kt_below, Kt_above, a, b, c);
+    double piv_t;
+    if (k == 0) {
+      cp_t = c / b;
+      piv_t = b;
+    } else {
+      piv_t = b - a * cp_t;
+      cp_t = c / piv_t;
+    }
+    double a_s = a, piv_s = piv_t;
+    if (TWO_FACTOR) {
+      double bs, cs;
+      tracer_row(k, nz, dt, pr.use_harmonic, hm1, h0, h1, Ks_below, Ks_above, a_s, bs, cs);
+      if (k == 0) {
+        cp_s = cs / bs;
+        piv_s = bs;
+      } else {
+        piv_s = bs - a_s * cp_s;
+        cp_s = cs / piv_s;
+      }
+      LV(s.fb, k) = cp_s;
+    }
+    LV(s.fc, k) = cp_t;
+
+    // right-hand sides (surface deposit on the first live layer)
+    double xT = T0, xS = S0;
+    if (k == k_fl) {
+      if (pr.has_heat) xT = conc_add_source(xT, h0, cellT);
+      if (pr.has_salt) xS = conc_add_source(xS, h0, cellS);
+    }
+    if (k == 0) {
+      rT = xT / piv_t;
+      rS = xS / piv_s;
+    } else {
+      rT = (xT - a * rT) / piv_t;
+      rS = (xS - a_s * rS) / piv_s;
+    }
+    LV(s.T, k) = rT;
+    LV(s.S, k) = rS;
+#pragma unroll
+    for (int t = 0; t < NTR; ++t) {
+      double* tr = trb + (size_t)t * tstride;
+      double x = LV(tr, k);
+      rtr[t] = (k == 0) ? x / piv_s : (x - a_s * rtr[t]) / piv_s;
+      LV(tr, k) = rtr[t];
+    }
+
+    // roll
+    hm1 = h0;
+    h0 = h1;
+    T0 = T1;
+    S0 = S1;
+    ub0 = ub1;
+    vb0 = vb1;
+    rho_0 = rho_1;
+    Kt_below = Kt_above;
+    Ks_below = Ks_above;
+  }
+
+  // ---------------- sweep C: back substitution + non-local ----------------
+  // gamma at interfaces, recomputed top-down exactly as the KPP pass 2 did.
+  double xT_up = 0.0, xS_up = 0.0, xtr_up[NTR > 0 ? NTR : 1];
+  double g_t_up = 0.0, g_s_up = 0.0;  // gamma at interface k+1
+  R d_running = R(0);
+  double h_up = 0.0;
+  double hk = LV(s.h, nz - 1);
+  const double dtv = dt;
+  for (int k = nz - 1; k >= 0; --k) {
+    double hdn = (k > 0) ? LV(s.h, k - 1) : 0.0;
+    double cpt = LV(s.fc, k);
+    double cps = TWO_FACTOR ? LV(s.fb, k) : cpt;
+    double xT = LV(s.T, k), xS = LV(s.S, k);
+    if (k < nz - 1) {
+      xT = xT - cpt * xT_up;
+      xS = xS - cps * xS_up;
+    }
+#pragma unroll
+    for (int t = 0; t < NTR; ++t) {
+      double* tr = trb + (size_t)t * tstride;
+      double x = LV(tr, k);
+      if (k < nz - 1) x = x - cps * xtr_up[t];
+      xtr_up[t] = x;
+      LV(tr, k) = x;
+    }
+    xT_up = xT;
+    xS_up = xS;
+    // gamma at interface k (bottom of layer k)
+    double g_t_dn = 0.0, g_s_dn = 0.0;
+    if (kpp_on && k <= k_sfc) {
+      d_running = d_running + R(hk);
+      if (k >= 1 && d_running < h_b_r && destab) {
+        R sigma = d_running / h_b_r;
+        R g_shape = sigma * (R(1) - sigma) * (R(1) - sigma);
+        R gf = R(pr.cs_nonlocal) * g_shape;
+        g_t_dn = (double)(gf * qT_r);
+        g_s_dn = (double)(gf * qS_r);
+      }
+    }
+    if (pr.use_kpp) {
+      double f_up_t = 0.0, f_dn_t = 0.0, f_up_s = 0.0, f_dn_s = 0.0;
+      if (k < nz - 1) {
+        f_up_t = nl_face(g_t_up, hk, h_up);
+        f_up_s = nl_face(g_s_up, hk, h_up);
+      }
+      if (k > 0) {
+        f_dn_t = nl_face(g_t_dn, hdn, hk);
+        f_dn_s = nl_face(g_s_dn, hdn, hk);
+      }
+      xT = conc_add_source(xT, hk, dtv * (f_up_t - f_dn_t));
+      xS = conc_add_source(xS, hk, dtv * (f_up_s - f_dn_s));
+    }
+    LV(s.T, k) = xT;
+    LV(s.S, k) = xS;
+    g_t_up = g_t_dn;
+    g_s_up = g_s_dn;
+    h_up = hk;
+    hk = hdn;
+  }
+  if (pr.guard && bad) atomicAdd(guard_count, bad);
+}
+
+// ---------------------------------------------------------------------------
+// A17 order-invariant integrals: each h*c product is rounded to a multiple of
+// 2^-44 and split into a signed high word and an unsigned low 32-bit word;
+// both are summed as 64-bit integers, which is associative, so the total does
+// not depend on the reduction order (the idea of Hallberg & Adcroft 2014).
+// ---------------------------------------------------------------------------
+__global__ void k_totals(StateView s, int p0, int p1, unsigned long long* acc) {
+  __shared__ long long sh[4][256];
+  int tid = threadIdx.x;
+  long long hiT = 0, loT = 0, hiS = 0, loS = 0;
+  const int P = s.P;
+  for (int p = p0 + blockIdx.x * blockDim.x + tid; p < p1; p += gridDim.x * blockDim.x) {
+    for (int k = 0; k < s.nz; ++k) {
+      double hk = LV(s.h, k);
+      double vt = hk * LV(s.T, k);
+      double vs = hk * LV(s.S, k);
+      long long it = __double2ll_rn(vt * 17592186044416.0);  // 2^44
+      long long is = __double2ll_rn(vs * 17592186044416.0);
+      hiT += it >> 32;
+      loT += (long long)(it & 0xffffffffll);
+      hiS += is >> 32;
+      loS += (long long)(is & 0xffffffffll);
+    }
+  }
+  sh[0][tid] = hiT;
+  sh[1][tid] = loT;
+  sh[2][tid] = hiS;
+  sh[3][tid] = loS;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (tid < w)
+      for (int q = 0; q < 4; ++q) sh[q][tid] += sh[q][tid + w];
+    __syncthreads();
+  }
+  if (tid == 0)
+    for (int q = 0; q < 4; ++q) atomicAdd(&acc[q], (unsigned long long)sh[q][0]);
+}
+
+// Explicit instantiations of the fused step.
+#define INST(R, N, TF) template __global__ void k_column_step<R, N, TF>(StateView, Params, double, int, int, unsigned int*);
+INST(double, 0, false) INST(double, 1, false) INST(double, 2, false) INST(double, 3, false) INST(double, 4, false)
+INST(double, 0, true) INST(double, 1, true) INST(double, 2, true) INST(double, 3, true) INST(double, 4, true)
+INST(float, 0, false) INST(float, 1, false) INST(float, 2, false) INST(float, 3, false) INST(float, 4, false)
+INST(float, 0, true) INST(float, 1, true) INST(float, 2, true) INST(float, 3, true) INST(float, 4, true)
+#undef INST
+
+}  // namespace colphys

@@ -1,0 +1,236 @@
+"""ctypes bindings to librdb_colphys.so (include/rdb_colphys.h).
+
+Mirrors the reference's ``python/rdb/_ffi.py`` (library discovery order,
+status codes, one prototype per exported symbol so a mismatched call raises
+``ctypes.ArgumentError`` instead of corrupting memory), for the column-physics
+library that replaces the vertical-mixing call site.
+
+Library discovery order (first hit wins), after ``python/rdb/_ffi.py:8-22``:
+
+  1. ``RDB_COLPHYS_LIB`` environment variable -- explicit path to the .so;
+  2. the in-tree build, ``roundabout_amd/lib/librdb_colphys.so``;
+  3. the dynamic-linker search path (``ctypes.CDLL("librdb_colphys.so")``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+# Status codes: same values as include/rdb_ocean.h:82-97.
+OK = 0
+ERR_CONFIG_PARSE = 1
+ERR_CONFIG_VALIDATE = 2
+ERR_SETUP = 3
+ERR_IO = 5
+ERR_BAD_HANDLE = 10
+ERR_ALREADY_EXISTS = 11
+ERR_NOT_INITIALISED = 12
+ERR_NOT_FOUND = 13
+ERR_BAD_SHAPE = 14
+GUARD_TRIPPED = 1
+
+PREC_FP64 = 0
+PREC_SPLIT = 1
+CLOSURE_PLAIN = 0
+CLOSURE_SURROGATE = 1
+
+_LIB_NAME = "librdb_colphys.so"
+PACKAGE_DIR = Path(__file__).resolve().parent.parent
+LIB_DIR = PACKAGE_DIR / "lib"
+
+
+class LibraryNotFoundError(FileNotFoundError):
+    """No usable librdb_colphys.so could be located (the HIP build is missing)."""
+
+
+class RdbError(RuntimeError):
+    """A non-zero status from the C ABI; ``status`` holds the code."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"status {status}: {message}")
+        self.status = status
+
+
+class Config(ctypes.Structure):
+    """ctypes mirror of ``rdb_colphys_config`` (field order is the ABI)."""
+
+    _fields_ = [
+        ("pp81_nu0", ctypes.c_double),
+        ("pp81_nu_bg", ctypes.c_double),
+        ("pp81_kappa_bg", ctypes.c_double),
+        ("pp81_alpha", ctypes.c_double),
+        ("rho0", ctypes.c_double),
+        ("shear2_floor", ctypes.c_double),
+        ("use_kpp", ctypes.c_int),
+        ("pad0", ctypes.c_int),
+        ("ri_crit", ctypes.c_double),
+        ("c_vt2", ctypes.c_double),
+        ("cs_nonlocal", ctypes.c_double),
+        ("alpha_T", ctypes.c_double),
+        ("beta_S", ctypes.c_double),
+        ("p_ref", ctypes.c_double),
+        ("cp", ctypes.c_double),
+        ("has_heat", ctypes.c_int),
+        ("has_salt", ctypes.c_int),
+        ("kv_bg", ctypes.c_double),
+        ("kt_bg", ctypes.c_double),
+        ("ks_bg", ctypes.c_double),
+        ("kv_max", ctypes.c_double),
+        ("kd_max", ctypes.c_double),
+        ("vmix_guard", ctypes.c_int),
+        ("conv_enable", ctypes.c_int),
+        ("kv_ml_invz2", ctypes.c_double),
+        ("ddiff_enable", ctypes.c_int),
+        ("kd_smooth_iterations", ctypes.c_int),
+        ("use_harmonic", ctypes.c_int),
+        ("precision", ctypes.c_int),
+    ]
+
+
+class Counters(ctypes.Structure):
+    """ctypes mirror of ``rdb_colphys_counters``."""
+
+    _fields_ = [
+        ("steps", ctypes.c_longlong),
+        ("wet_columns", ctypes.c_longlong),
+        ("land_columns", ctypes.c_longlong),
+        ("last_kernel_ms", ctypes.c_double),
+        ("sum_kernel_ms", ctypes.c_double),
+        ("timed_launches", ctypes.c_longlong),
+        ("algorithmic_bytes_per_step", ctypes.c_double),
+        ("fallback_fraction", ctypes.c_double),
+        ("block_threads", ctypes.c_int),
+        ("order_mode", ctypes.c_int),
+    ]
+
+
+def default_config() -> Config:
+    """Reference defaults (src/core/rdb_config.F90:1743-1893), filled in Python
+    so that hosts without the HIP library can still build a config."""
+    c = Config()
+    c.pp81_nu0 = 1.0e-2
+    c.pp81_nu_bg = 1.0e-4
+    c.pp81_kappa_bg = 1.0e-5
+    c.pp81_alpha = 5.0
+    c.rho0 = 1035.0
+    c.shear2_floor = 1.0e-10
+    c.use_kpp = 1
+    c.ri_crit = 0.3
+    c.c_vt2 = 1.8
+    c.cs_nonlocal = 6.3
+    c.alpha_T = 1.7e-4
+    c.beta_S = 7.6e-4
+    c.p_ref = 0.0
+    c.cp = 3992.0
+    c.has_heat = 1
+    c.has_salt = 1
+    c.kv_bg = 1.0e-4
+    c.kt_bg = 1.0e-5
+    c.ks_bg = 1.0e-5
+    c.kv_max = 1.7976931348623157e308
+    c.kd_max = 1.7976931348623157e308
+    c.vmix_guard = 0
+    c.conv_enable = 0
+    c.kv_ml_invz2 = 0.0
+    c.ddiff_enable = 0
+    c.kd_smooth_iterations = 0
+    c.use_harmonic = 0
+    c.precision = PREC_FP64
+    return c
+
+
+# (name, restype, argtypes) for every symbol declared in include/rdb_colphys.h
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_hp = ctypes.POINTER(ctypes.c_void_p)
+PROTOTYPES = [
+    ("rdb_colphys_config_default", ctypes.c_int, [ctypes.POINTER(Config)]),
+    ("rdb_colphys_working_precision", ctypes.c_int, []),
+    ("rdb_colphys_create", ctypes.c_int,
+     [ctypes.POINTER(Config), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+      _dp, _ip, _ip, _hp]),
+    ("rdb_colphys_destroy", ctypes.c_int, [_hp]),
+    ("rdb_colphys_set_field", ctypes.c_int,
+     [_vp, ctypes.c_char_p, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("rdb_colphys_get_field", ctypes.c_int,
+     [_vp, ctypes.c_char_p, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("rdb_colphys_eos", ctypes.c_int, [_vp]),
+    ("rdb_colphys_coefficients", ctypes.c_int, [_vp, ctypes.c_double]),
+    ("rdb_colphys_surface_flux", ctypes.c_int, [_vp, ctypes.c_double]),
+    ("rdb_colphys_diffuse_tracers", ctypes.c_int, [_vp, ctypes.c_double]),
+    ("rdb_colphys_nonlocal", ctypes.c_int, [_vp, ctypes.c_double]),
+    ("rdb_colphys_step", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int]),
+    ("rdb_colphys_synchronize", ctypes.c_int, [_vp]),
+    ("rdb_colphys_autotune", ctypes.c_int, [_vp, ctypes.c_double]),
+    ("rdb_colphys_set_precision", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("rdb_colphys_set_closure", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("rdb_colphys_heat_salt_totals", ctypes.c_int, [_vp, _dp, _dp]),
+    ("rdb_colphys_set_timing", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("rdb_colphys_stats", ctypes.c_int, [_vp, ctypes.POINTER(Counters)]),
+    ("rdb_colphys_last_error", ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
+]
+
+
+def _candidates():
+    env = os.environ.get("RDB_COLPHYS_LIB")
+    if env:
+        yield Path(env)
+    yield LIB_DIR / _LIB_NAME
+    yield Path(_LIB_NAME)
+
+
+_LIB = None
+
+
+def library_path() -> Path | None:
+    for cand in _candidates():
+        if cand.is_absolute() and cand.exists():
+            return cand
+    return None
+
+
+def load(path: str | os.PathLike | None = None) -> ctypes.CDLL:
+    """Load the library once and attach every prototype.
+
+    Raises ``LibraryNotFoundError`` when the HIP build is absent: there is no
+    CPU fallback in the product path."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    errors = []
+    cands = [Path(path)] if path is not None else list(_candidates())
+    for cand in cands:
+        try:
+            lib = ctypes.CDLL(str(cand))
+        except OSError as exc:
+            errors.append(f"{cand}: {exc}")
+            continue
+        for name, res, args in PROTOTYPES:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        if path is None:
+            _LIB = lib
+        return lib
+    raise LibraryNotFoundError(
+        "librdb_colphys.so not found or not loadable (build it with "
+        "__graft_entry__.build()): " + "; ".join(errors))
+
+
+def last_error(lib: ctypes.CDLL | None = None, idx: int = 0) -> str:
+    lib = lib or load()
+    cap = 1024
+    buf = ctypes.create_string_buffer(cap)
+    n = lib.rdb_colphys_last_error(idx, buf, cap)
+    return buf.value.decode(errors="replace") if n > 0 else ""
+
+
+def check(status: int, lib: ctypes.CDLL | None = None) -> int:
+    """Raise ``RdbError`` with the ring's most recent message on a non-zero
+    status (rdb_ocean.h:302-319 contract)."""
+    if status != OK:
+        raise RdbError(status, last_error(lib))
+    return status
