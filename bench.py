@@ -1,0 +1,212 @@
+"""Benchmark of the vertical-mixing column step on one MI355X.
+
+One step = one pass of the fused plain FP64 path (Wright EOS, PP81, KPP,
+split, assemble, surface deposit, Thomas solve of T/S/passives, non-local
+transport) over every wet column of the 1440 x 1080 x 75 z-level tile, with
+the inputs resident in HBM.  The timed region is exactly ``--steps`` steps
+replayed from one captured hipGraph, bracketed by stream synchronisation.
+
+Prints ONE JSON line (the contract): value = wet column-steps per second.
+``roofline`` uses the HIP-event duration of the step kernel (events recorded
+on the library's stream inside the graph) and the algorithmic bytes
+B_col = 8 (11 L + 3 (L + 1)) + 44 per wet column; ``cpu_baseline`` times the
+CPU oracle (an op-for-op C restatement, OpenMP over columns) on a bounded
+crop of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import numpy as np  # noqa: E402
+
+METRIC = json.loads((ROOT / "BASELINE.json").read_text())["metric"] if (ROOT / "BASELINE.json").exists() else \
+    "column-steps/s"
+PEAK_HBM_GBS = 8000.0
+DT = 3600.0
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_colphys_step_tile.txt"
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def read_traffic(workload: str):
+    """HBM bytes per step launch measured by rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, each scaled by the copy-kernel calibration), if recorded."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    vals = {}
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    if vals.get("workload") != workload:
+        return None
+    try:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(ocean, budget_s: float, crop_cols: tuple[int, int]):
+    from oracle import oracle as O
+    from roundabout_amd import synth
+    ci, cj = crop_cols
+    sub = synth.crop(ocean, 0, min(ci, ocean.nx), 0, min(cj, ocean.ny))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    O.set_threads(threads)
+    st = O.OracleState(sub)
+    n_wet = sub.n_wet
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        st.step(DT, 1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 24:
+            break
+    return {
+        "value": n_wet * steps / el,
+        "unit": "column-steps/s",
+        "cores": O.num_threads(),
+        "kind": "port",
+        "sample": f"oracle (C, -O2, no FMA contraction, OpenMP) on the {sub.nx}x{sub.ny}x{sub.nz} corner crop "
+                  f"of the same tile ({n_wet} wet columns), {steps} steps of dt={DT:.0f} s in {el:.1f} s",
+    }
+
+
+def copy_probe(nbytes: int = 1 << 30, reps: int = 3) -> None:
+    """Device-to-device copies of a known size (the runtime's copy kernel),
+    the calibration run for the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    for _ in range(reps):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-probe", action="store_true", help="run only the copy-kernel calibration")
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=24)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="tile", choices=["tile", "small", "thin"])
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "split"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+
+    if args.copy_probe:
+        copy_probe()
+        return 0
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the column path shards by columns "
+                          "(see DESIGN.md) but no multi-GPU run is part of it", "n_gpus": args.gpus}))
+        return 2
+
+    from roundabout_amd import synth
+    from roundabout_amd.rdb import PREC_FP64, PREC_SPLIT, ColumnPhysics, default_config
+
+    t0 = time.perf_counter()
+    ocean = synth.WORKLOADS[args.workload](args.seed)
+    log(f"[bench] generated {args.workload} {ocean.nx}x{ocean.ny}x{ocean.nz}, wet {ocean.n_wet} "
+        f"in {time.perf_counter() - t0:.1f} s")
+    cfg = default_config()
+    cp = ColumnPhysics.from_ocean(ocean, config=cfg)
+    cp.set_precision(PREC_SPLIT if args.precision == "split" else PREC_FP64)
+    log(f"[bench] uploaded in {time.perf_counter() - t0:.1f} s")
+    if not args.no_autotune:
+        cp.autotune(DT)
+    cp.set_timing(True)
+    if args.warmup > 0:
+        cp.step(DT, args.warmup)
+    cp.synchronize()
+    # capture + instantiate the K-step graph outside the timed region (its
+    # first replay is untimed too); the timed region replays the cached graph
+    cp.step(DT, args.steps)
+    cp.synchronize()
+    cp.set_timing(False)  # reset the event accumulators
+    cp.set_timing(True)
+    cp.synchronize()
+    t1 = time.perf_counter()
+    cp.step(DT, args.steps)
+    cp.synchronize()
+    wall = time.perf_counter() - t1
+    stats = cp.stats()
+    log(f"[bench] {args.steps} steps in {wall * 1e3:.2f} ms; kernel avg "
+        f"{stats['sum_kernel_ms'] / max(stats['timed_launches'], 1):.3f} ms; block {stats['block_threads']}")
+    heat, salt = cp.heat_salt_totals()
+    n_wet = int(stats["wet_columns"])
+    value = n_wet * args.steps / wall
+    kern_ms = stats["sum_kernel_ms"] / max(stats["timed_launches"], 1)
+    bytes_per_launch = stats["algorithmic_bytes_per_step"]
+    achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
+    workload = f"{args.workload} {ocean.nx}x{ocean.ny}x{ocean.nz}"
+    traffic = read_traffic(workload)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "column-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-assembly+f64-solve",
+        "data": "synthetic (seeded generator, roundabout_amd/synth.py)",
+        "config": {
+            "workload": f"{workload} z-level tile, 30% land, GRF bathymetry 50-5500 m; plain PP81+KPP "
+                        f"vmix_assemble + Thomas T/S/2 passives, dt={DT:.0f} s",
+            "wet_columns": n_wet,
+            "columns": ocean.nx * ocean.ny,
+            "nz": ocean.nz,
+            "path": "plain",
+            "precision": args.precision,
+            "kernel": "k_column_step (fused, one lane per column, hipGraph replay)",
+            "block_threads": stats["block_threads"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "kernel_ms": kern_ms,
+        },
+        "heat_salt_totals": [heat, salt],
+    }
+    cp.close()
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle) ...")
+        out["cpu_baseline"] = cpu_baseline(ocean, args.cpu_seconds, (384, 256))
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -118,7 +118,7 @@ typedef struct rdb_colphys_counters {
   long long steps;             /* column steps are steps * wet_columns     */
   long long wet_columns;       /* packed wet columns                       */
   long long land_columns;      /* columns outside the wet mask             */
-  double last_kernel_ms;       /* HIP-event time of the last step launch   */
+  double last_kernel_ms;       /* HIP-event time per step, last replay     */
   double sum_kernel_ms;        /* sum of timed step launches               */
   long long timed_launches;    /* number of launches in sum_kernel_ms      */
   double algorithmic_bytes_per_step; /* B_col * wet_columns                */
@@ -241,8 +241,9 @@ int rdb_colphys_set_closure(rdb_colphys *h, int mode);
  * src/core/ocean/diag/rdb_ocean_console_stats.F90:1120). */
 int rdb_colphys_heat_salt_totals(rdb_colphys *h, double *heat, double *salt);
 
-/* Counters and timings; enable_timing brackets every step launch with HIP
- * events on the handle's stream. */
+/* Counters and timings; enable_timing brackets every rdb_colphys_step
+ * graph replay with HIP events on the handle's stream (the bracketed region
+ * holds only the step kernels), accumulated into sum_kernel_ms. */
 int rdb_colphys_set_timing(rdb_colphys *h, int enable_timing);
 int rdb_colphys_stats(rdb_colphys *h, rdb_colphys_counters *out);
 

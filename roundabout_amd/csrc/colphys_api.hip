@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <numeric>
@@ -83,13 +84,13 @@ struct rdb_colphys {
   int graph_n = 0;
   int graph_block = 0;
   int graph_prec = -1;
-  bool graph_timing = false;
   std::vector<hipEvent_t> events;
   // measurement
   bool timing = false;
   double last_ms = 0.0, sum_ms = 0.0;
   long long timed = 0, steps = 0;
   int block = 256;
+  bool block_pinned = false;
   int order_mode = 0;
   int precision = RDB_COLPHYS_PREC_FP64;
   int closure = RDB_COLPHYS_CLOSURE_PLAIN;
@@ -298,17 +299,9 @@ int ensure_events(rdb_colphys* h, int n) {
 // into a graph.
 int build_graph(rdb_colphys* h, double dt, int n) {
   drop_graph(h);
-  if (h->timing) {
-    int st = ensure_events(h, n + 1);
-    if (st) return st;
-  }
   HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   int st = RDB_COLPHYS_OK;
-  if (h->timing) (void)hipEventRecord(h->events[0], h->stream);
-  for (int i = 0; i < n && st == RDB_COLPHYS_OK; ++i) {
-    st = launch_step(h, dt, 0, h->n_wet);
-    if (h->timing) (void)hipEventRecord(h->events[i + 1], h->stream);
-  }
+  for (int i = 0; i < n && st == RDB_COLPHYS_OK; ++i) st = launch_step(h, dt, 0, h->n_wet);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(h->stream, &g);
   if (st) {
@@ -327,7 +320,6 @@ int build_graph(rdb_colphys* h, double dt, int n) {
   h->graph_n = n;
   h->graph_block = h->block;
   h->graph_prec = h->precision;
-  h->graph_timing = h->timing;
   return RDB_COLPHYS_OK;
 }
 
@@ -401,6 +393,16 @@ int rdb_colphys_create(const rdb_colphys_config* cfg, int nx, int ny, int nz, in
   h->nz = nz;
   h->n_passive = n_passive;
   h->precision = cfg->precision;
+  // RDB_COLPHYS_BLOCK pins the launch shape (64, 128 or 256 threads) and
+  // turns rdb_colphys_autotune into a no-op, so that a profiled run can
+  // reproduce an autotuned configuration exactly.
+  if (const char* env = std::getenv("RDB_COLPHYS_BLOCK")) {
+    int b = std::atoi(env);
+    if (b == 64 || b == 128 || b == 256) {
+      h->block = b;
+      h->block_pinned = true;
+    }
+  }
   const int ncol = nx * ny;
   std::vector<int> wet_idx, dry_idx;
   h->dry_host.assign((size_t)ncol, 0);
@@ -719,20 +721,26 @@ int rdb_colphys_step(rdb_colphys* h, double dt, int n_steps) {
     }
   } else {
     if (!h->graph_exec || h->graph_dt != dt || h->graph_n != n_steps || h->graph_block != h->block ||
-        h->graph_prec != h->precision || h->graph_timing != h->timing) {
+        h->graph_prec != h->precision) {
       st = build_graph(h, dt, n_steps);
       if (st) return st;
     }
+    // Timing: events on the handle's stream around the graph replay; the
+    // bracketed region holds exactly the n_steps step kernels.
+    if (h->timing) {
+      st = ensure_events(h, 2);
+      if (st) return st;
+      HIP_TRY(hipEventRecord(h->events[0], h->stream));
+    }
     HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
     if (h->timing) {
-      HIP_TRY(hipStreamSynchronize(h->stream));
-      for (int i = 0; i < n_steps; ++i) {
-        float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, h->events[(size_t)i], h->events[(size_t)i + 1]));
-        h->last_ms = ms;
-        h->sum_ms += ms;
-        h->timed += 1;
-      }
+      HIP_TRY(hipEventRecord(h->events[1], h->stream));
+      HIP_TRY(hipEventSynchronize(h->events[1]));
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, h->events[0], h->events[1]));
+      h->last_ms = ms / n_steps;
+      h->sum_ms += ms;
+      h->timed += n_steps;
     }
   }
   h->steps += n_steps;
@@ -756,6 +764,7 @@ int rdb_colphys_synchronize(rdb_colphys* h) {
 
 int rdb_colphys_autotune(rdb_colphys* h, double dt) {
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "autotune: bad handle");
+  if (h->block_pinned) return RDB_COLPHYS_OK;
   int st = refresh_static(h, dt);
   if (st) return st;
   // back up the evolving state (T, S, passives, bl_depth)

@@ -112,7 +112,7 @@ class ColumnPhysics:
             return nx + 1, ny, 1
         if name == "tau_y":
             return nx, ny + 1, 1
-        return nx, ny, 1
+        return nx, ny, 1  # 2-D column fields; unknown names get NOT_FOUND from the library
 
     def set_field(self, name: str, data) -> None:
         n1, n2, n3 = self.shape_of(name)
@@ -127,7 +127,7 @@ class ColumnPhysics:
         out = np.empty((n1, n2, n3), order="F")
         st = self._lib.rdb_colphys_get_field(self._handle(), name.encode(), out.ctypes.data_as(_dp), n1, n2, n3)
         _ffi.check(st, self._lib)
-        if n3 == 1:
+        if name in COLUMN_FIELDS or name in ("tau_x", "tau_y"):
             out = out[:, :, 0]
         if name == "kbl":
             out = out.astype(np.int32)

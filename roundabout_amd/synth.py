@@ -199,6 +199,19 @@ def make_ocean(nx: int, ny: int, nz: int, seed: int = 1, *, land_frac: float = 0
                  np.asfortranarray(wet), k_top, np.asfortranarray(n_live), cor, bl)
 
 
+def crop(o: Ocean, i0: int, i1: int, j0: int, j1: int) -> Ocean:
+    """Sub-block [i0, i1) x [j0, j1) of an ocean with its bounding faces.
+
+    Every quantity of the path is column-local once the face averages are
+    formed, so a crop evolves exactly like the same columns of the full grid.
+    """
+    def c3(a, di=0, dj=0):
+        return np.asfortranarray(a[i0:i1 + di, j0:j1 + dj, ...].copy())
+    return Ocean(i1 - i0, j1 - j0, o.nz, o.n_passive, c3(o.h), c3(o.u, di=1), c3(o.v, dj=1), c3(o.T), c3(o.S),
+                 [c3(p) for p in o.passive], c3(o.tau_x, di=1), c3(o.tau_y, dj=1), c3(o.Q_heat), c3(o.Q_salt),
+                 c3(o.wet), c3(o.k_top), c3(o.n_live), c3(o.coriolis), c3(o.bl_depth))
+
+
 def small(seed: int = 1) -> Ocean:
     """64 x 64 x 20 z-star ocean, 10 % land (the CPU-runnable case)."""
     return make_ocean(64, 64, 20, seed, land_frac=0.1, topo="zstar", dmin=2000.0, dmax=5000.0)

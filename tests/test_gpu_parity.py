@@ -159,16 +159,24 @@ def test_single_layer_columns(orc, gpu_lib):
         compare_state(cp, ost, ocean.nz, ocean.n_passive)
 
 
-def test_guard_status(gpu_lib):
-    say("[case] vmix_guard trips on a NaN temperature (staged and fused)")
+def test_guard_status(orc, gpu_lib):
+    say("[case] vmix_guard trips on a negative raw diffusivity (staged and fused)")
+    # A negative PP81 background makes raw kv negative before the clip; the
+    # guard runs before the floors (rdb_ocean_vmix.F90:2265-2277).  A NaN
+    # temperature would not do: max(ri, 0) in PP81 maps a NaN Ri to 0.
     ocean = synth.small(10)
-    ocean.T[5, 5, 10] = np.nan
     cfg = default_config()
     cfg.vmix_guard = 1
+    cfg.pp81_nu_bg = -1.0
+    ost = orc.OracleState(ocean, cfg)
+    assert ost.step(DT, 1) == 1
     with ColumnPhysics.from_ocean(ocean, config=cfg) as cp:
         cp.eos()
         assert cp.coefficients(DT) == 1
         assert cp.step(DT, 1) == 1
+    cfg.pp81_nu_bg = 1.0e-4
+    with ColumnPhysics.from_ocean(ocean, config=cfg) as cp:
+        assert cp.step(DT, 2) == 0
 
 
 def test_heat_salt_totals_and_conservation(orc, gpu_lib):
