@@ -155,12 +155,13 @@ def main() -> int:
     wall = time.perf_counter() - t1
     stats = cp.stats()
     log(f"[bench] {args.steps} steps in {wall * 1e3:.2f} ms; kernel avg "
-        f"{stats['sum_kernel_ms'] / max(stats['timed_launches'], 1):.3f} ms; block {stats['block_threads']}")
+        f"{stats['sum_kernel_ms'] / max(stats['timed_launches'], 1):.3f} ms; block {stats['block_threads']} "
+        f"lds_levels {stats['lds_levels']}")
     heat, salt = cp.heat_salt_totals()
     n_wet = int(stats["wet_columns"])
     value = n_wet * args.steps / wall
     kern_ms = stats["sum_kernel_ms"] / max(stats["timed_launches"], 1)
-    bytes_per_launch = stats["algorithmic_bytes_per_step"]
+    bytes_per_launch = stats["algorithmic_bytes_per_step"]  # live levels (bed fillers are static)
     achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
     workload = f"{args.workload} {ocean.nx}x{ocean.ny}x{ocean.nz}"
     traffic = read_traffic(workload)
@@ -187,6 +188,8 @@ def main() -> int:
             "precision": args.precision,
             "kernel": "k_column_step (fused, one lane per column, hipGraph replay)",
             "block_threads": stats["block_threads"],
+            "lds_levels": stats["lds_levels"],
+            "live_levels_mean": stats["live_levels"] / max(stats["wet_columns"], 1),
         },
         "roofline": {
             "bound": "hbm",
@@ -196,6 +199,7 @@ def main() -> int:
             "frac": achieved / PEAK_HBM_GBS,
             "traffic": traffic,
             "algorithmic_bytes_per_launch": bytes_per_launch,
+            "all_level_bytes_per_launch": stats["reference_bytes_per_step"],
             "kernel_ms": kern_ms,
         },
         "heat_salt_totals": [heat, salt],

@@ -121,10 +121,14 @@ typedef struct rdb_colphys_counters {
   double last_kernel_ms;       /* HIP-event time per step, last replay     */
   double sum_kernel_ms;        /* sum of timed step launches               */
   long long timed_launches;    /* number of launches in sum_kernel_ms      */
-  double algorithmic_bytes_per_step; /* B_col * wet_columns                */
+  double algorithmic_bytes_per_step; /* sum over wet columns of B_col(L_c), L_c = live levels */
+  double reference_bytes_per_step;   /* B_col(nz) * wet_columns (all levels)  */
+  long long live_levels;       /* sum of live levels over wet columns      */
   double fallback_fraction;    /* surrogate closure: plain-path share      */
   int block_threads;           /* autotuned launch shape                   */
   int order_mode;              /* 0 natural order, 1 sorted by depth       */
+  int lds_levels;              /* autotuned live levels kept in LDS        */
+  int pad1;
 } rdb_colphys_counters;
 
 /* ---- configuration ------------------------------------------------------ */

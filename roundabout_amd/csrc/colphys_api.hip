@@ -83,6 +83,7 @@ struct rdb_colphys {
   double graph_dt = 0.0;
   int graph_n = 0;
   int graph_block = 0;
+  int graph_lds = -1;
   int graph_prec = -1;
   std::vector<hipEvent_t> events;
   // measurement
@@ -90,6 +91,8 @@ struct rdb_colphys {
   double last_ms = 0.0, sum_ms = 0.0;
   long long timed = 0, steps = 0;
   int block = 256;
+  int lds_levels = 0;  // live levels of Thomas state kept in LDS per lane
+  long long live_levels = 0;  // sum over wet columns of live levels
   bool block_pinned = false;
   int order_mode = 0;
   int precision = RDB_COLPHYS_PREC_FP64;
@@ -230,8 +233,11 @@ int launch_step(rdb_colphys* h, double dt, int p0, int p1) {
   const bool split = (h->precision == RDB_COLPHYS_PREC_SPLIT);
   const int ntr = h->n_passive;
   dim3 g = grid1(p1 - p0, h->block), b(h->block);
-#define LAUNCH(R, N, TF) \
-  hipLaunchKernelGGL((colphys::k_column_step<R, N, TF>), g, b, 0, h->stream, h->s, h->pr, dt, p0, p1, h->d_guard)
+  const int M = h->lds_levels;
+  const size_t shmem = (size_t)h->block * (size_t)M * (size_t)(3 + std::min(ntr, 4) + (two ? 1 : 0)) * 8;
+#define LAUNCH(R, N, TF)                                                                                      \
+  hipLaunchKernelGGL((colphys::k_column_step<R, N, TF>), g, b, shmem, h->stream, h->s, h->pr, dt, p0, p1, \
+                     h->d_guard, M)
 #define BY_NTR(R, TF)               \
   switch (ntr) {                    \
     case 0: LAUNCH(R, 0, TF); break; \
@@ -272,7 +278,10 @@ int read_guard(rdb_colphys* h, unsigned int* out) {
 // and Q: refresh them with one launch whenever one of those changed.
 int refresh_static(rdb_colphys* h, double dt) {
   if (!h->static_dirty) return RDB_COLPHYS_OK;
-  int st = launch_step(h, dt, h->n_wet, h->n_all);
+  hipLaunchKernelGGL(colphys::k_prepare, grid1(h->n_all, 256), dim3(256), 0, h->stream, h->s, h->pr, 0, h->n_all);
+  int st = launch_check("k_prepare");
+  if (st) return st;
+  st = launch_step(h, dt, h->n_wet, h->n_all);
   if (st) return st;
   h->static_dirty = false;
   return RDB_COLPHYS_OK;
@@ -319,6 +328,7 @@ int build_graph(rdb_colphys* h, double dt, int n) {
   h->graph_dt = dt;
   h->graph_n = n;
   h->graph_block = h->block;
+  h->graph_lds = h->lds_levels;
   h->graph_prec = h->precision;
   return RDB_COLPHYS_OK;
 }
@@ -401,6 +411,10 @@ int rdb_colphys_create(const rdb_colphys_config* cfg, int nx, int ny, int nz, in
     if (b == 64 || b == 128 || b == 256) {
       h->block = b;
       h->block_pinned = true;
+      const char* lv = std::getenv("RDB_COLPHYS_LDS_LEVELS");
+      int m = lv ? std::atoi(lv) : 0;
+      const size_t nq = (size_t)(3 + std::min(n_passive, 4) + (cfg->kt_bg != cfg->ks_bg ? 1 : 0));
+      if (m > 0 && (size_t)b * (size_t)m * nq * 8 <= 65536) h->lds_levels = m;
     }
   }
   const int ncol = nx * ny;
@@ -463,6 +477,8 @@ int rdb_colphys_create(const rdb_colphys_config* cfg, int nx, int ny, int nz, in
   A(&s.b0, 8 * P);
   A(&s.kbl, 4 * P);
   A(&s.ktop, 4 * P);
+  A(&s.kb, 4 * P);
+  A(&s.fdirty, 4 * P);
   A(&h->d_txc, 8 * P);
   A(&h->d_tyc, 8 * P);
   A(&h->d_smag, 8 * P);
@@ -558,6 +574,18 @@ int rdb_colphys_set_field(rdb_colphys* h, const char* name, const double* data, 
           return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE,
                             "set_field h_layer: dry column (i=%d, j=%d) has a live layer k=%d (h=%g > H_VANISHED)",
                             c % nx + 1, c / nx + 1, k + 1, data[(size_t)k * nx * ny + c]);
+  }
+  if (!std::strcmp(name, "h_layer")) {
+    // live levels per wet column (from the lowest live layer up), for the
+    // algorithmic-bytes figure of the step
+    long long live = 0;
+    for (int p = 0; p < h->n_wet; ++p) {
+      int c = h->col_host[(size_t)p], kb = nz;
+      for (int k = 0; k < nz; ++k)
+        if (data[(size_t)k * nx * ny + c] > colphys::H_VANISHED) { kb = k; break; }
+      live += nz - kb;
+    }
+    h->live_levels = live;
   }
   HIP_TRY(hipMemcpyAsync(h->d_stage, data, 8 * n, hipMemcpyHostToDevice, h->stream));
   const int ncol = h->n_all, P = h->P, B = 256;
@@ -669,6 +697,7 @@ int rdb_colphys_coefficients(rdb_colphys* h, double dt) {
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "coefficients: bad handle");
   int st = staged_coefficients(h, 0, h->n_all);
   if (st) return st;
+  h->static_dirty = true;  // the staged pass rewrote every interface
   if (h->cfg.vmix_guard) {
     unsigned int bad = 0;
     st = read_guard(h, &bad);
@@ -721,7 +750,7 @@ int rdb_colphys_step(rdb_colphys* h, double dt, int n_steps) {
     }
   } else {
     if (!h->graph_exec || h->graph_dt != dt || h->graph_n != n_steps || h->graph_block != h->block ||
-        h->graph_prec != h->precision) {
+        h->graph_lds != h->lds_levels || h->graph_prec != h->precision) {
       st = build_graph(h, dt, n_steps);
       if (st) return st;
     }
@@ -786,14 +815,24 @@ int rdb_colphys_autotune(rdb_colphys* h, double dt) {
                    hipMemcpyDeviceToDevice, h->stream);
   };
   save(false);
-  const int cands[3] = {64, 128, 256};
+  // candidates: launch block x live levels of Thomas state held in LDS
+  // (dynamic LDS per block <= 64 KiB)
+  const size_t nq = (size_t)(3 + std::min(h->n_passive, 4) + (h->cfg.kt_bg != h->cfg.ks_bg ? 1 : 0));
+  std::vector<std::pair<int, int>> cands;
+  for (int b : {64, 128, 256}) {
+    int mmax = (int)(65536 / ((size_t)b * nq * 8));
+    for (int m : {0, 4, 8, 12, 16, 24}) {
+      if (m <= mmax) cands.push_back({b, m});
+    }
+  }
   double best = 1e300;
-  int best_b = h->block;
+  int best_b = h->block, best_m = h->lds_levels;
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  for (int b : cands) {
+  for (auto [b, mm] : cands) {
     h->block = b;
+    h->lds_levels = mm;
     st = launch_step(h, dt, 0, h->n_wet);  // warm-up
     if (st) break;
     (void)hipEventRecord(e0, h->stream);
@@ -805,6 +844,7 @@ int rdb_colphys_autotune(rdb_colphys* h, double dt) {
     if (ms < best) {
       best = ms;
       best_b = b;
+      best_m = mm;
     }
     save(true);
   }
@@ -813,6 +853,7 @@ int rdb_colphys_autotune(rdb_colphys* h, double dt) {
   (void)hipEventDestroy(e1);
   (void)hipFree(bak);
   h->block = best_b;
+  h->lds_levels = best_m;
   if (st) return st;
   return RDB_COLPHYS_OK;
 }
@@ -878,7 +919,15 @@ int rdb_colphys_stats(rdb_colphys* h, rdb_colphys_counters* out) {
   out->sum_kernel_ms = h->sum_ms;
   out->timed_launches = h->timed;
   const double L = (double)h->nz;
-  out->algorithmic_bytes_per_step = (8.0 * (11.0 * L + 3.0 * (L + 1.0)) + 44.0) * (double)h->n_wet;
+  // B_col = 8 (11 L + 3 (L + 1)) + 44 per wet column: reads h, u, v, T, S and
+  // two passives, writes T, S, passives and kv/kt/ks, plus column scalars.
+  // The step touches only live levels (bed fillers are static), so its
+  // algorithmic figure uses each column's live-level count L_c.
+  out->reference_bytes_per_step = (8.0 * (11.0 * L + 3.0 * (L + 1.0)) + 44.0) * (double)h->n_wet;
+  const double Lsum = (double)h->live_levels;
+  out->algorithmic_bytes_per_step = 8.0 * (14.0 * Lsum + 3.0 * (double)h->n_wet) + 44.0 * (double)h->n_wet;
+  out->live_levels = h->live_levels;
+  out->lds_levels = h->lds_levels;
   out->fallback_fraction = h->closure == RDB_COLPHYS_CLOSURE_PLAIN ? 1.0 : 0.0;
   out->block_threads = h->block;
   out->order_mode = h->order_mode;

@@ -31,6 +31,8 @@ struct StateView {
   double* b0;       // [P]
   int* kbl;         // [P] KPP crossing layer (1-based, 0 none)
   int* ktop;        // [P] 0-based first-live scan start
+  int* kb;          // [P] first live layer from the bed (nz: none)
+  int* fdirty;      // [P] lowest bed-filler interface holding a KPP value
 };
 
 __global__ void k_pack_layer(const double* src, double* dst, const int* col, int ncol, int P, int nlev,
@@ -51,8 +53,9 @@ __global__ void k_split_assemble(StateView s, Params pr, int p0, int p1, unsigne
 __global__ void k_surface_flux(StateView s, Params pr, double dt, int p0, int p1);
 __global__ void k_vdiff(StateView s, Params pr, double dt, int p0, int p1);
 __global__ void k_nonlocal(StateView s, double dt, int p0, int p1);
+__global__ void k_prepare(StateView s, Params pr, int p0, int p1);
 template <typename R, int NTR, bool TWO_FACTOR>
-__global__ void k_column_step(StateView s, Params pr, double dt, int p0, int p1, unsigned int* guard_count);
+__global__ void k_column_step(StateView s, Params pr, double dt, int p0, int p1, unsigned int* guard_count, int M);
 __global__ void k_totals(StateView s, int p0, int p1, unsigned long long* acc);
 
 }  // namespace colphys

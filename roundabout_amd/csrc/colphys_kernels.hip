@@ -307,47 +307,100 @@ __global__ void k_nonlocal(StateView s, double dt, int p0, int p1) {
 }
 
 // ---------------------------------------------------------------------------
+// Column preparation (run whenever h, u, v, tau or Q changed): per column the
+// first live layer from the bed, kb (= nz when no layer is live), and the
+// diffusivities of the bed-filler interfaces 1..kb-1.  A filler (h <=
+// H_VANISHED) has rho = rho0 from the EOS, so N^2 = 0 across an interface
+// between two fillers and PP81 gives the same kv/kt every step
+// (rdb_ocean_vmix.F90:1050-1072 with Ri = 0): those interfaces are written
+// here once; the step kernel only revisits them if the KPP boundary layer
+// reaches them.  Interfaces 0 (bed) and nz (surface) are the closed-boundary
+// zeros (rdb_ocean_vmix.F90:1031-1036).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void filler_iface(const StateView& s, const Params& pr, int p, int f, double& kv,
+                                             double& kt) {
+  const int P = s.P;
+  pp81_iface<double>(pr, LV(s.ub, f - 1), LV(s.vb, f - 1), LV(s.ub, f), LV(s.vb, f), LV(s.h, f - 1), LV(s.h, f),
+                     pr.rho0, pr.rho0, kv, kt);
+}
+
+__global__ void k_prepare(StateView s, Params pr, int p0, int p1) {
+  int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= p1) return;
+  const int P = s.P, nz = s.nz;
+  int kb = nz;
+  for (int k = 0; k < nz; ++k)
+    if (LV(s.h, k) > H_VANISHED) { kb = k; break; }
+  s.kb[p] = kb;
+  s.fdirty[p] = nz + 1;
+  LV(s.kv, 0) = 0.0;
+  LV(s.kt, 0) = 0.0;
+  LV(s.ks, 0) = 0.0;
+  LV(s.kv, nz) = 0.0;
+  LV(s.kt, nz) = 0.0;
+  LV(s.ks, nz) = 0.0;
+  const int fmax = (kb < nz) ? kb - 1 : nz - 1;
+  for (int f = 1; f <= fmax; ++f) {
+    double kv, kt;
+    filler_iface(s, pr, p, f, kv, kt);
+    double ks = kt;
+    LV(s.kv, f) = fmn(fmx(kv, pr.kv_bg), pr.kv_max);
+    LV(s.kt, f) = fmn(fmx(kt, pr.kt_bg), pr.kd_max);
+    LV(s.ks, f) = fmn(fmx(ks, pr.ks_bg), pr.kd_max);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Fused column step: A1-A14 for one column per lane in the order
 // EOS -> PP81 -> KPP -> split -> assemble -> surface flux -> Thomas -> non-local
 // (apply_deferred_thermo, rdb_ocean_dyn.F90:3836-3856).  Three sweeps:
 //   A  (surface -> down, boundary layer only): KPP bulk-Ri search with the
-//      EOS evaluated on the fly, then the KPP shape-function values K_k of the
+//      EOS evaluated on the fly, then the shape-function values K_k of the
 //      boundary-layer interfaces, parked in kv[] until sweep B reads them;
-//   B  (bed -> surface): EOS + PP81 + KPP max + split + clip at every
-//      interface (kv/kt/ks written once), the tracer row, the Thomas
-//      elimination (pivot-scaled super-diagonal c' to scratch, forward
-//      residuals r in place in T/S/tracers), surface deposit folded into the
-//      right-hand side of the first live layer;
-//   C  (surface -> bed): back substitution, then the non-local tendency with
-//      gamma recomputed from the same top-down running depth as the KPP pass.
-// Each value is produced by the same floating-point operations, in the same
-// order, as the staged kernels and the reference, so results are bitwise
-// equal.  TWO_FACTOR is needed only when kt_bg != ks_bg (kt != ks after the
-// clip); otherwise one factorisation of kt serves every tracer, which is the
-// identical arithmetic the reference repeats on its bitwise copy ks = kt.
+//   B  (live bed -> surface): EOS + PP81 + KPP max + split + clip at every
+//      live interface (kv/kt/ks written once), the tracer row, the Thomas
+//      elimination, the surface deposit folded into the right-hand side of
+//      the first live layer.  The pivot-scaled super-diagonal c' and the
+//      forward residuals r of the lowest M live levels stay in LDS; higher
+//      levels spill (c' to scratch, r in place in T/S/tracers);
+//   C  (surface -> live bed): back substitution and the non-local tendency,
+//      gamma recomputed from the same top-down running depth as KPP pass 2.
+// Bed fillers (k < kb) are identity rows of the reference's tridiagonal
+// (alpha = beta = 0, b = 1), refuse deposits, and close their non-local
+// faces; the reference's elimination through them leaves every filler value
+// and the first live row's pivot, c' and r unchanged, so starting the
+// elimination at kb gives the same bits.  Every other value comes from the
+// same floating-point operations in the same order as the staged kernels and
+// the reference.  TWO_FACTOR is needed only when kt_bg != ks_bg; otherwise
+// one factorisation of kt serves every tracer (the reference repeats the
+// identical arithmetic on its bitwise copy ks = kt).
 // ---------------------------------------------------------------------------
 template <typename R, int NTR, bool TWO_FACTOR>
 __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, double dt, int p0, int p1,
-                                                     unsigned int* guard_count) {
+                                                     unsigned int* guard_count, int M) {
+  extern __shared__ double lds[];
   int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= p1) return;
   const int P = s.P, nz = s.nz;
   const double HV = H_VANISHED;
   const R rho0 = R(pr.rho0);
   const R p_ref = R(pr.p_ref);
+  const int kb = s.kb[p];
+  const int B = blockDim.x, tid = threadIdx.x;
+  // LDS slot of quantity q (c't, rT, rS, r of each passive, c's) at live
+  // level m of this lane
+#define LSLOT(q, m) lds[((size_t)(q) * (size_t)M + (size_t)(m)) * (size_t)B + (size_t)tid]
 
   // ---------------- sweep A: KPP pass 1 + pass 2 values -------------------
   int k_sfc = nz - 1;
-  for (int k = nz - 1; k >= 0; --k)
+  for (int k = nz - 1; k >= kb; --k)
     if (LV(s.h, k) > HV) { k_sfc = k; break; }
 
   const double tau_mag = s.tau_tot[p];
   const double qh = s.Qh[p], qsv = s.Qs[p];
   int k_lo = nz + 1;  // lowest interface holding a KPP value (none: > nz)
-  R qT_r = R(0), qS_r = R(0), h_b_r = R(0), w_s_r = R(0);
+  R qT_r = R(0), qS_r = R(0), h_b_r = R(0);
   bool destab = false, kpp_on = false;
-  double h_b_out = s.bl[p];
-  int kcross = 0;
   if (pr.use_kpp) {
     const R u_ref = R(LV(s.ub, k_sfc)), v_ref = R(LV(s.vb, k_sfc));
     double h_sfc = LV(s.h, k_sfc);
@@ -364,13 +417,14 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
     const R vt2_coef = R(pr.c_vt2) / R(pr.ri_crit);
     R ri_prev = R(0), d_prev = d_centre_ref, h_b = R(0), d_centre_k = d_centre_ref;
     R h_up = R(h_sfc);
+    int kcross = 0;
     for (int k = k_sfc - 1; k >= 0; --k) {
       double hk_d = LV(s.h, k);
       R hk = R(hk_d);
       d_centre_k = d_centre_k + R(0.5) * (h_up + hk);
       h_up = hk;
       R u_k = R(LV(s.ub, k)), v_k = R(LV(s.vb, k));
-      R rho_k = eos_wright<R>(R(LV(s.T, k)), R(LV(s.S, k)), hk_d, rho0, p_ref);
+      R rho_k = (hk_d > HV) ? eos_wright<R>(R(LV(s.T, k)), R(LV(s.S, k)), hk_d, rho0, p_ref) : rho0;
       R b_k = -(R(GRAVITY) * rho_k / rho0);
       R delta_d = d_centre_k - d_centre_ref;
       R delta_b = b_ref - b_k;
@@ -393,7 +447,6 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
       h_b = R(0);
       for (int k = 0; k <= k_sfc; ++k) h_b = h_b + R(LV(s.h, k));
     }
-    h_b_out = (double)h_b;
     s.b0[p] = (double)B0;
     destab = (B0 < R(0));
     R wstar3 = fmx(R(0), -B0) * h_b;
@@ -401,7 +454,6 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
     R w_s = sqrt(u_star * u_star + w_star * w_star);
     kpp_on = (h_b > R(0) && w_s > R(0));
     h_b_r = h_b;
-    w_s_r = w_s;
     if (kpp_on) {
       R d_running = R(0);
       for (int k = k_sfc; k >= 1; --k) {
@@ -410,16 +462,45 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
         R sigma = d_running / h_b;
         R g_shape = sigma * (R(1) - sigma) * (R(1) - sigma);
         R kv_kpp = h_b * w_s * g_shape;
-        LV(s.kv, k) = (double)kv_kpp;  // parked until sweep B
         k_lo = k;
+        if (k >= kb) {
+          LV(s.kv, k) = (double)kv_kpp;  // parked until sweep B
+        } else {
+          // bed-filler interface inside the boundary layer (rare): final value
+          double kv, kt;
+          filler_iface(s, pr, p, k, kv, kt);
+          double kk = (double)kv_kpp;
+          kv = fmx(kv, kk);
+          kt = fmx(kt, kk);
+          double ks = kt;
+          LV(s.kv, k) = fmn(fmx(kv, pr.kv_bg), pr.kv_max);
+          LV(s.kt, k) = fmn(fmx(kt, pr.kt_bg), pr.kd_max);
+          LV(s.ks, k) = fmn(fmx(ks, pr.ks_bg), pr.kd_max);
+        }
       }
     }
-    s.bl[p] = h_b_out;
+    s.bl[p] = (double)h_b;
     s.kbl[p] = kcross;
   }
+  {
+    // filler interfaces the boundary layer reached last step but not now get
+    // their plain value back
+    const int fd = s.fdirty[p];
+    const int top = min(kb - 1, k_lo - 1);
+    for (int f = max(fd, 1); f <= top; ++f) {
+      double kv, kt;
+      filler_iface(s, pr, p, f, kv, kt);
+      double ks = kt;
+      LV(s.kv, f) = fmn(fmx(kv, pr.kv_bg), pr.kv_max);
+      LV(s.kt, f) = fmn(fmx(kt, pr.kt_bg), pr.kd_max);
+      LV(s.ks, f) = fmn(fmx(ks, pr.ks_bg), pr.kd_max);
+    }
+    int nd = (k_lo < kb) ? k_lo : nz + 1;
+    if (nd != fd) s.fdirty[p] = nd;
+  }
+  if (kb >= nz) return;  // no live layer: nothing to mix or diffuse
 
   // ---------------- sweep B: coefficients + Thomas elimination ------------
-  // first live layer for the surface deposit, scanning down from k_top
   int k_fl = s.ktop[p];
   for (int k = k_fl; k >= 0; --k)
     if (LV(s.h, k) > HV) { k_fl = k; break; }
@@ -429,25 +510,49 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
 
   double* const trb = s.tr;
   const size_t tstride = (size_t)nz * (size_t)P;
+  unsigned int bad = 0;
 
-  // rolling state of the layer below (m1), current (0) and above (p1)
-  double h0 = LV(s.h, 0);
-  double T0 = LV(s.T, 0), S0 = LV(s.S, 0);
-  R ub0 = R(LV(s.ub, 0)), vb0 = R(LV(s.vb, 0));
+  // coefficients of interface kf between layers kf-1 (below) and kf (above)
+  auto iface = [&](int kf, R ub_lo, R vb_lo, R ub_hi, R vb_hi, double h_lo, double h_hi, R rho_lo, R rho_hi,
+                   double& kt_out, double& ks_out) {
+    R kv_r, kt_r;
+    pp81_iface<R>(pr, ub_lo, vb_lo, ub_hi, vb_hi, R(h_lo), R(h_hi), rho_lo, rho_hi, kv_r, kt_r);
+    double kvv = (double)kv_r, ktv = (double)kt_r;
+    if (kf >= k_lo && kf <= k_sfc) {
+      double kk = LV(s.kv, kf);  // KPP value parked by sweep A
+      kvv = fmx(kvv, kk);
+      ktv = fmx(ktv, kk);
+    }
+    if (pr.guard) {
+      if (isnan(kvv) || isnan(ktv) || kvv < 0.0 || ktv < 0.0) ++bad;
+    }
+    double ksv = ktv;
+    kvv = fmn(fmx(kvv, pr.kv_bg), pr.kv_max);
+    ktv = fmn(fmx(ktv, pr.kt_bg), pr.kd_max);
+    ksv = fmn(fmx(ksv, pr.ks_bg), pr.kd_max);
+    LV(s.kv, kf) = kvv;
+    LV(s.kt, kf) = ktv;
+    LV(s.ks, kf) = ksv;
+    kt_out = ktv;
+    ks_out = ksv;
+  };
+
+  double h0 = LV(s.h, kb);
+  double T0 = LV(s.T, kb), S0 = LV(s.S, kb);
+  R ub0 = R(LV(s.ub, kb)), vb0 = R(LV(s.vb, kb));
   R rho_0 = eos_wright<R>(R(T0), R(S0), h0, rho0, p_ref);
   double hm1 = 0.0;
-  double Kt_below = 0.0, Ks_below = 0.0;  // interface 0 (bed)
-  LV(s.kv, 0) = 0.0;
-  LV(s.kt, 0) = 0.0;
-  LV(s.ks, 0) = 0.0;
-  double cp_t = 0.0, cp_s = 0.0;  // c' of the row below
+  double Kt_below = 0.0, Ks_below = 0.0;  // interface kb
+  if (kb >= 1) {
+    hm1 = LV(s.h, kb - 1);
+    iface(kb, R(LV(s.ub, kb - 1)), R(LV(s.vb, kb - 1)), ub0, vb0, hm1, h0, rho0, rho_0, Kt_below, Ks_below);
+  }
+  double cp_t = 0.0, cp_s = 0.0;
   double rT = 0.0, rS = 0.0, rtr[NTR > 0 ? NTR : 1];
 #pragma unroll
   for (int t = 0; t < NTR; ++t) rtr[t] = 0.0;
-  unsigned int bad = 0;
 
-  for (int k = 0; k < nz; ++k) {
-    // interface k+1 (top of layer k): coefficients
+  for (int k = kb; k < nz; ++k) {
     double h1 = 0.0, T1 = 0.0, S1 = 0.0;
     R ub1 = R(0), vb1 = R(0), rho_1 = R(0);
     double Kt_above = 0.0, Ks_above = 0.0;
@@ -458,37 +563,12 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
       ub1 = R(LV(s.ub, k + 1));
       vb1 = R(LV(s.vb, k + 1));
       rho_1 = eos_wright<R>(R(T1), R(S1), h1, rho0, p_ref);
-      R kv_r, kt_r;
-      pp81_iface<R>(pr, ub0, vb0, ub1, vb1, R(h0), R(h1), rho_0, rho_1, kv_r, kt_r);
-      double kvv = (double)kv_r, ktv = (double)kt_r;
-      if (k + 1 >= k_lo && k + 1 <= k_sfc) {
-        double kk = LV(s.kv, k + 1);  // KPP value parked by sweep A
-        kvv = fmx(kvv, kk);
-        ktv = fmx(ktv, kk);
-      }
-      if (pr.guard) {
-        if (isnan(kvv) || isnan(ktv) || kvv < 0.0 || ktv < 0.0) ++bad;
-      }
-      double ksv = ktv;
-      kvv = fmn(fmx(kvv, pr.kv_bg), pr.kv_max);
-      ktv = fmn(fmx(ktv, pr.kt_bg), pr.kd_max);
-      ksv = fmn(fmx(ksv, pr.ks_bg), pr.kd_max);
-      LV(s.kv, k + 1) = kvv;
-      LV(s.kt, k + 1) = ktv;
-      LV(s.ks, k + 1) = ksv;
-      Kt_above = ktv;
-      Ks_above = ksv;
-    } else {
-      LV(s.kv, nz) = 0.0;
-      LV(s.kt, nz) = 0.0;
-      LV(s.ks, nz) = 0.0;
+      iface(k + 1, ub0, vb0, ub1, vb1, h0, h1, rho_0, rho_1, Kt_above, Ks_above);
     }
-
-    // Thomas row k and elimination
     double a, b, c;
     tracer_row(k, nz, dt, pr.use_harmonic, hm1, h0, h1, Kt_below, Kt_above, a, b, c);
     double piv_t;
-    if (k == 0) {
+    if (k == kb) {
       cp_t = c / b;
       piv_t = b;
     } else {
@@ -499,41 +579,47 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
     if (TWO_FACTOR) {
       double bs, cs;
       tracer_row(k, nz, dt, pr.use_harmonic, hm1, h0, h1, Ks_below, Ks_above, a_s, bs, cs);
-      if (k == 0) {
+      if (k == kb) {
         cp_s = cs / bs;
         piv_s = bs;
       } else {
         piv_s = bs - a_s * cp_s;
         cp_s = cs / piv_s;
       }
-      LV(s.fb, k) = cp_s;
     }
-    LV(s.fc, k) = cp_t;
-
-    // right-hand sides (surface deposit on the first live layer)
     double xT = T0, xS = S0;
     if (k == k_fl) {
       if (pr.has_heat) xT = conc_add_source(xT, h0, cellT);
       if (pr.has_salt) xS = conc_add_source(xS, h0, cellS);
     }
-    if (k == 0) {
+    if (k == kb) {
       rT = xT / piv_t;
       rS = xS / piv_s;
     } else {
       rT = (xT - a * rT) / piv_t;
       rS = (xS - a_s * rS) / piv_s;
     }
-    LV(s.T, k) = rT;
-    LV(s.S, k) = rS;
 #pragma unroll
     for (int t = 0; t < NTR; ++t) {
-      double* tr = trb + (size_t)t * tstride;
-      double x = LV(tr, k);
-      rtr[t] = (k == 0) ? x / piv_s : (x - a_s * rtr[t]) / piv_s;
-      LV(tr, k) = rtr[t];
+      double x = LV(trb + (size_t)t * tstride, k);
+      rtr[t] = (k == kb) ? x / piv_s : (x - a_s * rtr[t]) / piv_s;
     }
-
-    // roll
+    const int m = k - kb;
+    if (m < M) {
+      LSLOT(0, m) = cp_t;
+      LSLOT(1, m) = rT;
+      LSLOT(2, m) = rS;
+#pragma unroll
+      for (int t = 0; t < NTR; ++t) LSLOT(3 + t, m) = rtr[t];
+      if (TWO_FACTOR) LSLOT(3 + NTR, m) = cp_s;
+    } else {
+      LV(s.fc, k) = cp_t;
+      if (TWO_FACTOR) LV(s.fb, k) = cp_s;
+      LV(s.T, k) = rT;
+      LV(s.S, k) = rS;
+#pragma unroll
+      for (int t = 0; t < NTR; ++t) LV(trb + (size_t)t * tstride, k) = rtr[t];
+    }
     hm1 = h0;
     h0 = h1;
     T0 = T1;
@@ -546,65 +632,79 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
   }
 
   // ---------------- sweep C: back substitution + non-local ----------------
-  // gamma at interfaces, recomputed top-down exactly as the KPP pass 2 did.
+  const bool nl_on = pr.use_kpp && kpp_on && destab;
+  const int nl_lo = k_lo - 1;  // lowest layer touched by a non-zero gamma
   double xT_up = 0.0, xS_up = 0.0, xtr_up[NTR > 0 ? NTR : 1];
+#pragma unroll
+  for (int t = 0; t < NTR; ++t) xtr_up[t] = 0.0;
   double g_t_up = 0.0, g_s_up = 0.0;  // gamma at interface k+1
   R d_running = R(0);
   double h_up = 0.0;
-  double hk = LV(s.h, nz - 1);
-  const double dtv = dt;
-  for (int k = nz - 1; k >= 0; --k) {
-    double hdn = (k > 0) ? LV(s.h, k - 1) : 0.0;
-    double cpt = LV(s.fc, k);
-    double cps = TWO_FACTOR ? LV(s.fb, k) : cpt;
-    double xT = LV(s.T, k), xS = LV(s.S, k);
+  for (int k = nz - 1; k >= kb; --k) {
+    const int m = k - kb;
+    double cpt, cps, xT, xS;
+    double xt[NTR > 0 ? NTR : 1];
+    if (m < M) {
+      cpt = LSLOT(0, m);
+      xT = LSLOT(1, m);
+      xS = LSLOT(2, m);
+#pragma unroll
+      for (int t = 0; t < NTR; ++t) xt[t] = LSLOT(3 + t, m);
+      cps = TWO_FACTOR ? LSLOT(3 + NTR, m) : cpt;
+    } else {
+      cpt = LV(s.fc, k);
+      cps = TWO_FACTOR ? LV(s.fb, k) : cpt;
+      xT = LV(s.T, k);
+      xS = LV(s.S, k);
+#pragma unroll
+      for (int t = 0; t < NTR; ++t) xt[t] = LV(trb + (size_t)t * tstride, k);
+    }
     if (k < nz - 1) {
       xT = xT - cpt * xT_up;
       xS = xS - cps * xS_up;
+#pragma unroll
+      for (int t = 0; t < NTR; ++t) xt[t] = xt[t] - cps * xtr_up[t];
     }
 #pragma unroll
     for (int t = 0; t < NTR; ++t) {
-      double* tr = trb + (size_t)t * tstride;
-      double x = LV(tr, k);
-      if (k < nz - 1) x = x - cps * xtr_up[t];
-      xtr_up[t] = x;
-      LV(tr, k) = x;
+      xtr_up[t] = xt[t];
+      LV(trb + (size_t)t * tstride, k) = xt[t];
     }
     xT_up = xT;
     xS_up = xS;
-    // gamma at interface k (bottom of layer k)
-    double g_t_dn = 0.0, g_s_dn = 0.0;
-    if (kpp_on && k <= k_sfc) {
+    if (nl_on && k <= k_sfc && k >= nl_lo) {
+      // gamma at interface k (bottom of layer k), KPP pass 2 arithmetic
+      double hk = LV(s.h, k);
+      double g_t_dn = 0.0, g_s_dn = 0.0;
       d_running = d_running + R(hk);
-      if (k >= 1 && d_running < h_b_r && destab) {
+      if (k >= 1 && d_running < h_b_r) {
         R sigma = d_running / h_b_r;
         R g_shape = sigma * (R(1) - sigma) * (R(1) - sigma);
         R gf = R(pr.cs_nonlocal) * g_shape;
         g_t_dn = (double)(gf * qT_r);
         g_s_dn = (double)(gf * qS_r);
       }
-    }
-    if (pr.use_kpp) {
       double f_up_t = 0.0, f_dn_t = 0.0, f_up_s = 0.0, f_dn_s = 0.0;
       if (k < nz - 1) {
         f_up_t = nl_face(g_t_up, hk, h_up);
         f_up_s = nl_face(g_s_up, hk, h_up);
       }
       if (k > 0) {
+        double hdn = LV(s.h, k - 1);
         f_dn_t = nl_face(g_t_dn, hdn, hk);
         f_dn_s = nl_face(g_s_dn, hdn, hk);
       }
-      xT = conc_add_source(xT, hk, dtv * (f_up_t - f_dn_t));
-      xS = conc_add_source(xS, hk, dtv * (f_up_s - f_dn_s));
+      xT = conc_add_source(xT, hk, dt * (f_up_t - f_dn_t));
+      xS = conc_add_source(xS, hk, dt * (f_up_s - f_dn_s));
+      g_t_up = g_t_dn;
+      g_s_up = g_s_dn;
+      h_up = hk;
     }
     LV(s.T, k) = xT;
     LV(s.S, k) = xS;
-    g_t_up = g_t_dn;
-    g_s_up = g_s_dn;
-    h_up = hk;
-    hk = hdn;
   }
   if (pr.guard && bad) atomicAdd(guard_count, bad);
+#undef LSLOT
 }
 
 // ---------------------------------------------------------------------------
@@ -646,7 +746,8 @@ __global__ void k_totals(StateView s, int p0, int p1, unsigned long long* acc) {
 }
 
 // Explicit instantiations of the fused step.
-#define INST(R, N, TF) template __global__ void k_column_step<R, N, TF>(StateView, Params, double, int, int, unsigned int*);
+#define INST(R, N, TF) \
+  template __global__ void k_column_step<R, N, TF>(StateView, Params, double, int, int, unsigned int*, int);
 INST(double, 0, false) INST(double, 1, false) INST(double, 2, false) INST(double, 3, false) INST(double, 4, false)
 INST(double, 0, true) INST(double, 1, true) INST(double, 2, true) INST(double, 3, true) INST(double, 4, true)
 INST(float, 0, false) INST(float, 1, false) INST(float, 2, false) INST(float, 3, false) INST(float, 4, false)

@@ -100,9 +100,13 @@ class Counters(ctypes.Structure):
         ("sum_kernel_ms", ctypes.c_double),
         ("timed_launches", ctypes.c_longlong),
         ("algorithmic_bytes_per_step", ctypes.c_double),
+        ("reference_bytes_per_step", ctypes.c_double),
+        ("live_levels", ctypes.c_longlong),
         ("fallback_fraction", ctypes.c_double),
         ("block_threads", ctypes.c_int),
         ("order_mode", ctypes.c_int),
+        ("lds_levels", ctypes.c_int),
+        ("pad1", ctypes.c_int),
     ]
 
 
