@@ -61,6 +61,38 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+FORTRAN_DIR = PKG / "fortran"
+FORTRAN_CHECK = FORTRAN_DIR / "colphys_fortran_check"
+
+
+def flang() -> str | None:
+    for cand in (os.environ.get("FC"), "/opt/rocm/llvm/bin/flang", shutil.which("flang")):
+        if cand and Path(cand).exists():
+            return cand
+    return None
+
+
+def build_fortran(verbose: bool = False) -> Path | None:
+    """Compile the ISO_C_BINDING host module and its check program (AMD flang)."""
+    fc = flang()
+    if fc is None:
+        return None
+    mod_dir = LIB_DIR / "fortran_mod"
+    mod_dir.mkdir(parents=True, exist_ok=True)
+    obj = mod_dir / "rdb_colphys.o"
+    cmds = [
+        [fc, "-O2", f"-J{mod_dir}", "-c", str(FORTRAN_DIR / "rdb_colphys.F90"), "-o", str(obj)],
+        [fc, "-O2", f"-J{mod_dir}", str(FORTRAN_DIR / "colphys_fortran_check.F90"), str(obj),
+         f"-L{LIB_DIR}", "-lrdb_colphys", "-Wl,-rpath,$ORIGIN/../lib", "-o", str(FORTRAN_CHECK)],
+    ]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    return FORTRAN_CHECK
+
+
 if __name__ == "__main__":
     build(verbose="-v" in sys.argv)
+    build_fortran(verbose="-v" in sys.argv)
     print(LIB)

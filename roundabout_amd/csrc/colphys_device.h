@@ -5,7 +5,10 @@
 // order and the library is compiled with -ffp-contract=off, so in FP64 each
 // result is bitwise the value the reference's contraction-off CPU build
 // computes.  The assembly functions are templates on the arithmetic type R
-// so that the split-precision mode can run them in FP32.
+// so that the split-precision mode can run them in FP32; the EOS and the
+// density/buoyancy differences that feed N^2 and the bulk Richardson number
+// stay FP64 in every mode (single-precision densities of ~1027 kg/m^3 would
+// cancel in those differences).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -63,8 +66,8 @@ __device__ __forceinline__ double cbrt_kahan(double x) {
   return (bits >> 63) ? -y : y;
 }
 
-// FP32 cube root for the split-precision assembly: same method, one Halley
-// step suffices for single precision.
+// FP32 cube root for the split-precision assembly: same method and seed, two
+// Halley steps.
 __device__ __forceinline__ float cbrt_kahan(float x) {
   uint32_t bits = (uint32_t)__float_as_int(x);
   uint32_t bexp = (bits >> 23) & 0xffu;
@@ -113,9 +116,13 @@ __device__ __forceinline__ R eos_wright(R T, R S, double h, R rho0, R p_ref) {
 // A3 PP81 at one interior interface between layers km1 (below) and k (above),
 // from centred velocities (rdb_ocean_vmix.F90:1043-1072).
 // ---------------------------------------------------------------------------
+// The density difference is formed in FP64 and handed over as drho =
+// rho_k - rho_km1: the first operation of the reference's N^2 expression, so
+// the FP64 instantiation is bitwise the reference; the FP32 instantiation
+// avoids cancelling two ~1027 kg/m^3 single-precision densities.
 template <typename R>
 __device__ __forceinline__ void pp81_iface(const Params& pr, R u_km1, R v_km1, R u_k, R v_k,
-                                           R h_km1, R h_k, R rho_km1, R rho_k, R& kv, R& kt) {
+                                           R h_km1, R h_k, double drho, R& kv, R& kt) {
   R dz_face = R(0.5) * (h_km1 + h_k);
   if (dz_face <= R(0)) {
     kv = R(pr.nu_bg);
@@ -125,7 +132,7 @@ __device__ __forceinline__ void pp81_iface(const Params& pr, R u_km1, R v_km1, R
   R du_dz = (u_k - u_km1) / dz_face;
   R dv_dz = (v_k - v_km1) / dz_face;
   R shear2 = fmx(du_dz * du_dz + dv_dz * dv_dz, R(pr.shear2_floor));
-  R n2 = -(R(GRAVITY) * (rho_k - rho_km1) / (R(pr.rho0) * dz_face));
+  R n2 = -(R(GRAVITY) * R(drho) / (R(pr.rho0) * dz_face));
   R ri = n2 / shear2;
   R denom = R(1) + R(pr.pp_alpha) * fmx(ri, R(0));
   R ri_factor = R(1) / (denom * denom);

@@ -100,7 +100,7 @@ __global__ void k_pp81(StateView s, Params pr, int p0, int p1) {
   for (int k = 1; k < nz; ++k) {
     double ub1 = LV(s.ub, k), vb1 = LV(s.vb, k), h1 = LV(s.h, k), r1 = LV(s.rho, k);
     double kv, kt;
-    pp81_iface<double>(pr, ub0, vb0, ub1, vb1, h0, h1, r0, r1, kv, kt);
+    pp81_iface<double>(pr, ub0, vb0, ub1, vb1, h0, h1, r1 - r0, kv, kt);
     LV(s.kv, k) = kv;
     LV(s.kt, k) = kt;
     ub0 = ub1; vb0 = vb1; h0 = h1; r0 = r1;
@@ -321,7 +321,7 @@ __device__ __forceinline__ void filler_iface(const StateView& s, const Params& p
                                              double& kt) {
   const int P = s.P;
   pp81_iface<double>(pr, LV(s.ub, f - 1), LV(s.vb, f - 1), LV(s.ub, f), LV(s.vb, f), LV(s.h, f - 1), LV(s.h, f),
-                     pr.rho0, pr.rho0, kv, kt);
+                     pr.rho0 - pr.rho0, kv, kt);
 }
 
 __global__ void k_prepare(StateView s, Params pr, int p0, int p1) {
@@ -384,7 +384,6 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
   const int P = s.P, nz = s.nz;
   const double HV = H_VANISHED;
   const R rho0 = R(pr.rho0);
-  const R p_ref = R(pr.p_ref);
   const int kb = s.kb[p];
   const int B = blockDim.x, tid = threadIdx.x;
   // LDS slot of quantity q (c't, rT, rS, r of each passive, c's) at live
@@ -404,8 +403,9 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
   if (pr.use_kpp) {
     const R u_ref = R(LV(s.ub, k_sfc)), v_ref = R(LV(s.vb, k_sfc));
     double h_sfc = LV(s.h, k_sfc);
-    R rho_sfc = eos_wright<R>(R(LV(s.T, k_sfc)), R(LV(s.S, k_sfc)), h_sfc, rho0, p_ref);
-    const R b_ref = -(R(GRAVITY) * rho_sfc / rho0);
+    // EOS and the buoyancy difference stay FP64 in every precision mode
+    const double rho_sfc = eos_wright<double>(LV(s.T, k_sfc), LV(s.S, k_sfc), h_sfc, pr.rho0, pr.p_ref);
+    const double b_ref = -(GRAVITY * rho_sfc / pr.rho0);
     const R d_centre_ref = R(0.5) * R(h_sfc);
     R u_star = sqrt(R(tau_mag) / rho0);
     qT_r = R(qh) / (rho0 * R(pr.cp));
@@ -424,10 +424,10 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
       d_centre_k = d_centre_k + R(0.5) * (h_up + hk);
       h_up = hk;
       R u_k = R(LV(s.ub, k)), v_k = R(LV(s.vb, k));
-      R rho_k = (hk_d > HV) ? eos_wright<R>(R(LV(s.T, k)), R(LV(s.S, k)), hk_d, rho0, p_ref) : rho0;
-      R b_k = -(R(GRAVITY) * rho_k / rho0);
+      double rho_k = (hk_d > HV) ? eos_wright<double>(LV(s.T, k), LV(s.S, k), hk_d, pr.rho0, pr.p_ref) : pr.rho0;
+      double b_k = -(GRAVITY * rho_k / pr.rho0);
       R delta_d = d_centre_k - d_centre_ref;
-      R delta_b = b_ref - b_k;
+      R delta_b = R(b_ref - b_k);
       R n_brunt = sqrt(fmx(R(0), delta_b / delta_d));
       R vt2 = vt2_coef * d_centre_k * n_brunt * w_s_col;
       R du = u_k - u_ref, dv = v_k - v_ref;
@@ -513,10 +513,10 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
   unsigned int bad = 0;
 
   // coefficients of interface kf between layers kf-1 (below) and kf (above)
-  auto iface = [&](int kf, R ub_lo, R vb_lo, R ub_hi, R vb_hi, double h_lo, double h_hi, R rho_lo, R rho_hi,
-                   double& kt_out, double& ks_out) {
+  auto iface = [&](int kf, R ub_lo, R vb_lo, R ub_hi, R vb_hi, double h_lo, double h_hi, double rho_lo,
+                   double rho_hi, double& kt_out, double& ks_out) {
     R kv_r, kt_r;
-    pp81_iface<R>(pr, ub_lo, vb_lo, ub_hi, vb_hi, R(h_lo), R(h_hi), rho_lo, rho_hi, kv_r, kt_r);
+    pp81_iface<R>(pr, ub_lo, vb_lo, ub_hi, vb_hi, R(h_lo), R(h_hi), rho_hi - rho_lo, kv_r, kt_r);
     double kvv = (double)kv_r, ktv = (double)kt_r;
     if (kf >= k_lo && kf <= k_sfc) {
       double kk = LV(s.kv, kf);  // KPP value parked by sweep A
@@ -540,12 +540,12 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
   double h0 = LV(s.h, kb);
   double T0 = LV(s.T, kb), S0 = LV(s.S, kb);
   R ub0 = R(LV(s.ub, kb)), vb0 = R(LV(s.vb, kb));
-  R rho_0 = eos_wright<R>(R(T0), R(S0), h0, rho0, p_ref);
+  double rho_0 = eos_wright<double>(T0, S0, h0, pr.rho0, pr.p_ref);
   double hm1 = 0.0;
   double Kt_below = 0.0, Ks_below = 0.0;  // interface kb
   if (kb >= 1) {
     hm1 = LV(s.h, kb - 1);
-    iface(kb, R(LV(s.ub, kb - 1)), R(LV(s.vb, kb - 1)), ub0, vb0, hm1, h0, rho0, rho_0, Kt_below, Ks_below);
+    iface(kb, R(LV(s.ub, kb - 1)), R(LV(s.vb, kb - 1)), ub0, vb0, hm1, h0, pr.rho0, rho_0, Kt_below, Ks_below);
   }
   double cp_t = 0.0, cp_s = 0.0;
   double rT = 0.0, rS = 0.0, rtr[NTR > 0 ? NTR : 1];
@@ -554,7 +554,8 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
 
   for (int k = kb; k < nz; ++k) {
     double h1 = 0.0, T1 = 0.0, S1 = 0.0;
-    R ub1 = R(0), vb1 = R(0), rho_1 = R(0);
+    R ub1 = R(0), vb1 = R(0);
+    double rho_1 = 0.0;
     double Kt_above = 0.0, Ks_above = 0.0;
     if (k + 1 < nz) {
       h1 = LV(s.h, k + 1);
@@ -562,7 +563,7 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
       S1 = LV(s.S, k + 1);
       ub1 = R(LV(s.ub, k + 1));
       vb1 = R(LV(s.vb, k + 1));
-      rho_1 = eos_wright<R>(R(T1), R(S1), h1, rho0, p_ref);
+      rho_1 = eos_wright<double>(T1, S1, h1, pr.rho0, pr.p_ref);
       iface(k + 1, ub0, vb0, ub1, vb1, h0, h1, rho_0, rho_1, Kt_above, Ks_above);
     }
     double a, b, c;

@@ -1,0 +1,122 @@
+!> Fortran host check of the column-physics C ABI.
+!>   colphys_fortran_check cpu  -- ABI reachable without a GPU: defaults, error
+!>                                 ring, refused knobs, NULL-safe destroy
+!>   colphys_fortran_check gpu  -- a 16 x 12 x 8 ocean stepped 3 times; prints
+!>                                 the outputs' checksums and a parity line
+!>                                 against the staged entry points
+program colphys_fortran_check
+   use, intrinsic :: iso_c_binding
+   use rdb_colphys
+   implicit none
+   type(rdb_colphys_config_t) :: cfg
+   type(c_ptr) :: h
+   integer(c_int) :: st
+   character(len=8) :: mode
+   integer, parameter :: nx = 16, ny = 12, nz = 8
+   real(c_double) :: wet(nx, ny), hl(nx, ny, nz), T(nx, ny, nz), S(nx, ny, nz)
+   real(c_double) :: u(nx + 1, ny, nz), v(nx, ny + 1, nz), tx(nx + 1, ny, 1), ty(nx, ny + 1, 1)
+   real(c_double) :: qh(nx, ny, 1), qs(nx, ny, 1), T1(nx, ny, nz), T2(nx, ny, nz), kt1(nx, ny, nz + 1)
+   real(c_double) :: kt2(nx, ny, nz + 1), heat, salt
+   integer :: i, j, k
+
+   call get_command_argument(1, mode)
+   st = rdb_colphys_config_default(cfg)
+   if (st /= 0 .or. cfg%rho0 /= 1035.0_c_double .or. cfg%pp81_nu0 /= 1.0e-2_c_double) stop 2
+   if (rdb_colphys_working_precision() /= 8) stop 3
+   h = c_null_ptr
+   st = rdb_colphys_destroy(h)
+   if (st /= 0) stop 4
+
+   wet = 1.0_c_double
+   wet(1, 1) = 0.0_c_double
+   do k = 1, nz
+      do j = 1, ny
+         do i = 1, nx
+            hl(i, j, k) = 25.0_c_double + real(mod(i + j, 3), c_double)
+            T(i, j, k) = 4.0_c_double + 0.5_c_double*real(k, c_double) + 0.01_c_double*real(i - j, c_double)
+            S(i, j, k) = 35.0_c_double - 0.01_c_double*real(k, c_double)
+         end do
+      end do
+   end do
+   hl(1, 1, :) = 1.5e-4_c_double
+   T(1, 1, :) = 0.0_c_double
+   S(1, 1, :) = 0.0_c_double
+   do k = 1, nz
+      do j = 1, ny
+         do i = 1, nx + 1
+            u(i, j, k) = 0.05_c_double*sin(real(i*k, c_double))
+         end do
+      end do
+      do j = 1, ny + 1
+         do i = 1, nx
+            v(i, j, k) = 0.05_c_double*cos(real(j*k, c_double))
+         end do
+      end do
+   end do
+   tx = 0.08_c_double
+   ty = 0.02_c_double
+   qh = -60.0_c_double
+   qs = 0.0_c_double
+
+   if (trim(mode) == "cpu") then
+      cfg%conv_enable = 1
+      st = rdb_colphys_create(cfg, nx, ny, nz, 0, wet, c_null_ptr, c_null_ptr, h)
+      if (st /= RDB_COLPHYS_ERR_CONFIG_VALIDATE .or. c_associated(h)) stop 5
+      if (index(colphys_last_error(), "conv_enable") == 0) stop 6
+      cfg%conv_enable = 0
+      st = rdb_colphys_create(cfg, nx, ny, nz, 0, wet, c_null_ptr, c_null_ptr, h)
+      if (st /= RDB_COLPHYS_ERR_SETUP .or. c_associated(h)) stop 7
+      print '(a)', "fortran-abi-cpu: ok"
+      stop
+   end if
+
+   ! gpu: fused step vs the staged entry points on two identical handles
+   st = rdb_colphys_create(cfg, nx, ny, nz, 0, wet, c_null_ptr, c_null_ptr, h)
+   if (st /= 0) then
+      print '(a)', trim(colphys_last_error())
+      stop 10
+   end if
+   call upload()
+   st = rdb_colphys_step(h, 3600.0_c_double, 3)
+   if (st /= 0) stop 11
+   st = colphys_get(h, "temperature", T1)
+   st = colphys_get(h, "kt", kt1)
+   st = rdb_colphys_heat_salt_totals(h, heat, salt)
+   st = rdb_colphys_destroy(h)
+   if (c_associated(h)) stop 12
+
+   st = rdb_colphys_create(cfg, nx, ny, nz, 0, wet, c_null_ptr, c_null_ptr, h)
+   call upload()
+   do i = 1, 3
+      st = rdb_colphys_eos(h)
+      st = rdb_colphys_coefficients(h, 3600.0_c_double)
+      st = rdb_colphys_surface_flux(h, 3600.0_c_double)
+      st = rdb_colphys_diffuse_tracers(h, 3600.0_c_double)
+      st = rdb_colphys_nonlocal(h, 3600.0_c_double)
+   end do
+   st = colphys_get(h, "temperature", T2)
+   st = colphys_get(h, "kt", kt2)
+   st = rdb_colphys_destroy(h)
+   if (any(T1 /= T2) .or. any(kt1 /= kt2)) then
+      print '(a)', "fortran-abi-gpu: fused and staged paths differ"
+      stop 13
+   end if
+   print '(a, es24.16, a, es24.16)', "fortran-abi-gpu: ok heat=", heat, " sum(T)=", sum(T1)
+
+contains
+
+   subroutine upload()
+      real(c_double) :: w3(nx, ny, 1)
+      w3(:, :, 1) = wet
+      if (colphys_set(h, "h_layer", hl) /= 0) stop 20
+      if (colphys_set(h, "temperature", T) /= 0) stop 21
+      if (colphys_set(h, "salinity", S) /= 0) stop 22
+      if (colphys_set(h, "u_face_x_layer", u) /= 0) stop 23
+      if (colphys_set(h, "v_face_y_layer", v) /= 0) stop 24
+      if (colphys_set(h, "tau_x", tx) /= 0) stop 25
+      if (colphys_set(h, "tau_y", ty) /= 0) stop 26
+      if (colphys_set(h, "Q_heat", qh) /= 0) stop 27
+      if (colphys_set(h, "Q_salt", qs) /= 0) stop 28
+   end subroutine upload
+
+end program colphys_fortran_check

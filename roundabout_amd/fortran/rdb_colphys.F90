@@ -1,0 +1,187 @@
+!> ISO_C_BINDING interface to librdb_colphys.so (include/rdb_colphys.h).
+!>
+!> Drop-in for the vertical-mixing call site of the Roundabout dyn-core:
+!> where rdb_ocean_dyn.F90 calls
+!>   vmix_build_coefficients(...)           (rdb_ocean_dyn.F90:1473)
+!>   vdiff_apply_tracers(..., kt, ks)       (rdb_ocean_dyn.F90:1856)
+!>   vmix_apply_nonlocal_tendencies(...)    (rdb_ocean_dyn.F90:1859)
+!> a host code built against this module calls rdb_colphys_step (fused) or the
+!> staged entry points with the same state arrays.  Arrays are the reference
+!> layout: column-major (i fastest), k = 1 at the bed; passed by address.
+module rdb_colphys
+   use, intrinsic :: iso_c_binding, only: c_int, c_double, c_char, c_ptr, c_null_ptr, &
+                                          c_long_long, c_null_char, c_loc, c_associated
+   implicit none
+   private
+
+   integer(c_int), parameter, public :: RDB_COLPHYS_OK = 0
+   integer(c_int), parameter, public :: RDB_COLPHYS_ERR_CONFIG_VALIDATE = 2
+   integer(c_int), parameter, public :: RDB_COLPHYS_ERR_SETUP = 3
+   integer(c_int), parameter, public :: RDB_COLPHYS_ERR_BAD_HANDLE = 10
+   integer(c_int), parameter, public :: RDB_COLPHYS_ERR_ALREADY_EXISTS = 11
+   integer(c_int), parameter, public :: RDB_COLPHYS_ERR_NOT_FOUND = 13
+   integer(c_int), parameter, public :: RDB_COLPHYS_ERR_BAD_SHAPE = 14
+   integer(c_int), parameter, public :: RDB_COLPHYS_PREC_FP64 = 0
+   integer(c_int), parameter, public :: RDB_COLPHYS_PREC_SPLIT = 1
+
+   !> Mirror of rdb_colphys_config (field order is the ABI).
+   type, bind(c), public :: rdb_colphys_config_t
+      real(c_double) :: pp81_nu0, pp81_nu_bg, pp81_kappa_bg, pp81_alpha
+      real(c_double) :: rho0, shear2_floor
+      integer(c_int) :: use_kpp, pad0
+      real(c_double) :: ri_crit, c_vt2, cs_nonlocal
+      real(c_double) :: alpha_T, beta_S, p_ref
+      real(c_double) :: cp
+      integer(c_int) :: has_heat, has_salt
+      real(c_double) :: kv_bg, kt_bg, ks_bg, kv_max, kd_max
+      integer(c_int) :: vmix_guard
+      integer(c_int) :: conv_enable
+      real(c_double) :: kv_ml_invz2
+      integer(c_int) :: ddiff_enable, kd_smooth_iterations
+      integer(c_int) :: use_harmonic, precision
+   end type rdb_colphys_config_t
+
+   public :: rdb_colphys_config_default, rdb_colphys_working_precision
+   public :: rdb_colphys_create, rdb_colphys_destroy
+   public :: rdb_colphys_set_field, rdb_colphys_get_field
+   public :: rdb_colphys_eos, rdb_colphys_coefficients, rdb_colphys_surface_flux
+   public :: rdb_colphys_diffuse_tracers, rdb_colphys_nonlocal, rdb_colphys_step
+   public :: rdb_colphys_synchronize, rdb_colphys_autotune
+   public :: rdb_colphys_set_precision, rdb_colphys_set_closure
+   public :: rdb_colphys_heat_salt_totals, rdb_colphys_last_error
+   public :: colphys_set, colphys_get, colphys_last_error
+
+   interface
+      integer(c_int) function rdb_colphys_config_default(cfg) bind(c, name="rdb_colphys_config_default")
+         import :: c_int, rdb_colphys_config_t
+         type(rdb_colphys_config_t), intent(out) :: cfg
+      end function
+      integer(c_int) function rdb_colphys_working_precision() bind(c, name="rdb_colphys_working_precision")
+         import :: c_int
+      end function
+      integer(c_int) function rdb_colphys_create(cfg, nx, ny, nz, n_passive, wet, k_top, sort_key, handle) &
+         bind(c, name="rdb_colphys_create")
+         import :: c_int, c_double, c_ptr, rdb_colphys_config_t
+         type(rdb_colphys_config_t), intent(in) :: cfg
+         integer(c_int), value :: nx, ny, nz, n_passive
+         real(c_double), intent(in) :: wet(*)
+         type(c_ptr), value :: k_top, sort_key
+         type(c_ptr), intent(out) :: handle
+      end function
+      integer(c_int) function rdb_colphys_destroy(handle) bind(c, name="rdb_colphys_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), intent(inout) :: handle
+      end function
+      integer(c_int) function rdb_colphys_set_field(handle, name, data, n1, n2, n3) &
+         bind(c, name="rdb_colphys_set_field")
+         import :: c_int, c_double, c_char, c_ptr
+         type(c_ptr), value :: handle
+         character(kind=c_char), intent(in) :: name(*)
+         real(c_double), intent(in) :: data(*)
+         integer(c_int), value :: n1, n2, n3
+      end function
+      integer(c_int) function rdb_colphys_get_field(handle, name, out, n1, n2, n3) &
+         bind(c, name="rdb_colphys_get_field")
+         import :: c_int, c_double, c_char, c_ptr
+         type(c_ptr), value :: handle
+         character(kind=c_char), intent(in) :: name(*)
+         real(c_double), intent(out) :: out(*)
+         integer(c_int), value :: n1, n2, n3
+      end function
+      integer(c_int) function rdb_colphys_eos(handle) bind(c, name="rdb_colphys_eos")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function
+      integer(c_int) function rdb_colphys_coefficients(handle, dt) bind(c, name="rdb_colphys_coefficients")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: handle
+         real(c_double), value :: dt
+      end function
+      integer(c_int) function rdb_colphys_surface_flux(handle, dt) bind(c, name="rdb_colphys_surface_flux")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: handle
+         real(c_double), value :: dt
+      end function
+      integer(c_int) function rdb_colphys_diffuse_tracers(handle, dt) bind(c, name="rdb_colphys_diffuse_tracers")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: handle
+         real(c_double), value :: dt
+      end function
+      integer(c_int) function rdb_colphys_nonlocal(handle, dt) bind(c, name="rdb_colphys_nonlocal")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: handle
+         real(c_double), value :: dt
+      end function
+      integer(c_int) function rdb_colphys_step(handle, dt, n_steps) bind(c, name="rdb_colphys_step")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: handle
+         real(c_double), value :: dt
+         integer(c_int), value :: n_steps
+      end function
+      integer(c_int) function rdb_colphys_synchronize(handle) bind(c, name="rdb_colphys_synchronize")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function
+      integer(c_int) function rdb_colphys_autotune(handle, dt) bind(c, name="rdb_colphys_autotune")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: handle
+         real(c_double), value :: dt
+      end function
+      integer(c_int) function rdb_colphys_set_precision(handle, mode) bind(c, name="rdb_colphys_set_precision")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int), value :: mode
+      end function
+      integer(c_int) function rdb_colphys_set_closure(handle, mode) bind(c, name="rdb_colphys_set_closure")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int), value :: mode
+      end function
+      integer(c_int) function rdb_colphys_heat_salt_totals(handle, heat, salt) &
+         bind(c, name="rdb_colphys_heat_salt_totals")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: handle
+         real(c_double), intent(out) :: heat, salt
+      end function
+      integer(c_int) function rdb_colphys_last_error(idx, buf, cap) bind(c, name="rdb_colphys_last_error")
+         import :: c_int, c_char
+         integer(c_int), value :: idx
+         character(kind=c_char), intent(out) :: buf(*)
+         integer(c_int), value :: cap
+      end function
+   end interface
+
+contains
+
+   !> Upload a 3-D field (n1, n2, n3) by name.
+   integer(c_int) function colphys_set(handle, name, data) result(st)
+      type(c_ptr), intent(in) :: handle
+      character(len=*), intent(in) :: name
+      real(c_double), intent(in) :: data(:, :, :)
+      st = rdb_colphys_set_field(handle, trim(name)//c_null_char, data, &
+                                 int(size(data, 1), c_int), int(size(data, 2), c_int), int(size(data, 3), c_int))
+   end function colphys_set
+
+   !> Download a 3-D field (n1, n2, n3) by name.
+   integer(c_int) function colphys_get(handle, name, out) result(st)
+      type(c_ptr), intent(in) :: handle
+      character(len=*), intent(in) :: name
+      real(c_double), intent(out) :: out(:, :, :)
+      st = rdb_colphys_get_field(handle, trim(name)//c_null_char, out, &
+                                 int(size(out, 1), c_int), int(size(out, 2), c_int), int(size(out, 3), c_int))
+   end function colphys_get
+
+   !> Most recent error-ring message as a Fortran string.
+   function colphys_last_error() result(msg)
+      character(len=512) :: msg
+      character(kind=c_char) :: buf(512)
+      integer(c_int) :: n, i
+      buf = ' '
+      n = rdb_colphys_last_error(0_c_int, buf, 512_c_int)
+      msg = ' '
+      do i = 1, min(int(n), 511)
+         msg(i:i) = buf(i)
+      end do
+   end function colphys_last_error
+
+end module rdb_colphys

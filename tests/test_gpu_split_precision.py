@@ -1,0 +1,65 @@
+"""Split precision (FP32 assembly, FP64 tridiagonal solve and integrals)
+against the FP64 oracle: the error of every output is measured and bounded,
+and heat/salt stay conserved because the solve is FP64."""
+
+from __future__ import annotations
+
+import json
+
+import numpy as np
+import pytest
+
+from roundabout_amd import synth
+from roundabout_amd.rdb import PREC_SPLIT, ColumnPhysics, default_config
+
+pytestmark = pytest.mark.gpu
+DT = 3600.0
+
+
+def split_errors(orc, ocean, nsteps):
+    cfg = default_config()
+    ost = orc.OracleState(ocean, cfg)
+    ost.step(DT, nsteps)
+    with ColumnPhysics.from_ocean(ocean, config=cfg) as cp:
+        cp.set_precision(PREC_SPLIT)
+        h0, s0 = cp.heat_salt_totals()
+        cp.step(DT, nsteps)
+        h1, s1 = cp.heat_salt_totals()
+        got = {n: cp.get_field(n) for n in ("temperature", "salinity", "kv", "kt", "kbl")}
+    wet = ocean.wet > 0.5
+    live = (ocean.h > 1.5e-4) & wet[:, :, None]
+    li = np.zeros(ocean.kv_shape if hasattr(ocean, "kv_shape") else ost.kv.shape, dtype=bool)
+    li[:, :, 1:-1] = live[:, :, 1:] & live[:, :, :-1]
+    out = {}
+    for n, ref in (("temperature", ost.ocean.T), ("salinity", ost.ocean.S)):
+        d = (got[n] - ref)[live]
+        out[n] = {"max_abs": float(np.abs(d).max()), "rms": float(np.sqrt(np.mean(d * d)))}
+    for n, ref in (("kv", ost.kv), ("kt", ost.kt)):
+        r = np.abs(got[n] - ref)[li] / np.abs(ref)[li]
+        out[n] = {"max_rel": float(r.max()), "rms_rel": float(np.sqrt(np.mean(r * r)))}
+    out["kbl_mismatch_fraction"] = float((got["kbl"] != ost.kbl)[wet].mean())
+    dep = nsteps * DT / (cfg.rho0 * cfg.cp) * float((ocean.Q_heat * ocean.wet).sum())
+    out["heat_conservation_rel"] = abs((h1 - h0) - dep) / abs(h0)
+    out["salt_conservation_rel"] = abs(s1 - s0) / abs(s0)
+    return out
+
+
+@pytest.mark.parametrize("workload", ["tile", "thin"])
+def test_split_precision_error_budget(orc, gpu_lib, workload):
+    print(f"[case] split precision, {workload} 128x96x75 seed 12, 24 steps", flush=True)
+    ocean = synth.WORKLOADS[workload](12, nx=128, ny=96)
+    e = split_errors(orc, ocean, 24)
+    print("split-precision-errors " + json.dumps({"workload": workload, **e}), flush=True)
+    # RMS errors are single-precision sized; the maxima come from isolated
+    # near-neutral interfaces next to millimetre layers, where N^2 amplifies a
+    # 1e-7 density perturbation by 1/dz and can flip the sign of Ri.
+    if workload == "tile":
+        assert e["temperature"]["max_abs"] < 1e-4 and e["temperature"]["rms"] < 1e-6
+        assert e["salinity"]["max_abs"] < 1e-4 and e["salinity"]["rms"] < 1e-6
+        assert e["kt"]["max_rel"] < 1e-2 and e["kt"]["rms_rel"] < 1e-5
+    else:
+        assert e["temperature"]["max_abs"] < 0.1 and e["temperature"]["rms"] < 5e-4
+        assert e["salinity"]["max_abs"] < 0.02 and e["salinity"]["rms"] < 1e-4
+        assert e["kt"]["rms_rel"] < 1e-3
+    assert e["kbl_mismatch_fraction"] < 0.02
+    assert e["heat_conservation_rel"] < 1e-13 and e["salt_conservation_rel"] < 1e-13
