@@ -251,6 +251,19 @@ int rdb_colphys_heat_salt_totals(rdb_colphys *h, double *heat, double *salt);
 int rdb_colphys_set_timing(rdb_colphys *h, int enable_timing);
 int rdb_colphys_stats(rdb_colphys *h, rdb_colphys_counters *out);
 
+/* ---- learned closure: the MLP on host arrays --------------------------
+ * No counterpart in the reference: these run the BF16 MFMA network of the
+ * surrogate closure (widths[0] inputs, widths[1..n-2] GELU hidden layers,
+ * widths[n-1] linear outputs) on a host batch, so that its kernels can be
+ * checked against an FP64 CPU oracle.  params is the unpadded FP32 layout
+ * W_0 (widths[1] x widths[0], row-major), b_0, W_1, b_1, ...; x is
+ * (batch x widths[0]), y and t and w are (batch x widths[n-1]), row-major.
+ * rdb_mlp_loss_grad returns the weighted mean-squared error
+ * sum(w (y - t)^2) / sum(w) and its gradient in the params layout. */
+int rdb_mlp_eval(const int *widths, int n_widths, const float *params, int batch, const float *x, float *y);
+int rdb_mlp_loss_grad(const int *widths, int n_widths, const float *params, int batch, const float *x,
+                      const float *t, const float *w, float *loss, float *grad);
+
 /* ---- error ring (mirrors rdb_ocean_last_error, rdb_ocean.h:302-314) ---- */
 int rdb_colphys_last_error(int idx, char *buf, int cap);
 

@@ -60,6 +60,9 @@ struct FieldSpec {
 
 }  // namespace
 
+// error ring shared with the MLP entry points
+int rdb_colphys_push_error(int status, const char* msg) { return push_error(status, "%s", msg); }
+
 struct rdb_colphys {
   rdb_colphys_config cfg;
   Params pr;

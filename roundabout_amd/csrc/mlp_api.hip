@@ -1,0 +1,120 @@
+// mlp_api.hip -- stateless C entry points of the BF16 MFMA MLP (forward and
+// loss gradient on host arrays).  They have no counterpart in the reference;
+// they exist so that the network kernels can be checked against an FP64 CPU
+// oracle on small batches.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "../../include/rdb_colphys.h"
+#include "mlp_net.h"
+
+extern int rdb_colphys_push_error(int status, const char* msg);
+
+namespace {
+
+int build_net(mlp::Net& net, const int* widths, int n_widths, const float* params, hipStream_t s) {
+  if (!widths || n_widths < 2 || n_widths > 16 || !params)
+    return rdb_colphys_push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "mlp: bad widths or NULL params");
+  std::vector<int> dims(widths, widths + n_widths);
+  int st = net.init(dims, s);
+  if (st) return rdb_colphys_push_error(st, net.err.c_str());
+  st = net.set_params(params);
+  if (st) return rdb_colphys_push_error(st, net.err.c_str());
+  return 0;
+}
+
+int upload_x(mlp::Net& net, int batch, int Mpad, const float* x) {
+  const int in = net.din[0], ld = net.pin[0];
+  std::vector<uint16_t> xb((size_t)Mpad * ld, 0);
+  for (int m = 0; m < batch; ++m)
+    for (int c = 0; c < in; ++c) {
+      float f = x[(size_t)m * in + c];
+      uint32_t u;
+      std::memcpy(&u, &f, 4);
+      u += 0x7fffu + ((u >> 16) & 1u);
+      xb[(size_t)m * ld + c] = (uint16_t)(u >> 16);
+    }
+  if (hipMemcpyAsync(net.X0, xb.data(), 2 * xb.size(), hipMemcpyHostToDevice, net.stream) != hipSuccess)
+    return rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, "mlp: input upload failed");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rdb_mlp_eval(const int* widths, int n_widths, const float* params, int batch, const float* x, float* y) {
+  if (batch < 1 || !x || !y) return rdb_colphys_push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "mlp_eval: bad batch or NULL");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, "mlp_eval: no HIP device available");
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+    return rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, "mlp_eval: stream create failed");
+  int st;
+  {
+    mlp::Net net;
+    st = build_net(net, widths, n_widths, params, s);
+    const int Mpad = mlp::round_up(batch, 64);
+    if (!st && net.ensure_workspace(Mpad, false)) st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, net.err.c_str());
+    if (!st) st = upload_x(net, batch, Mpad, x);
+    if (!st && net.forward(Mpad, false)) st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, net.err.c_str());
+    if (!st) {
+      const int out = net.dout[net.nl - 1], po = net.pout[net.nl - 1];
+      std::vector<float> yp((size_t)Mpad * po);
+      if (hipMemcpyAsync(yp.data(), net.Y, 4 * yp.size(), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, "mlp_eval: output download failed");
+      else
+        for (int m = 0; m < batch; ++m) std::memcpy(y + (size_t)m * out, yp.data() + (size_t)m * po, 4 * (size_t)out);
+    }
+    (void)hipStreamSynchronize(s);
+  }
+  (void)hipStreamDestroy(s);
+  return st;
+}
+
+int rdb_mlp_loss_grad(const int* widths, int n_widths, const float* params, int batch, const float* x,
+                      const float* t, const float* w, float* loss, float* grad) {
+  if (batch < 1 || !x || !t || !w || !loss || !grad)
+    return rdb_colphys_push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "mlp_loss_grad: bad batch or NULL");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, "mlp_loss_grad: no HIP device available");
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+    return rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, "mlp_loss_grad: stream create failed");
+  int st;
+  {
+    mlp::Net net;
+    st = build_net(net, widths, n_widths, params, s);
+    const int Mpad = mlp::round_up(batch, 64);
+    if (!st && net.ensure_workspace(Mpad, true)) st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, net.err.c_str());
+    if (!st) st = upload_x(net, batch, Mpad, x);
+    if (!st) {
+      const int out = net.dout[net.nl - 1], po = net.pout[net.nl - 1];
+      std::vector<float> tp((size_t)Mpad * po, 0.f), wp((size_t)Mpad * po, 0.f);
+      double count = 0.0;
+      for (int m = 0; m < batch; ++m)
+        for (int c = 0; c < out; ++c) {
+          tp[(size_t)m * po + c] = t[(size_t)m * out + c];
+          wp[(size_t)m * po + c] = w[(size_t)m * out + c];
+          count += w[(size_t)m * out + c];
+        }
+      if (hipMemcpyAsync(net.T, tp.data(), 4 * tp.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+          hipMemcpyAsync(net.Wm, wp.data(), 4 * wp.size(), hipMemcpyHostToDevice, s) != hipSuccess)
+        st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, "mlp_loss_grad: target upload failed");
+      if (!st && net.forward(Mpad, true)) st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, net.err.c_str());
+      if (!st && net.backward(Mpad, batch, count, loss))
+        st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, net.err.c_str());
+      if (!st && net.get_grads(grad)) st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, net.err.c_str());
+    }
+    (void)hipStreamSynchronize(s);
+  }
+  (void)hipStreamDestroy(s);
+  return st;
+}
+
+}  // extern "C"

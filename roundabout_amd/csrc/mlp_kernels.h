@@ -1,0 +1,27 @@
+// mlp_kernels.h -- kernels of the learned closure (BF16 MFMA MLP).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mlp {
+
+typedef uint16_t bf16;  // raw bfloat16 bits in memory
+
+enum { EPI_F32 = 0, EPI_BIAS_F32 = 1, EPI_BIAS_GELU = 2 };
+
+template <int EPI>
+__global__ void k_gemm_bf16(const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, const float* bias,
+                            float* Cf, int ldcf, bf16* Cb, int ldcb);
+__global__ void k_transpose_bf16(const bf16* in, int ldi, bf16* out, int ldo, int R, int C);
+__global__ void k_gelu_backward(const float* g, const float* pre, bf16* d, long long n);
+__global__ void k_mse(const float* Y, const float* T, const float* W, int M, int Mvalid, int N, int ld,
+                      float inv_count, bf16* dY, float* loss);
+__global__ void k_bias_grad(const bf16* d, int M, int N, int ld, float* db);
+__global__ void k_adam(float* p, const float* g, float* m1, float* m2, long long n, float lr, float b1, float b2,
+                       float eps, float c1, float c2);
+__global__ void k_refresh_weights(const float* w, int out, int in, bf16* Wb, int ldw, bf16* WTb, int ldwt,
+                                  int out_pad, int in_pad);
+__global__ void k_to_bf16_padded(const float* x, int rows, int cols, bf16* y, int rows_pad, int ld);
+
+}  // namespace mlp

@@ -1,0 +1,281 @@
+// mlp_net.hip -- forward, backward and Adam of the BF16 MFMA MLP.
+#include "mlp_net.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <utility>
+
+namespace mlp {
+
+namespace {
+inline dim3 g1(long long n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
+
+template <int EPI>
+void gemm(hipStream_t s, const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, const float* bias,
+          float* Cf, int ldcf, bf16* Cb, int ldcb) {
+  dim3 grid((unsigned)(M / 64), (unsigned)(N / 64));
+  hipLaunchKernelGGL(k_gemm_bf16<EPI>, grid, dim3(64), 0, s, A, lda, B, ldb, M, N, K, bias, Cf, ldcf, Cb, ldcb);
+}
+
+void transpose(hipStream_t s, const bf16* in, int ldi, bf16* out, int ldo, int R, int C) {
+  dim3 grid((unsigned)((C + 31) / 32), (unsigned)((R + 31) / 32));
+  hipLaunchKernelGGL(k_transpose_bf16, grid, dim3(32, 8), 0, s, in, ldi, out, ldo, R, C);
+}
+}  // namespace
+
+int Net::fail(const char* what, hipError_t e) {
+  char buf[256];
+  snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+  err = buf;
+  return 3;
+}
+
+#define NT_TRY(expr)                      \
+  do {                                    \
+    hipError_t _e = (expr);               \
+    if (_e != hipSuccess) return fail(#expr, _e); \
+  } while (0)
+
+int Net::init(const std::vector<int>& dims, hipStream_t s) {
+  release();
+  stream = s;
+  if (dims.size() < 2) {
+    err = "network needs at least an input and an output width";
+    return 2;
+  }
+  nl = (int)dims.size() - 1;
+  din.assign(dims.begin(), dims.end() - 1);
+  dout.assign(dims.begin() + 1, dims.end());
+  pin.resize(nl);
+  pout.resize(nl);
+  for (int l = 0; l < nl; ++l) {
+    if (din[l] < 1 || dout[l] < 1) {
+      err = "layer widths must be positive";
+      return 2;
+    }
+    pin[l] = round_up(din[l], 64);
+    pout[l] = round_up(dout[l], 64);
+  }
+  np = 0;
+  nwb = 0;
+  offW.resize(nl);
+  offB.resize(nl);
+  offWb.resize(nl);
+  for (int l = 0; l < nl; ++l) {
+    offW[l] = np;
+    np += (size_t)pout[l] * pin[l];
+    offB[l] = np;
+    np += (size_t)pout[l];
+    offWb[l] = nwb;
+    nwb += (size_t)pout[l] * pin[l];
+  }
+  NT_TRY(hipMalloc((void**)&p, 4 * np));
+  NT_TRY(hipMalloc((void**)&g, 4 * np));
+  NT_TRY(hipMalloc((void**)&m1, 4 * np));
+  NT_TRY(hipMalloc((void**)&m2, 4 * np));
+  NT_TRY(hipMalloc((void**)&Wb, 2 * nwb));
+  NT_TRY(hipMalloc((void**)&WTb, 2 * nwb));
+  NT_TRY(hipMalloc((void**)&dloss, 64));
+  NT_TRY(hipMemsetAsync(p, 0, 4 * np, stream));
+  NT_TRY(hipMemsetAsync(g, 0, 4 * np, stream));
+  NT_TRY(hipMemsetAsync(m1, 0, 4 * np, stream));
+  NT_TRY(hipMemsetAsync(m2, 0, 4 * np, stream));
+  adam_t = 0;
+  return 0;
+}
+
+size_t Net::n_public() const {
+  size_t n = 0;
+  for (int l = 0; l < nl; ++l) n += (size_t)dout[l] * din[l] + dout[l];
+  return n;
+}
+
+int Net::set_params(const float* host) {
+  std::vector<float> pad(np, 0.f);
+  size_t q = 0;
+  for (int l = 0; l < nl; ++l) {
+    for (int r = 0; r < dout[l]; ++r)
+      for (int c = 0; c < din[l]; ++c) pad[offW[l] + (size_t)r * pin[l] + c] = host[q++];
+    for (int r = 0; r < dout[l]; ++r) pad[offB[l] + r] = host[q++];
+  }
+  NT_TRY(hipMemcpyAsync(p, pad.data(), 4 * np, hipMemcpyHostToDevice, stream));
+  NT_TRY(hipMemsetAsync(m1, 0, 4 * np, stream));
+  NT_TRY(hipMemsetAsync(m2, 0, 4 * np, stream));
+  adam_t = 0;
+  for (int l = 0; l < nl; ++l) {
+    dim3 grid((unsigned)((pin[l] + 255) / 256), (unsigned)pout[l]);
+    hipLaunchKernelGGL(k_refresh_weights, grid, dim3(256), 0, stream, p + offW[l], pout[l], pin[l], Wb + offWb[l],
+                       pin[l], WTb + offWb[l], pout[l], pout[l], pin[l]);
+  }
+  NT_TRY(hipGetLastError());
+  NT_TRY(hipStreamSynchronize(stream));
+  return 0;
+}
+
+static int copy_public(const Net& n, const float* dev, float* host, hipStream_t s, std::string& err) {
+  std::vector<float> pad(n.np);
+  if (hipMemcpyAsync(pad.data(), dev, 4 * n.np, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    err = "device to host copy of parameters failed";
+    return 3;
+  }
+  size_t q = 0;
+  for (int l = 0; l < n.nl; ++l) {
+    for (int r = 0; r < n.dout[l]; ++r)
+      for (int c = 0; c < n.din[l]; ++c) host[q++] = pad[n.offW[l] + (size_t)r * n.pin[l] + c];
+    for (int r = 0; r < n.dout[l]; ++r) host[q++] = pad[n.offB[l] + r];
+  }
+  return 0;
+}
+
+int Net::get_params(float* host) const { return copy_public(*this, p, host, stream, const_cast<Net*>(this)->err); }
+int Net::get_grads(float* host) const { return copy_public(*this, g, host, stream, const_cast<Net*>(this)->err); }
+
+void Net::free_workspace() {
+  auto F = [](void* q) {
+    if (q) (void)hipFree(q);
+  };
+  F(X0);
+  F(X0T);
+  for (auto q : pre) F(q);
+  for (auto q : act) F(q);
+  for (auto q : actT) F(q);
+  F(Y);
+  F(T);
+  F(Wm);
+  F(dY);
+  F(dYT);
+  F(dXf);
+  F(dpre);
+  F(dpreT);
+  X0 = X0T = nullptr;
+  pre.clear();
+  act.clear();
+  actT.clear();
+  Y = T = Wm = dXf = nullptr;
+  dY = dYT = dpre = dpreT = nullptr;
+  Mcap = 0;
+}
+
+void Net::release() {
+  free_workspace();
+  auto F = [](void* q) {
+    if (q) (void)hipFree(q);
+  };
+  F(p);
+  F(g);
+  F(m1);
+  F(m2);
+  F(Wb);
+  F(WTb);
+  F(dloss);
+  p = g = m1 = m2 = dloss = nullptr;
+  Wb = WTb = nullptr;
+  nl = 0;
+}
+
+int Net::ensure_workspace(int Mpad, bool training) {
+  if (Mpad <= Mcap && (!training || T)) return 0;
+  free_workspace();
+  const size_t M = (size_t)Mpad;
+  int wmax = 0;
+  for (int l = 0; l < nl; ++l) wmax = std::max(wmax, std::max(pin[l], pout[l]));
+  NT_TRY(hipMalloc((void**)&X0, 2 * M * pin[0]));
+  pre.assign(nl - 1, nullptr);
+  act.assign(nl - 1, nullptr);
+  actT.assign(nl - 1, nullptr);
+  for (int l = 0; l < nl - 1; ++l) {
+    NT_TRY(hipMalloc((void**)&act[l], 2 * M * pout[l]));
+    if (training) {
+      NT_TRY(hipMalloc((void**)&pre[l], 4 * M * pout[l]));
+      NT_TRY(hipMalloc((void**)&actT[l], 2 * M * pout[l]));
+    }
+  }
+  const int po = pout[nl - 1];
+  NT_TRY(hipMalloc((void**)&Y, 4 * M * po));
+  if (training) {
+    NT_TRY(hipMalloc((void**)&X0T, 2 * M * pin[0]));
+    NT_TRY(hipMalloc((void**)&T, 4 * M * po));
+    NT_TRY(hipMalloc((void**)&Wm, 4 * M * po));
+    NT_TRY(hipMalloc((void**)&dY, 2 * M * wmax));
+    NT_TRY(hipMalloc((void**)&dYT, 2 * M * wmax));
+    NT_TRY(hipMalloc((void**)&dXf, 4 * M * wmax));
+    NT_TRY(hipMalloc((void**)&dpre, 2 * M * wmax));
+    NT_TRY(hipMalloc((void**)&dpreT, 2 * M * wmax));
+  }
+  Mcap = Mpad;
+  return 0;
+}
+
+int Net::forward(int Mpad, bool keep_pre) {
+  const bf16* x = X0;
+  int ldx = pin[0];
+  for (int l = 0; l < nl; ++l) {
+    const bf16* W = Wb + offWb[l];
+    const float* b = p + offB[l];
+    if (l < nl - 1) {
+      gemm<EPI_BIAS_GELU>(stream, x, ldx, W, pin[l], Mpad, pout[l], pin[l], b, keep_pre ? pre[l] : nullptr, pout[l],
+                          act[l], pout[l]);
+      x = act[l];
+      ldx = pout[l];
+    } else {
+      gemm<EPI_BIAS_F32>(stream, x, ldx, W, pin[l], Mpad, pout[l], pin[l], b, Y, pout[l], nullptr, 0);
+    }
+  }
+  NT_TRY(hipGetLastError());
+  return 0;
+}
+
+int Net::backward(int Mpad, int Mvalid, double count, float* loss_out) {
+  const int L = nl - 1;
+  const int po = pout[L];
+  NT_TRY(hipMemsetAsync(dloss, 0, 4, stream));
+  const float inv = count > 0 ? (float)(1.0 / count) : 0.f;
+  hipLaunchKernelGGL(k_mse, g1((long long)Mpad * po, 256), dim3(256), 0, stream, Y, T, Wm, Mpad, Mvalid, dout[L], po,
+                     inv, dY, dloss);
+  // d (bf16, Mpad x pout[l]) is the gradient w.r.t. layer l's output (pre-activation)
+  bf16* d = dY;
+  for (int l = L; l >= 0; --l) {
+    const int o = pout[l], i = pin[l];
+    const bf16* xin = (l == 0) ? X0 : act[l - 1];
+    bf16* xinT = (l == 0) ? X0T : actT[l - 1];
+    // dW_l = d^T x_in : (o x Mpad) (i x Mpad)^T
+    transpose(stream, d, o, dYT, Mpad, Mpad, o);
+    transpose(stream, xin, i, xinT, Mpad, Mpad, i);
+    gemm<EPI_F32>(stream, dYT, Mpad, xinT, Mpad, o, i, Mpad, nullptr, g + offW[l], i, nullptr, 0);
+    hipLaunchKernelGGL(k_bias_grad, g1(o, 256), dim3(256), 0, stream, d, Mpad, o, o, g + offB[l]);
+    if (l > 0) {
+      // dX = d W_l : (Mpad x o) (i x o)^T, then through GELU of layer l-1
+      gemm<EPI_F32>(stream, d, o, WTb + offWb[l], o, Mpad, i, o, nullptr, dXf, i, nullptr, 0);
+      hipLaunchKernelGGL(k_gelu_backward, g1((long long)Mpad * i, 256), dim3(256), 0, stream, dXf, pre[l - 1], dpre,
+                         (long long)Mpad * i);
+      // the next layer's d lives in dpre; swap the role of the two buffers
+      std::swap(dY, dpre);
+      d = dY;
+    }
+  }
+  NT_TRY(hipGetLastError());
+  if (loss_out) {
+    NT_TRY(hipMemcpyAsync(loss_out, dloss, 4, hipMemcpyDeviceToHost, stream));
+    NT_TRY(hipStreamSynchronize(stream));
+  }
+  return 0;
+}
+
+int Net::adam(float lr, float b1, float b2, float eps) {
+  ++adam_t;
+  float c1 = 1.f - std::pow(b1, (float)adam_t), c2 = 1.f - std::pow(b2, (float)adam_t);
+  hipLaunchKernelGGL(k_adam, g1((long long)np, 256), dim3(256), 0, stream, p, g, m1, m2, (long long)np, lr, b1, b2,
+                     eps, c1, c2);
+  for (int l = 0; l < nl; ++l) {
+    dim3 grid((unsigned)((pin[l] + 255) / 256), (unsigned)pout[l]);
+    hipLaunchKernelGGL(k_refresh_weights, grid, dim3(256), 0, stream, p + offW[l], pout[l], pin[l], Wb + offWb[l],
+                       pin[l], WTb + offWb[l], pout[l], pout[l], pin[l]);
+  }
+  NT_TRY(hipGetLastError());
+  return 0;
+}
+
+}  // namespace mlp

@@ -147,6 +147,7 @@ def default_config() -> Config:
 
 # (name, restype, argtypes) for every symbol declared in include/rdb_colphys.h
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
 _hp = ctypes.POINTER(ctypes.c_void_p)
@@ -175,6 +176,10 @@ PROTOTYPES = [
     ("rdb_colphys_set_timing", ctypes.c_int, [_vp, ctypes.c_int]),
     ("rdb_colphys_stats", ctypes.c_int, [_vp, ctypes.POINTER(Counters)]),
     ("rdb_colphys_last_error", ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
+    ("rdb_mlp_eval", ctypes.c_int,
+     [_ip, ctypes.c_int, _fp, ctypes.c_int, _fp, _fp]),
+    ("rdb_mlp_loss_grad", ctypes.c_int,
+     [_ip, ctypes.c_int, _fp, ctypes.c_int, _fp, _fp, _fp, _fp, _fp]),
 ]
 
 
