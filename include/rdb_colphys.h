@@ -124,7 +124,8 @@ typedef struct rdb_colphys_counters {
   double algorithmic_bytes_per_step; /* sum over wet columns of B_col(L_c), L_c = live levels */
   double reference_bytes_per_step;   /* B_col(nz) * wet_columns (all levels)  */
   long long live_levels;       /* sum of live levels over wet columns      */
-  double fallback_fraction;    /* surrogate closure: plain-path share      */
+  double fallback_fraction;    /* surrogate closure: share of wet columns
+                                  whose guard tripped in the last step      */
   int block_threads;           /* autotuned launch shape                   */
   int order_mode;              /* 0 natural order, 1 sorted by depth       */
   int lds_levels;              /* autotuned live levels kept in LDS        */
@@ -186,9 +187,10 @@ int rdb_colphys_destroy(rdb_colphys **handle);
  *   "rho_layer"      (nx, ny, nz)     get; set overrides the EOS output
  *   "stress_mag"     (nx, ny, 1)      get only
  *   "b0"             (nx, ny, 1)      get only
- *   "kbl"            (nx, ny, 1)      get only; KPP crossing layer (0 none),
+ *   "kbl"            (nx, ny, 1)      get only; KPP crossing layer (0 none,
+ *                                     -1 where the surrogate closure was used),
  *                                     stored as double
- *   "surrogate_used" (nx, ny, 1)      get only; 1 where the MLP K was used
+ *   "coriolis"       (nx, ny, 1)      Coriolis parameter f (surrogate input)
  * A shape mismatch returns RDB_COLPHYS_ERR_BAD_SHAPE, an unknown name
  * RDB_COLPHYS_ERR_NOT_FOUND.
  */
@@ -250,6 +252,52 @@ int rdb_colphys_heat_salt_totals(rdb_colphys *h, double *heat, double *salt);
  * holds only the step kernels), accumulated into sum_kernel_ms. */
 int rdb_colphys_set_timing(rdb_colphys *h, int enable_timing);
 int rdb_colphys_stats(rdb_colphys *h, rdb_colphys_counters *out);
+
+/* ---- learned closure (no counterpart in the reference) -----------------
+ * A multilayer perceptron predicts, per wet column, log10 kt and log10 kv at
+ * the interior interfaces and the KPP boundary-layer depth from the column
+ * state and forcing (T, S, centred u, v, log10 h at every level; |tau|,
+ * Q_heat, Q_salt, Coriolis f, lagged boundary-layer depth, column depth).
+ * BF16 weights and activations on MFMA with FP32 accumulation.  Its kv/kt go
+ * through the same assembly clip and FP64 tridiagonal solve as the plain
+ * path; the non-local transport uses the predicted depth.  A column falls
+ * back to the plain PP81 + KPP path when its guard trips. */
+#define RDB_COLPHYS_GUARD_STRICT 0  /* any live interface with N^2 < 0, or an input outside the training range */
+#define RDB_COLPHYS_GUARD_SURFACE 1 /* N^2 < 0 below the surface layer, or an input out of range */
+#define RDB_COLPHYS_GUARD_RANGE 2   /* an input outside the training range only */
+
+typedef struct rdb_colphys_mlp_config {
+  int hidden;          /* 256 hidden width */
+  int n_hidden;        /* 4 hidden layers (GELU) */
+  int batch;           /* 8192 */
+  int epochs;          /* passes over the training columns */
+  double lr;           /* Adam 1e-3 */
+  double beta1, beta2, eps; /* 0.9, 0.999, 1e-8 */
+  int seed;            /* initialisation and shuffling */
+  int guard_mode;      /* RDB_COLPHYS_GUARD_* used at inference */
+  double val_fraction; /* held-out share of the collected columns (0.05) */
+} rdb_colphys_mlp_config;
+
+int rdb_colphys_mlp_config_default(rdb_colphys_mlp_config *cfg);
+
+/* Run n_steps plain steps, recording (features before, coefficients after)
+ * of every stride-th wet column; *n_samples returns the collected total.
+ * The handle's state advances by n_steps. */
+int rdb_colphys_mlp_collect(rdb_colphys *h, double dt, int n_steps, int stride, long long *n_samples);
+
+/* Normalise the collected columns and train with Adam on the GPU (forward,
+ * backward and update kernels of this library).  Per epoch the mean training
+ * and held-out losses (weighted MSE of the normalised targets) go to
+ * train_loss / val_loss (n_hist entries at most). */
+int rdb_colphys_mlp_train(rdb_colphys *h, const rdb_colphys_mlp_config *cfg, float *train_loss, float *val_loss,
+                          int n_hist, int *n_epochs_done);
+
+/* Network + normalisation statistics to / from a file. */
+int rdb_colphys_mlp_save(rdb_colphys *h, const char *path);
+int rdb_colphys_mlp_load(rdb_colphys *h, const char *path);
+
+/* RDB_COLPHYS_GUARD_* for subsequent surrogate steps. */
+int rdb_colphys_set_guard(rdb_colphys *h, int guard_mode);
 
 /* ---- learned closure: the MLP on host arrays --------------------------
  * No counterpart in the reference: these run the BF16 MFMA network of the

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/rdb_colphys.h"
+#include "colphys_handle.h"
 #include "colphys_kernels.h"
 
 using colphys::Params;
@@ -63,45 +64,7 @@ struct FieldSpec {
 // error ring shared with the MLP entry points
 int rdb_colphys_push_error(int status, const char* msg) { return push_error(status, "%s", msg); }
 
-struct rdb_colphys {
-  rdb_colphys_config cfg;
-  Params pr;
-  int nx, ny, nz, n_passive;
-  int n_all, n_wet, P;
-  std::vector<int> col_host;
-  std::vector<unsigned char> dry_host;  // per full-grid column
-  int* d_col = nullptr;
-  StateView s{};
-  double* d_stage = nullptr;
-  size_t stage_elems = 0;
-  double *d_txc = nullptr, *d_tyc = nullptr, *d_smag = nullptr, *d_shelf = nullptr;
-  unsigned int* d_guard = nullptr;
-  unsigned long long* d_acc = nullptr;
-  hipStream_t stream = nullptr;
-  bool static_dirty = true;
-  std::vector<double> h_u, h_v, h_taux, h_tauy;
-  // graph cache
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  double graph_dt = 0.0;
-  int graph_n = 0;
-  int graph_block = 0;
-  int graph_lds = -1;
-  int graph_prec = -1;
-  std::vector<hipEvent_t> events;
-  // measurement
-  bool timing = false;
-  double last_ms = 0.0, sum_ms = 0.0;
-  long long timed = 0, steps = 0;
-  int block = 256;
-  int lds_levels = 0;  // live levels of Thomas state kept in LDS per lane
-  long long live_levels = 0;  // sum over wet columns of live levels
-  bool block_pinned = false;
-  int order_mode = 0;
-  int precision = RDB_COLPHYS_PREC_FP64;
-  int closure = RDB_COLPHYS_CLOSURE_PLAIN;
-  std::vector<void*> allocations;
-};
+// the handle (struct rdb_colphys) is defined in colphys_handle.h
 
 namespace {
 
@@ -160,6 +123,7 @@ bool lookup_field(const char* name, int n_passive, FieldSpec& spec, int& tracer)
       {"stress_mag", {F_COL, false, true}},
       {"b0", {F_COL, false, true}},
       {"kbl", {F_COL_INT, false, true}},
+      {"coriolis", {F_COL, true, true}},
   };
   for (const E& e : table)
     if (std::strcmp(e.n, name) == 0) {
@@ -197,6 +161,7 @@ double* layer_ptr(rdb_colphys* h, const char* name, int tracer) {
   if (!std::strcmp(name, "bl_depth")) return s.bl;
   if (!std::strcmp(name, "stress_mag")) return h->d_smag;
   if (!std::strcmp(name, "b0")) return s.b0;
+  if (!std::strcmp(name, "coriolis")) return h->d_coriolis;
   return nullptr;
 }
 
@@ -211,6 +176,8 @@ int dalloc(rdb_colphys* h, void** p, size_t bytes) {
 }
 
 void release(rdb_colphys* h) {
+  colphys::surr_free(h->surr);
+  h->surr = nullptr;
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
   for (hipEvent_t e : h->events) (void)hipEventDestroy(e);
@@ -229,6 +196,10 @@ int launch_check(const char* what) {
   return RDB_COLPHYS_OK;
 }
 
+}  // namespace
+
+namespace colphys {
+
 // Launch the fused step over packed columns [p0, p1).
 int launch_step(rdb_colphys* h, double dt, int p0, int p1) {
   if (p1 <= p0) return RDB_COLPHYS_OK;
@@ -238,9 +209,10 @@ int launch_step(rdb_colphys* h, double dt, int p0, int p1) {
   dim3 g = grid1(p1 - p0, h->block), b(h->block);
   const int M = h->lds_levels;
   const size_t shmem = (size_t)h->block * (size_t)M * (size_t)(3 + std::min(ntr, 4) + (two ? 1 : 0)) * 8;
-#define LAUNCH(R, N, TF)                                                                                      \
-  hipLaunchKernelGGL((colphys::k_column_step<R, N, TF>), g, b, shmem, h->stream, h->s, h->pr, dt, p0, p1, \
-                     h->d_guard, M)
+  const colphys::SurrView nosv{nullptr, 0, nullptr, nullptr, nullptr};
+#define LAUNCH(R, N, TF)                                                                                        \
+  hipLaunchKernelGGL((colphys::k_column_step<R, N, TF, false>), g, b, shmem, h->stream, h->s, h->pr, dt, p0, p1, \
+                     h->d_guard, M, nosv)
 #define BY_NTR(R, TF)               \
   switch (ntr) {                    \
     case 0: LAUNCH(R, 0, TF); break; \
@@ -289,6 +261,14 @@ int refresh_static(rdb_colphys* h, double dt) {
   h->static_dirty = false;
   return RDB_COLPHYS_OK;
 }
+
+}  // namespace colphys
+
+namespace {
+using colphys::launch_step;
+using colphys::read_guard;
+using colphys::refresh_static;
+using colphys::staged_coefficients;
 
 void drop_graph(rdb_colphys* h) {
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
@@ -486,6 +466,7 @@ int rdb_colphys_create(const rdb_colphys_config* cfg, int nx, int ny, int nz, in
   A(&h->d_tyc, 8 * P);
   A(&h->d_smag, 8 * P);
   A(&h->d_shelf, 8 * P);
+  A(&h->d_coriolis, 8 * P);
   A(&h->d_guard, 64);
   A(&h->d_acc, 64);
   h->stage_elems = (size_t)(nx + 1) * (size_t)(ny + 1) * (size_t)(nz + 1);
@@ -739,11 +720,30 @@ int rdb_colphys_nonlocal(rdb_colphys* h, double dt) {
 int rdb_colphys_step(rdb_colphys* h, double dt, int n_steps) {
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "step: bad handle");
   if (n_steps <= 0) return RDB_COLPHYS_OK;
-  if (h->closure != RDB_COLPHYS_CLOSURE_PLAIN)
-    return push_error(RDB_COLPHYS_ERR_SETUP, "step: the surrogate closure is not loaded");
   int st = refresh_static(h, dt);
   if (st) return st;
-  if (h->n_passive > 4) {
+  if (h->closure == RDB_COLPHYS_CLOSURE_SURROGATE) {
+    // features -> MLP (MFMA GEMMs) -> fused step with the predicted
+    // coefficients, eagerly launched; events bracket the whole sequence
+    if (h->timing) {
+      st = ensure_events(h, 2);
+      if (st) return st;
+      HIP_TRY(hipEventRecord(h->events[0], h->stream));
+    }
+    for (int i = 0; i < n_steps; ++i) {
+      st = colphys::surr_step(h, dt);
+      if (st) return st;
+    }
+    if (h->timing) {
+      HIP_TRY(hipEventRecord(h->events[1], h->stream));
+      HIP_TRY(hipEventSynchronize(h->events[1]));
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, h->events[0], h->events[1]));
+      h->last_ms = ms / n_steps;
+      h->sum_ms += ms;
+      h->timed += n_steps;
+    }
+  } else if (h->n_passive > 4) {
     // beyond the fused kernel's tracer slots: the staged sequence, per step
     for (int i = 0; i < n_steps; ++i) {
       if ((st = rdb_colphys_eos(h)) || (st = staged_coefficients(h, 0, h->n_wet)) ||
@@ -876,8 +876,15 @@ int rdb_colphys_set_closure(rdb_colphys* h, int mode) {
     h->closure = mode;
     return RDB_COLPHYS_OK;
   }
-  if (mode == RDB_COLPHYS_CLOSURE_SURROGATE)
-    return push_error(RDB_COLPHYS_ERR_SETUP, "set_closure: no surrogate network is loaded into this handle");
+  if (mode == RDB_COLPHYS_CLOSURE_SURROGATE) {
+    if (!colphys::surr_ready(h))
+      return push_error(RDB_COLPHYS_ERR_NOT_INITIALISED,
+                        "set_closure: no surrogate network is trained or loaded into this handle");
+    if (h->precision != RDB_COLPHYS_PREC_FP64)
+      return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "set_closure: the surrogate runs with the FP64 solve only");
+    h->closure = mode;
+    return RDB_COLPHYS_OK;
+  }
   return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "set_closure: unknown mode %d", mode);
 }
 
@@ -931,10 +938,57 @@ int rdb_colphys_stats(rdb_colphys* h, rdb_colphys_counters* out) {
   out->algorithmic_bytes_per_step = 8.0 * (14.0 * Lsum + 3.0 * (double)h->n_wet) + 44.0 * (double)h->n_wet;
   out->live_levels = h->live_levels;
   out->lds_levels = h->lds_levels;
-  out->fallback_fraction = h->closure == RDB_COLPHYS_CLOSURE_PLAIN ? 1.0 : 0.0;
+  out->fallback_fraction =
+      h->closure == RDB_COLPHYS_CLOSURE_PLAIN ? 1.0 : (double)h->last_fallback / (double)std::max(h->n_wet, 1);
   out->block_threads = h->block;
   out->order_mode = h->order_mode;
   return RDB_COLPHYS_OK;
+}
+
+int rdb_colphys_mlp_config_default(rdb_colphys_mlp_config* c) {
+  if (!c) return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "mlp_config_default: NULL config");
+  std::memset(c, 0, sizeof *c);
+  c->hidden = 256;
+  c->n_hidden = 4;
+  c->batch = 8192;
+  c->epochs = 8;
+  c->lr = 1.0e-3;
+  c->beta1 = 0.9;
+  c->beta2 = 0.999;
+  c->eps = 1.0e-8;
+  c->seed = 1;
+  c->guard_mode = RDB_COLPHYS_GUARD_STRICT;
+  c->val_fraction = 0.05;
+  return RDB_COLPHYS_OK;
+}
+
+int rdb_colphys_mlp_collect(rdb_colphys* h, double dt, int n_steps, int stride, long long* n_samples) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "mlp_collect: bad handle");
+  if (n_steps < 0) return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "mlp_collect: negative step count");
+  return colphys::surr_collect(h, dt, n_steps, stride, n_samples);
+}
+
+int rdb_colphys_mlp_train(rdb_colphys* h, const rdb_colphys_mlp_config* cfg, float* train_loss, float* val_loss,
+                          int n_hist, int* n_epochs_done) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "mlp_train: bad handle");
+  return colphys::surr_train(h, cfg, train_loss, val_loss, n_hist, n_epochs_done);
+}
+
+int rdb_colphys_mlp_save(rdb_colphys* h, const char* path) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "mlp_save: bad handle");
+  if (!path) return push_error(RDB_COLPHYS_ERR_IO, "mlp_save: NULL path");
+  return colphys::surr_save(h, path);
+}
+
+int rdb_colphys_mlp_load(rdb_colphys* h, const char* path) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "mlp_load: bad handle");
+  if (!path) return push_error(RDB_COLPHYS_ERR_IO, "mlp_load: NULL path");
+  return colphys::surr_load(h, path);
+}
+
+int rdb_colphys_set_guard(rdb_colphys* h, int mode) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "set_guard: bad handle");
+  return colphys::surr_set_guard(h, mode);
 }
 
 int rdb_colphys_last_error(int idx, char* buf, int cap) {

@@ -1,0 +1,72 @@
+// colphys_handle.h -- the column-physics handle (internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/rdb_colphys.h"
+#include "colphys_kernels.h"
+
+namespace colphys {
+struct SurrState;
+// shared by the C ABI translation units
+int launch_step(rdb_colphys* h, double dt, int p0, int p1);
+int refresh_static(rdb_colphys* h, double dt);
+int surr_step(rdb_colphys* h, double dt);
+void surr_free(SurrState* s);
+int surr_collect(rdb_colphys* h, double dt, int n_steps, int stride, long long* n_out);
+int surr_train(rdb_colphys* h, const rdb_colphys_mlp_config* cfg, float* train_loss, float* val_loss, int n_hist,
+               int* n_done);
+int surr_save(rdb_colphys* h, const char* path);
+int surr_load(rdb_colphys* h, const char* path);
+int surr_set_guard(rdb_colphys* h, int mode);
+bool surr_ready(rdb_colphys* h);
+}  // namespace colphys
+
+int rdb_colphys_push_error(int status, const char* msg);
+
+struct rdb_colphys {
+  rdb_colphys_config cfg;
+  colphys::Params pr;
+  int nx, ny, nz, n_passive;
+  int n_all, n_wet, P;
+  std::vector<int> col_host;
+  std::vector<unsigned char> dry_host;  // per full-grid column
+  int* d_col = nullptr;
+  colphys::StateView s{};
+  double* d_stage = nullptr;
+  size_t stage_elems = 0;
+  double *d_txc = nullptr, *d_tyc = nullptr, *d_smag = nullptr, *d_shelf = nullptr;
+  unsigned int* d_guard = nullptr;
+  unsigned long long* d_acc = nullptr;
+  hipStream_t stream = nullptr;
+  bool static_dirty = true;
+  std::vector<double> h_u, h_v, h_taux, h_tauy;
+  // graph cache
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  double graph_dt = 0.0;
+  int graph_n = 0;
+  int graph_block = 0;
+  int graph_lds = -1;
+  int graph_prec = -1;
+  std::vector<hipEvent_t> events;
+  // measurement
+  bool timing = false;
+  double last_ms = 0.0, sum_ms = 0.0;
+  long long timed = 0, steps = 0;
+  int block = 256;
+  int lds_levels = 0;  // live levels of Thomas state kept in LDS per lane
+  long long live_levels = 0;  // sum over wet columns of live levels
+  bool block_pinned = false;
+  int order_mode = 0;
+  int precision = RDB_COLPHYS_PREC_FP64;
+  int closure = RDB_COLPHYS_CLOSURE_PLAIN;
+  std::vector<void*> allocations;
+  // learned closure (trained or loaded network, dataset, guard state)
+  double* d_coriolis = nullptr;
+  colphys::SurrState* surr = nullptr;
+  long long last_fallback = 0;
+};

@@ -53,9 +53,21 @@ __global__ void k_split_assemble(StateView s, Params pr, int p0, int p1, unsigne
 __global__ void k_surface_flux(StateView s, Params pr, double dt, int p0, int p1);
 __global__ void k_vdiff(StateView s, Params pr, double dt, int p0, int p1);
 __global__ void k_nonlocal(StateView s, double dt, int p0, int p1);
+// Learned-closure inputs of the fused step: the network's outputs stored
+// output-major (Yt[n * ldy + p], p the packed wet column), their
+// normalisation, and a per-column guard flag (non-zero: plain path).
+struct SurrView {
+  const float* Yt;
+  long long ldy;
+  const float* mean;
+  const float* stdv;
+  const unsigned char* guard;
+};
+
 __global__ void k_prepare(StateView s, Params pr, int p0, int p1);
-template <typename R, int NTR, bool TWO_FACTOR>
-__global__ void k_column_step(StateView s, Params pr, double dt, int p0, int p1, unsigned int* guard_count, int M);
+template <typename R, int NTR, bool TWO_FACTOR, bool SURR>
+__global__ void k_column_step(StateView s, Params pr, double dt, int p0, int p1, unsigned int* guard_count, int M,
+                              SurrView sv);
 __global__ void k_totals(StateView s, int p0, int p1, unsigned long long* acc);
 
 }  // namespace colphys

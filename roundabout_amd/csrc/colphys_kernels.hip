@@ -375,9 +375,9 @@ __global__ void k_prepare(StateView s, Params pr, int p0, int p1) {
 // one factorisation of kt serves every tracer (the reference repeats the
 // identical arithmetic on its bitwise copy ks = kt).
 // ---------------------------------------------------------------------------
-template <typename R, int NTR, bool TWO_FACTOR>
+template <typename R, int NTR, bool TWO_FACTOR, bool SURR>
 __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, double dt, int p0, int p1,
-                                                     unsigned int* guard_count, int M) {
+                                                     unsigned int* guard_count, int M, SurrView sv) {
   extern __shared__ double lds[];
   int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= p1) return;
@@ -385,6 +385,12 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
   const double HV = H_VANISHED;
   const R rho0 = R(pr.rho0);
   const int kb = s.kb[p];
+  // learned closure: this column takes the network's kv/kt and h_b unless
+  // its guard tripped (then the plain PP81 + KPP path below runs for it)
+  const bool use_s = SURR && sv.guard[p] == 0;
+  auto pred = [&](int n) -> double {
+    return (double)(sv.Yt[(size_t)n * (size_t)sv.ldy + (size_t)p] * sv.stdv[n] + sv.mean[n]);
+  };
   const int B = blockDim.x, tid = threadIdx.x;
   // LDS slot of quantity q (c't, rT, rS, r of each passive, c's) at live
   // level m of this lane
@@ -400,7 +406,31 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
   int k_lo = nz + 1;  // lowest interface holding a KPP value (none: > nz)
   R qT_r = R(0), qS_r = R(0), h_b_r = R(0);
   bool destab = false, kpp_on = false;
-  if (pr.use_kpp) {
+  if (use_s) {
+    // predicted boundary-layer depth; B0, w_s and the gamma range follow the
+    // KPP pass-2 formulas from it (rdb_ocean_vmix.F90:1428-1515)
+    R u_star = sqrt(R(tau_mag) / rho0);
+    qT_r = R(qh) / (rho0 * R(pr.cp));
+    qS_r = R(qsv) / rho0;
+    R B0 = kpp_b0<R>(pr, qT_r, qS_r);
+    R h_b = fmx(R(pred(2 * (nz - 1))), R(0));
+    s.b0[p] = (double)B0;
+    destab = (B0 < R(0));
+    R w_star = cbrt_kahan(fmx(R(0), -B0) * h_b);
+    R w_s = sqrt(u_star * u_star + w_star * w_star);
+    kpp_on = pr.use_kpp && (h_b > R(0) && w_s > R(0));
+    h_b_r = h_b;
+    if (kpp_on) {
+      R d_running = R(0);
+      for (int k = k_sfc; k >= 1; --k) {
+        d_running = d_running + R(LV(s.h, k));
+        if (!(d_running < h_b)) break;
+        k_lo = k;
+      }
+    }
+    s.bl[p] = (double)h_b;
+    s.kbl[p] = -1;  // no bulk-Richardson crossing was searched
+  } else if (pr.use_kpp) {
     const R u_ref = R(LV(s.ub, k_sfc)), v_ref = R(LV(s.vb, k_sfc));
     double h_sfc = LV(s.h, k_sfc);
     // EOS and the buoyancy difference stay FP64 in every precision mode
@@ -486,7 +516,8 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
     // filler interfaces the boundary layer reached last step but not now get
     // their plain value back
     const int fd = s.fdirty[p];
-    const int top = min(kb - 1, k_lo - 1);
+    const int k_lo_fill = use_s ? nz + 1 : k_lo;  // the surrogate leaves filler interfaces plain
+    const int top = min(kb - 1, k_lo_fill - 1);
     for (int f = max(fd, 1); f <= top; ++f) {
       double kv, kt;
       filler_iface(s, pr, p, f, kv, kt);
@@ -495,7 +526,7 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
       LV(s.kt, f) = fmn(fmx(kt, pr.kt_bg), pr.kd_max);
       LV(s.ks, f) = fmn(fmx(ks, pr.ks_bg), pr.kd_max);
     }
-    int nd = (k_lo < kb) ? k_lo : nz + 1;
+    int nd = (k_lo_fill < kb) ? k_lo_fill : nz + 1;
     if (nd != fd) s.fdirty[p] = nd;
   }
   if (kb >= nz) return;  // no live layer: nothing to mix or diffuse
@@ -515,10 +546,18 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
   // coefficients of interface kf between layers kf-1 (below) and kf (above)
   auto iface = [&](int kf, R ub_lo, R vb_lo, R ub_hi, R vb_hi, double h_lo, double h_hi, double rho_lo,
                    double rho_hi, double& kt_out, double& ks_out) {
-    R kv_r, kt_r;
-    pp81_iface<R>(pr, ub_lo, vb_lo, ub_hi, vb_hi, R(h_lo), R(h_hi), rho_hi - rho_lo, kv_r, kt_r);
-    double kvv = (double)kv_r, ktv = (double)kt_r;
-    if (kf >= k_lo && kf <= k_sfc) {
+    double kvv, ktv;
+    if (use_s) {
+      // predicted log10 kt (outputs 0..nz-2) and log10 kv (nz-1..2nz-3)
+      ktv = exp2(pred(kf - 1) * 3.321928094887362);
+      kvv = exp2(pred(nz - 1 + kf - 1) * 3.321928094887362);
+    } else {
+      R kv_r, kt_r;
+      pp81_iface<R>(pr, ub_lo, vb_lo, ub_hi, vb_hi, R(h_lo), R(h_hi), rho_hi - rho_lo, kv_r, kt_r);
+      kvv = (double)kv_r;
+      ktv = (double)kt_r;
+    }
+    if (!use_s && kf >= k_lo && kf <= k_sfc) {
       double kk = LV(s.kv, kf);  // KPP value parked by sweep A
       kvv = fmx(kvv, kk);
       ktv = fmx(ktv, kk);
@@ -747,12 +786,21 @@ __global__ void k_totals(StateView s, int p0, int p1, unsigned long long* acc) {
 }
 
 // Explicit instantiations of the fused step.
-#define INST(R, N, TF) \
-  template __global__ void k_column_step<R, N, TF>(StateView, Params, double, int, int, unsigned int*, int);
-INST(double, 0, false) INST(double, 1, false) INST(double, 2, false) INST(double, 3, false) INST(double, 4, false)
-INST(double, 0, true) INST(double, 1, true) INST(double, 2, true) INST(double, 3, true) INST(double, 4, true)
-INST(float, 0, false) INST(float, 1, false) INST(float, 2, false) INST(float, 3, false) INST(float, 4, false)
-INST(float, 0, true) INST(float, 1, true) INST(float, 2, true) INST(float, 3, true) INST(float, 4, true)
+#define INST(R, N, TF, SU)                                                                             \
+  template __global__ void k_column_step<R, N, TF, SU>(StateView, Params, double, int, int, unsigned int*, int, \
+                                                       SurrView);
+INST(double, 0, false, false) INST(double, 1, false, false) INST(double, 2, false, false)
+INST(double, 3, false, false) INST(double, 4, false, false)
+INST(double, 0, true, false) INST(double, 1, true, false) INST(double, 2, true, false)
+INST(double, 3, true, false) INST(double, 4, true, false)
+INST(float, 0, false, false) INST(float, 1, false, false) INST(float, 2, false, false)
+INST(float, 3, false, false) INST(float, 4, false, false)
+INST(float, 0, true, false) INST(float, 1, true, false) INST(float, 2, true, false)
+INST(float, 3, true, false) INST(float, 4, true, false)
+INST(double, 0, false, true) INST(double, 1, false, true) INST(double, 2, false, true)
+INST(double, 3, false, true) INST(double, 4, false, true)
+INST(double, 0, true, true) INST(double, 1, true, true) INST(double, 2, true, true)
+INST(double, 3, true, true) INST(double, 4, true, true)
 #undef INST
 
 }  // namespace colphys

@@ -8,16 +8,17 @@ namespace mlp {
 
 typedef uint16_t bf16;  // raw bfloat16 bits in memory
 
-enum { EPI_F32 = 0, EPI_BIAS_F32 = 1, EPI_BIAS_GELU = 2 };
+enum { EPI_F32 = 0, EPI_BIAS_F32 = 1, EPI_BIAS_GELU = 2, EPI_BIAS_F32_T = 3 };
 
 template <int EPI>
 __global__ void k_gemm_bf16(const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, const float* bias,
-                            float* Cf, int ldcf, bf16* Cb, int ldcb);
+                            float* Cf, int ldcf, bf16* Cb, int ldcb, int kc, long long split_stride);
 __global__ void k_transpose_bf16(const bf16* in, int ldi, bf16* out, int ldo, int R, int C);
 __global__ void k_gelu_backward(const float* g, const float* pre, bf16* d, long long n);
 __global__ void k_mse(const float* Y, const float* T, const float* W, int M, int Mvalid, int N, int ld,
                       float inv_count, bf16* dY, float* loss);
-__global__ void k_bias_grad(const bf16* d, int M, int N, int ld, float* db);
+__global__ void k_bias_partial(const bf16* d, int M, int N, int ld, int rows, float* part);
+__global__ void k_reduce_splits(const float* part, int S, long long n, long long stride, float* out);
 __global__ void k_adam(float* p, const float* g, float* m1, float* m2, long long n, float lr, float b1, float b2,
                        float eps, float c1, float c2);
 __global__ void k_refresh_weights(const float* w, int out, int in, bf16* Wb, int ldw, bf16* WTb, int ldwt,

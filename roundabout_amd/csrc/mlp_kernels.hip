@@ -45,10 +45,21 @@ __device__ __forceinline__ float gelu_grad(float x) {
 template <int EPI>
 __global__ void __launch_bounds__(64) k_gemm_bf16(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B,
                                                   int ldb, int M, int N, int K, const float* __restrict__ bias,
-                                                  float* __restrict__ Cf, int ldcf, bf16* __restrict__ Cb, int ldcb) {
+                                                  float* __restrict__ Cf, int ldcf, bf16* __restrict__ Cb, int ldcb,
+                                                  int kc, long long split_stride) {
   const int lane = threadIdx.x;
-  const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+  // n-tiles fastest, so that the workgroups sharing a row block of A run
+  // together and find it in L2
+  const int nt = N / 64;
+  const int m0 = (int)(blockIdx.x / nt) * 64, n0 = (int)(blockIdx.x % nt) * 64;
   if (m0 >= M || n0 >= N) return;
+  // split-K: this workgroup sums k in [kbeg, kend) into its own partial
+  const int kbeg = blockIdx.y * kc;
+  const int kend = min(K, kbeg + kc);
+  A += kbeg;
+  B += kbeg;
+  K = max(kend - kbeg, 0);
+  if (EPI == EPI_F32) Cf += (size_t)blockIdx.y * (size_t)split_stride;
   const int r = lane & 15, q = lane >> 4;
   f32x4 acc[4][4];
 #pragma unroll
@@ -63,10 +74,12 @@ __global__ void __launch_bounds__(64) k_gemm_bf16(const bf16* __restrict__ A, in
     Bp[i] = B + (size_t)(n0 + 16 * i + r) * (size_t)ldb + q * 8;
   }
   bf16x8 a_cur[4], b_cur[4];
+  if (K > 0) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    a_cur[i] = *(const bf16x8*)(Ap[i]);
-    b_cur[i] = *(const bf16x8*)(Bp[i]);
+    for (int i = 0; i < 4; ++i) {
+      a_cur[i] = *(const bf16x8*)(Ap[i]);
+      b_cur[i] = *(const bf16x8*)(Bp[i]);
+    }
   }
   for (int k0 = 0; k0 < K; k0 += 32) {
     bf16x8 a_nxt[4], b_nxt[4];
@@ -104,6 +117,8 @@ __global__ void __launch_bounds__(64) k_gemm_bf16(const bf16* __restrict__ A, in
           Cf[(size_t)m * ldcf + n] = v;
         } else if (EPI == EPI_BIAS_F32) {
           Cf[(size_t)m * ldcf + n] = v + bias[n];
+        } else if (EPI == EPI_BIAS_F32_T) {  // output-major: Cf[n][m]
+          Cf[(size_t)n * ldcf + m] = v + bias[n];
         } else {  // EPI_BIAS_GELU: bf16 activation, optional fp32 pre-activation
           float pre = v + bias[n];
           if (Cf) Cf[(size_t)m * ldcf + n] = pre;
@@ -113,12 +128,14 @@ __global__ void __launch_bounds__(64) k_gemm_bf16(const bf16* __restrict__ A, in
     }
 }
 
-template __global__ void k_gemm_bf16<EPI_F32>(const bf16*, int, const bf16*, int, int, int, int, const float*, float*,
-                                             int, bf16*, int);
+template __global__ void k_gemm_bf16<EPI_F32>(const bf16*, int, const bf16*, int, int, int, int, const float*,
+                                             float*, int, bf16*, int, int, long long);
 template __global__ void k_gemm_bf16<EPI_BIAS_F32>(const bf16*, int, const bf16*, int, int, int, int, const float*,
-                                                  float*, int, bf16*, int);
+                                                  float*, int, bf16*, int, int, long long);
+template __global__ void k_gemm_bf16<EPI_BIAS_F32_T>(const bf16*, int, const bf16*, int, int, int, int, const float*,
+                                                    float*, int, bf16*, int, int, long long);
 template __global__ void k_gemm_bf16<EPI_BIAS_GELU>(const bf16*, int, const bf16*, int, int, int, int, const float*,
-                                                   float*, int, bf16*, int);
+                                                   float*, int, bf16*, int, int, long long);
 
 // out[c][r] = in[r][c] for an R x C bf16 matrix (row pitches ldi, ldo).
 __global__ void k_transpose_bf16(const bf16* __restrict__ in, int ldi, bf16* __restrict__ out, int ldo, int R, int C) {
@@ -172,13 +189,32 @@ __global__ void k_mse(const float* __restrict__ Y, const float* __restrict__ T, 
   if (threadIdx.x == 0) atomicAdd(loss, red[0] * inv_count);
 }
 
-// db[n] = sum over rows of d (bf16, M x N, pitch ld).
-__global__ void k_bias_grad(const bf16* __restrict__ d, int M, int N, int ld, float* __restrict__ db) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// Partial column sums of d (bf16, M x N, pitch ld) over the row chunk
+// [blockIdx.y * rows, (blockIdx.y + 1) * rows): part[blockIdx.y * N + n].
+// Block (64, 4): 64 columns, 4 row phases combined in a fixed order.
+__global__ void k_bias_partial(const bf16* __restrict__ d, int M, int N, int ld, int rows, float* __restrict__ part) {
+  __shared__ float red[4][64];
+  const int n = blockIdx.x * 64 + threadIdx.x;
+  const int r0 = blockIdx.y * rows, r1 = min(M, r0 + rows);
   float s = 0.f;
-  for (int m = 0; m < M; ++m) s += bf2f(d[(size_t)m * ld + n]);
-  db[n] = s;
+  if (n < N)
+    for (int m = r0 + threadIdx.y; m < r1; m += 4) s += bf2f(d[(size_t)m * ld + n]);
+  red[threadIdx.y][threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.y == 0 && n < N)
+    part[(size_t)blockIdx.y * N + n] = (red[0][threadIdx.x] + red[1][threadIdx.x]) +
+                                       (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+// out[i] = sum over s < S of part[s * stride + i], in order of s
+// (deterministic split reductions).
+__global__ void k_reduce_splits(const float* __restrict__ part, int S, long long n, long long stride,
+                                float* __restrict__ out) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float a = 0.f;
+  for (int s = 0; s < S; ++s) a += part[(size_t)s * (size_t)stride + (size_t)i];
+  out[i] = a;
 }
 
 // Adam on FP32 master parameters (Kingma & Ba 2015, bias-corrected).

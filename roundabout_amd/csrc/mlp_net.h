@@ -46,6 +46,7 @@ struct Net {
   bf16* dpre = nullptr;
   bf16* dpreT = nullptr;
   float* dloss = nullptr;
+  float* ws = nullptr;  // split-K partials of the weight and bias gradients
   hipStream_t stream = nullptr;
   std::string err;
 
@@ -56,10 +57,13 @@ struct Net {
   int get_params(float* host) const;
   int get_grads(float* host) const;
   size_t n_public() const;
-  // X0 (Mpad x pin[0] bf16) must hold the inputs; Y receives the outputs
-  int forward(int Mpad, bool keep_pre);
-  // T, Wm (Mpad x pout[last]) hold targets and weights; rows >= Mvalid ignored
-  int backward(int Mpad, int Mvalid, double count, float* loss_out);
+  // X0 (Mpad x pin[0] bf16) must hold the inputs; Y receives the outputs,
+  // row-major (Mpad x pout) or, with out_major, output-major (pout x Mpad)
+  int forward(int Mpad, bool keep_pre, bool out_major = false);
+  // T, Wm (Mpad x pout[last]) hold targets and weights; rows >= Mvalid
+  // ignored.  The loss is added to *loss_dev (device, no synchronisation)
+  // when given, else returned in *loss_out (synchronises) when that is given.
+  int backward(int Mpad, int Mvalid, double count, float* loss_out, float* loss_dev = nullptr);
   int adam(float lr, float b1, float b2, float eps);
   void release();
 

@@ -12,11 +12,21 @@ namespace mlp {
 namespace {
 inline dim3 g1(long long n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
 
+// M, N multiples of 64, K of 32; splits > 1 (EPI_F32 only) writes the
+// partial products of k-chunks of length kc to Cf + s * split_stride.
 template <int EPI>
 void gemm(hipStream_t s, const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, const float* bias,
-          float* Cf, int ldcf, bf16* Cb, int ldcb) {
-  dim3 grid((unsigned)(M / 64), (unsigned)(N / 64));
-  hipLaunchKernelGGL(k_gemm_bf16<EPI>, grid, dim3(64), 0, s, A, lda, B, ldb, M, N, K, bias, Cf, ldcf, Cb, ldcb);
+          float* Cf, int ldcf, bf16* Cb, int ldcb, int splits = 1, int kc = 0, long long split_stride = 0) {
+  dim3 grid((unsigned)((long long)(M / 64) * (N / 64)), (unsigned)splits);
+  hipLaunchKernelGGL(k_gemm_bf16<EPI>, grid, dim3(64), 0, s, A, lda, B, ldb, M, N, K, bias, Cf, ldcf, Cb, ldcb,
+                     splits > 1 ? kc : K, split_stride);
+}
+
+// k-splits of a weight-gradient product over Mpad batch rows: at most 32
+// chunks of at least 64 rows, chunk length a multiple of 32
+inline void batch_splits(int Mpad, int* S, int* kc) {
+  *S = std::max(1, std::min(32, Mpad / 64));
+  *kc = round_up((Mpad + *S - 1) / *S, 32);
 }
 
 void transpose(hipStream_t s, const bf16* in, int ldi, bf16* out, int ldo, int R, int C) {
@@ -150,6 +160,8 @@ void Net::free_workspace() {
   F(dXf);
   F(dpre);
   F(dpreT);
+  F(ws);
+  ws = nullptr;
   X0 = X0T = nullptr;
   pre.clear();
   act.clear();
@@ -204,12 +216,17 @@ int Net::ensure_workspace(int Mpad, bool training) {
     NT_TRY(hipMalloc((void**)&dXf, 4 * M * wmax));
     NT_TRY(hipMalloc((void**)&dpre, 2 * M * wmax));
     NT_TRY(hipMalloc((void**)&dpreT, 2 * M * wmax));
+    int S, kc;
+    batch_splits(Mpad, &S, &kc);
+    size_t per = 0;
+    for (int l = 0; l < nl; ++l) per = std::max(per, (size_t)pout[l] * pin[l] + pout[l]);
+    NT_TRY(hipMalloc((void**)&ws, 4 * (size_t)S * per));
   }
   Mcap = Mpad;
   return 0;
 }
 
-int Net::forward(int Mpad, bool keep_pre) {
+int Net::forward(int Mpad, bool keep_pre, bool out_major) {
   const bf16* x = X0;
   int ldx = pin[0];
   for (int l = 0; l < nl; ++l) {
@@ -220,6 +237,8 @@ int Net::forward(int Mpad, bool keep_pre) {
                           act[l], pout[l]);
       x = act[l];
       ldx = pout[l];
+    } else if (out_major) {
+      gemm<EPI_BIAS_F32_T>(stream, x, ldx, W, pin[l], Mpad, pout[l], pin[l], b, Y, Mpad, nullptr, 0);
     } else {
       gemm<EPI_BIAS_F32>(stream, x, ldx, W, pin[l], Mpad, pout[l], pin[l], b, Y, pout[l], nullptr, 0);
     }
@@ -228,13 +247,16 @@ int Net::forward(int Mpad, bool keep_pre) {
   return 0;
 }
 
-int Net::backward(int Mpad, int Mvalid, double count, float* loss_out) {
+int Net::backward(int Mpad, int Mvalid, double count, float* loss_out, float* loss_dev) {
   const int L = nl - 1;
   const int po = pout[L];
-  NT_TRY(hipMemsetAsync(dloss, 0, 4, stream));
+  float* lacc = loss_dev ? loss_dev : dloss;
+  if (!loss_dev) NT_TRY(hipMemsetAsync(dloss, 0, 4, stream));
   const float inv = count > 0 ? (float)(1.0 / count) : 0.f;
   hipLaunchKernelGGL(k_mse, g1((long long)Mpad * po, 256), dim3(256), 0, stream, Y, T, Wm, Mpad, Mvalid, dout[L], po,
-                     inv, dY, dloss);
+                     inv, dY, lacc);
+  int S, kc;
+  batch_splits(Mpad, &S, &kc);
   // d (bf16, Mpad x pout[l]) is the gradient w.r.t. layer l's output (pre-activation)
   bf16* d = dY;
   for (int l = L; l >= 0; --l) {
@@ -244,8 +266,16 @@ int Net::backward(int Mpad, int Mvalid, double count, float* loss_out) {
     // dW_l = d^T x_in : (o x Mpad) (i x Mpad)^T
     transpose(stream, d, o, dYT, Mpad, Mpad, o);
     transpose(stream, xin, i, xinT, Mpad, Mpad, i);
-    gemm<EPI_F32>(stream, dYT, Mpad, xinT, Mpad, o, i, Mpad, nullptr, g + offW[l], i, nullptr, 0);
-    hipLaunchKernelGGL(k_bias_grad, g1(o, 256), dim3(256), 0, stream, d, Mpad, o, o, g + offB[l]);
+    // split over the batch (K = Mpad is long, the output small), then a
+    // fixed-order sum of the partials
+    const long long oi = (long long)o * i;
+    gemm<EPI_F32>(stream, dYT, Mpad, xinT, Mpad, o, i, Mpad, nullptr, ws, i, nullptr, 0, S, kc, oi);
+    hipLaunchKernelGGL(k_reduce_splits, g1(oi, 256), dim3(256), 0, stream, ws, S, oi, oi, g + offW[l]);
+    float* bpart = ws + (size_t)S * (size_t)oi;
+    hipLaunchKernelGGL(k_bias_partial, dim3((unsigned)(o / 64), (unsigned)S), dim3(64, 4), 0, stream, d, Mpad, o, o,
+                       kc, bpart);
+    hipLaunchKernelGGL(k_reduce_splits, g1(o, 256), dim3(256), 0, stream, bpart, S, (long long)o, (long long)o,
+                       g + offB[l]);
     if (l > 0) {
       // dX = d W_l : (Mpad x o) (i x o)^T, then through GELU of layer l-1
       gemm<EPI_F32>(stream, d, o, WTb + offWb[l], o, Mpad, i, o, nullptr, dXf, i, nullptr, 0);
@@ -257,7 +287,7 @@ int Net::backward(int Mpad, int Mvalid, double count, float* loss_out) {
     }
   }
   NT_TRY(hipGetLastError());
-  if (loss_out) {
+  if (loss_out && !loss_dev) {
     NT_TRY(hipMemcpyAsync(loss_out, dloss, 4, hipMemcpyDeviceToHost, stream));
     NT_TRY(hipStreamSynchronize(stream));
   }

@@ -1,0 +1,650 @@
+// surrogate.hip -- the learned closure of the column engine: training data
+// from the plain path, feature and guard kernels, normalisation, the Adam
+// training loop, guarded inference feeding the FP64 solve, save/load.
+//
+// Features per wet column (5 nz + 6): T, S, centred u, centred v and
+// log10 h at every level, then |tau| (+ shelf stress), Q_heat, Q_salt,
+// Coriolis f, the lagged KPP boundary-layer depth and the live column depth.
+// T, S, u and v enter as differences to the live level above (the surface
+// level, or one under a vanished level, as its value; 0 on vanished
+// levels): an invertible recoding of the profile that keeps the
+// level-to-level contrasts, which set N^2, shear and Ri, above the BF16
+// rounding of the normalised inputs (raw values would bury a 0.1 K step
+// under the 8-bit mantissa of a profile spanning 25 K).
+// Targets (2 (nz-1) + 1): log10 kt and log10 kv at the interior interfaces
+// 1..nz-1 after the plain coefficient build, and the new boundary-layer
+// depth.  Interfaces below the first live layer carry weight 0 (they keep
+// their plain filler value at inference).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "colphys_handle.h"
+#include "mlp_net.h"
+
+namespace colphys {
+
+using mlp::bf16;
+
+struct SurrState {
+  int nz = 0, din = 0, dout = 0;
+  mlp::Net net;
+  bool have_net = false;
+  std::vector<int> widths;
+  // raw dataset on the device
+  float* Xraw = nullptr;
+  float* Traw = nullptr;
+  float* Wraw = nullptr;
+  float* rowsum = nullptr;
+  long long n = 0, cap = 0;
+  // normalised training set
+  bf16* Xn = nullptr;
+  float* Tn = nullptr;
+  float* Wn = nullptr;
+  unsigned* idx = nullptr;
+  // statistics
+  std::vector<float> fmean, fstd, fmin, fmax, tmean, tstd;
+  float* d_stats = nullptr;  // fmean | fstd | fmin | fmax | tmean | tstd
+  // inference
+  unsigned char* guard = nullptr;
+  unsigned int* nfall = nullptr;
+  int guard_mode = RDB_COLPHYS_GUARD_STRICT;
+  int inf_rows = 0;
+  ~SurrState();
+  void free_dataset();
+};
+
+namespace {
+
+#define LV(arr, k) (arr)[(size_t)(k) * (size_t)P + (size_t)p]
+
+int perr(int st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  return rdb_colphys_push_error(st, buf);
+}
+
+#define SG_TRY(expr)                                                                                \
+  do {                                                                                              \
+    hipError_t _e = (expr);                                                                         \
+    if (_e != hipSuccess) return perr(RDB_COLPHYS_ERR_SETUP, "%s: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+__device__ __forceinline__ bf16 f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16)(u >> 16);
+}
+
+// feature f of packed column p (raw, un-normalised)
+__device__ __forceinline__ float feature(const StateView& s, const double* cor, int p, int f, double depth) {
+  const int P = s.P, nz = s.nz;
+  const int blk = f / nz, k = f % nz;
+  if (blk < 5) {
+    const double hk = LV(s.h, k);
+    if (blk == 4) return (float)log10(fmax(hk, 1e-30));
+    if (!(hk > H_VANISHED)) return 0.f;
+    const double* a = blk == 0 ? s.T : blk == 1 ? s.S : blk == 2 ? s.ub : s.vb;
+    double v = LV(a, k);
+    if (k + 1 < nz && LV(s.h, k + 1) > H_VANISHED) v -= LV(a, k + 1);
+    return (float)v;
+  }
+  switch (f - 5 * nz) {
+    case 0: return (float)s.tau_tot[p];
+    case 1: return (float)s.Qh[p];
+    case 2: return (float)s.Qs[p];
+    case 3: return (float)cor[p];
+    case 4: return (float)s.bl[p];
+    default: return (float)depth;
+  }
+}
+
+__device__ __forceinline__ double live_depth(const StateView& s, int p) {
+  const int P = s.P;
+  double d = 0.0;
+  for (int k = s.kb[p]; k < s.nz; ++k) {
+    double hk = LV(s.h, k);
+    if (hk > H_VANISHED) d += hk;
+  }
+  return d;
+}
+
+// Raw features of packed columns c = i * stride (i < ncols) into rows row0 + i.
+__global__ void k_features_raw(StateView s, const double* cor, int ncols, int stride, int din, float* X, long long row0) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncols) return;
+  int p = i * stride;
+  double depth = live_depth(s, p);
+  float* row = X + (size_t)(row0 + i) * (size_t)din;
+  for (int f = 0; f < din; ++f) row[f] = feature(s, cor, p, f, depth);
+}
+
+// Targets of the same columns after the plain step: log10 kt, log10 kv at
+// interfaces 1..nz-1, bl_depth; weights 1 on interfaces >= kb.
+__global__ void k_targets_raw(StateView s, int ncols, int stride, int dout, float* T, float* W, float* rowsum,
+                              long long row0) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncols) return;
+  const int P = s.P, nz = s.nz;
+  int p = i * stride;
+  const int kb = s.kb[p];
+  float* t = T + (size_t)(row0 + i) * (size_t)dout;
+  float* w = W + (size_t)(row0 + i) * (size_t)dout;
+  float sum = 0.f;
+  for (int k = 1; k < nz; ++k) {
+    float wk = (k >= kb) ? 1.f : 0.f;
+    t[k - 1] = (float)log10(fmax(LV(s.kt, k), 1e-30));
+    t[nz - 1 + k - 1] = (float)log10(fmax(LV(s.kv, k), 1e-30));
+    w[k - 1] = wk;
+    w[nz - 1 + k - 1] = wk;
+    sum += 2.f * wk;
+  }
+  t[2 * (nz - 1)] = (float)s.bl[p];
+  w[2 * (nz - 1)] = 1.f;
+  rowsum[row0 + i] = sum + 1.f;
+}
+
+// Per-column mean/std (weighted when W is given) and min/max of an n x d
+// row-major matrix; one block per column.
+__global__ void k_col_stats(const float* X, const float* W, long long n, int d, float* mean, float* stdv, float* mn,
+                            float* mx) {
+  __shared__ double s1[256], s2[256], sw[256];
+  __shared__ float smn[256], smx[256];
+  const int j = blockIdx.x;
+  double a = 0, b = 0, c = 0;
+  float lo = 3.4e38f, hi = -3.4e38f;
+  for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+    float w = W ? W[(size_t)i * d + j] : 1.f;
+    if (w > 0.f) {
+      double v = X[(size_t)i * d + j];
+      a += w * v;
+      b += w * v * v;
+      c += w;
+      lo = fminf(lo, (float)v);
+      hi = fmaxf(hi, (float)v);
+    }
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  sw[threadIdx.x] = c;
+  smn[threadIdx.x] = lo;
+  smx[threadIdx.x] = hi;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      s1[threadIdx.x] += s1[threadIdx.x + st];
+      s2[threadIdx.x] += s2[threadIdx.x + st];
+      sw[threadIdx.x] += sw[threadIdx.x + st];
+      smn[threadIdx.x] = fminf(smn[threadIdx.x], smn[threadIdx.x + st]);
+      smx[threadIdx.x] = fmaxf(smx[threadIdx.x], smx[threadIdx.x + st]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double cnt = sw[0] > 0 ? sw[0] : 1.0;
+    double mu = s1[0] / cnt;
+    double var = s2[0] / cnt - mu * mu;
+    mean[j] = (float)mu;
+    stdv[j] = (float)sqrt(fmax(var, 1e-12));
+    if (mn) mn[j] = smn[0];
+    if (mx) mx[j] = smx[0];
+  }
+}
+
+__global__ void k_norm_features(const float* X, long long n, int din, const float* mean, const float* stdv, bf16* Xn,
+                                int ld) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * ld) return;
+  long long r = i / ld;
+  int c = (int)(i % ld);
+  Xn[i] = f2bf(c < din ? (X[(size_t)r * din + c] - mean[c]) / stdv[c] : 0.f);
+}
+
+__global__ void k_norm_targets(const float* T, const float* W, long long n, int dout, const float* mean,
+                               const float* stdv, float* Tn, float* Wn, int ld) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * ld) return;
+  long long r = i / ld;
+  int c = (int)(i % ld);
+  bool in = c < dout;
+  Tn[i] = in ? (T[(size_t)r * dout + c] - mean[c]) / stdv[c] : 0.f;
+  Wn[i] = in ? W[(size_t)r * dout + c] : 0.f;
+}
+
+// rows idx[b] of the normalised set into a training batch (rows >= nb zero)
+__global__ void k_gather(const bf16* Xn, int ldx, const float* Tn, const float* Wn, int ldt, const unsigned* idx,
+                         int nb, bf16* X0, float* T, float* W) {
+  const int b = blockIdx.x;
+  const bool valid = b < nb;
+  const size_t src = valid ? (size_t)idx[b] : 0;
+  for (int c = threadIdx.x; c < ldx; c += blockDim.x) X0[(size_t)b * ldx + c] = valid ? Xn[src * ldx + c] : (bf16)0;
+  for (int c = threadIdx.x; c < ldt; c += blockDim.x) {
+    T[(size_t)b * ldt + c] = valid ? Tn[src * ldt + c] : 0.f;
+    W[(size_t)b * ldt + c] = valid ? Wn[src * ldt + c] : 0.f;
+  }
+}
+
+// Inference features of wet columns [0, n): normalised bf16 rows written
+// through an LDS tile so that each row is stored with coalesced writes, and
+// the per-column guard (non-zero: plain path).
+constexpr int FT = 64;
+__global__ void __launch_bounds__(FT) k_features_infer(StateView s, const double* cor, int n, int din, int ld,
+                                                     const float* st, bf16* X0, unsigned char* guard,
+                                                     unsigned int* nfall, int guard_mode, double rho0, double p_ref) {
+  __shared__ bf16 tile[384][FT + 1];
+  const float* mean = st;
+  const float* stdv = st + din;
+  const float* mn = st + 2 * din;
+  const float* mx = st + 3 * din;
+  const int tid = threadIdx.x;
+  const int p = blockIdx.x * FT + tid;
+  const int P = s.P, nz = s.nz;
+  if (p < n) {
+    double depth = live_depth(s, p);
+    bool out_of_range = false;
+    for (int f = 0; f < din; ++f) {
+      float v = feature(s, cor, p, f, depth);
+      out_of_range |= (v < mn[f] || v > mx[f]);
+      tile[f][tid] = f2bf((v - mean[f]) / stdv[f]);
+    }
+    for (int f = din; f < ld; ++f) tile[f][tid] = 0;
+    // static stability: N^2 < 0 where density increases upward between two
+    // live layers (N^2 = -g (rho_k - rho_km1) / (rho0 dz), A3's sign)
+    bool unstable = false;
+    const int kb = s.kb[p];
+    int k_sfc = nz - 1;
+    for (int k = nz - 1; k >= kb; --k)
+      if (LV(s.h, k) > H_VANISHED) { k_sfc = k; break; }
+    if (guard_mode == RDB_COLPHYS_GUARD_STRICT) {
+      double rho_m = 0.0;
+      bool live_m = false;
+      for (int k = kb; k < nz; ++k) {
+        double hk = LV(s.h, k);
+        bool live = hk > H_VANISHED;
+        double rho = live ? eos_wright<double>(LV(s.T, k), LV(s.S, k), hk, rho0, p_ref) : 0.0;
+        if (live && live_m && rho > rho_m) unstable = true;
+        rho_m = rho;
+        live_m = live;
+      }
+    } else if (guard_mode == RDB_COLPHYS_GUARD_SURFACE && k_sfc > kb) {
+      double h1 = LV(s.h, k_sfc), h0 = LV(s.h, k_sfc - 1);
+      if (h1 > H_VANISHED && h0 > H_VANISHED) {
+        double r1 = eos_wright<double>(LV(s.T, k_sfc), LV(s.S, k_sfc), h1, rho0, p_ref);
+        double r0 = eos_wright<double>(LV(s.T, k_sfc - 1), LV(s.S, k_sfc - 1), h0, rho0, p_ref);
+        unstable = r1 > r0;
+      }
+    }
+    unsigned char gflag = (out_of_range ? 1 : 0) | (unstable ? 2 : 0);
+    guard[p] = gflag;
+    if (gflag) atomicAdd(nfall, 1u);
+  }
+  __syncthreads();
+  // cooperative row stores: the FT rows of this block, 2 bf16 per lane step
+  const int rows = min(FT, n - blockIdx.x * FT);
+  for (int r = 0; r < rows; ++r) {
+    uint32_t* dst = (uint32_t*)(X0 + (size_t)(blockIdx.x * FT + r) * (size_t)ld);
+    for (int c2 = tid; c2 < ld / 2; c2 += FT) {
+      uint32_t lo = tile[2 * c2][r], hi = tile[2 * c2 + 1][r];
+      dst[c2] = lo | (hi << 16);
+    }
+  }
+}
+
+inline dim3 gs(long long n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
+
+}  // namespace
+
+SurrState::~SurrState() {
+  free_dataset();
+  if (d_stats) (void)hipFree(d_stats);
+  if (guard) (void)hipFree(guard);
+  if (nfall) (void)hipFree(nfall);
+}
+
+void SurrState::free_dataset() {
+  for (void* q : {(void*)Xraw, (void*)Traw, (void*)Wraw, (void*)rowsum, (void*)Xn, (void*)Tn, (void*)Wn, (void*)idx})
+    if (q) (void)hipFree(q);
+  Xraw = Traw = Wraw = rowsum = Tn = Wn = nullptr;
+  Xn = nullptr;
+  idx = nullptr;
+  n = cap = 0;
+}
+
+void surr_free(SurrState* s) { delete s; }
+
+static SurrState* surr_get(rdb_colphys* h) {
+  if (!h->surr) {
+    h->surr = new SurrState();
+    h->surr->nz = h->nz;
+    h->surr->din = 5 * h->nz + 6;
+    h->surr->dout = 2 * (h->nz - 1) + 1;
+  }
+  return h->surr;
+}
+
+int surr_collect(rdb_colphys* h, double dt, int n_steps, int stride, long long* n_out) {
+  if (h->nz < 2) return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "mlp_collect: needs nz >= 2");
+  if (5 * h->nz + 6 > 384) return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "mlp_collect: nz > 75 exceeds the feature tile");
+  if (stride < 1) stride = 1;
+  SurrState* S = surr_get(h);
+  const int ncols = (h->n_wet + stride - 1) / stride;
+  const long long need = S->n + (long long)ncols * n_steps;
+  if (need > S->cap) {
+    // grow (copying what is already collected)
+    const long long cap = std::max(need, S->cap * 2);
+    float *X = nullptr, *T = nullptr, *W = nullptr, *R = nullptr;
+    SG_TRY(hipMalloc((void**)&X, 4 * (size_t)cap * S->din));
+    SG_TRY(hipMalloc((void**)&T, 4 * (size_t)cap * S->dout));
+    SG_TRY(hipMalloc((void**)&W, 4 * (size_t)cap * S->dout));
+    SG_TRY(hipMalloc((void**)&R, 4 * (size_t)cap));
+    if (S->n) {
+      SG_TRY(hipMemcpyAsync(X, S->Xraw, 4 * (size_t)S->n * S->din, hipMemcpyDeviceToDevice, h->stream));
+      SG_TRY(hipMemcpyAsync(T, S->Traw, 4 * (size_t)S->n * S->dout, hipMemcpyDeviceToDevice, h->stream));
+      SG_TRY(hipMemcpyAsync(W, S->Wraw, 4 * (size_t)S->n * S->dout, hipMemcpyDeviceToDevice, h->stream));
+      SG_TRY(hipMemcpyAsync(R, S->rowsum, 4 * (size_t)S->n, hipMemcpyDeviceToDevice, h->stream));
+      SG_TRY(hipStreamSynchronize(h->stream));
+    }
+    long long n_keep = S->n;
+    S->free_dataset();
+    S->Xraw = X;
+    S->Traw = T;
+    S->Wraw = W;
+    S->rowsum = R;
+    S->cap = cap;
+    S->n = n_keep;
+  }
+  int st = refresh_static(h, dt);
+  if (st) return st;
+  for (int i = 0; i < n_steps; ++i) {
+    hipLaunchKernelGGL(k_features_raw, gs(ncols, 128), dim3(128), 0, h->stream, h->s, h->d_coriolis, ncols, stride,
+                       S->din, S->Xraw, S->n);
+    st = launch_step(h, dt, 0, h->n_wet);
+    if (st) return st;
+    hipLaunchKernelGGL(k_targets_raw, gs(ncols, 128), dim3(128), 0, h->stream, h->s, ncols, stride, S->dout, S->Traw,
+                       S->Wraw, S->rowsum, S->n);
+    SG_TRY(hipGetLastError());
+    S->n += ncols;
+    h->steps += 1;
+  }
+  SG_TRY(hipStreamSynchronize(h->stream));
+  if (n_out) *n_out = S->n;
+  return RDB_COLPHYS_OK;
+}
+
+int surr_train(rdb_colphys* h, const rdb_colphys_mlp_config* cfg, float* train_loss, float* val_loss, int n_hist,
+               int* n_done) {
+  SurrState* S = h->surr;
+  if (!S || S->n < 64) return perr(RDB_COLPHYS_ERR_NOT_INITIALISED, "mlp_train: collect training columns first");
+  if (!cfg || cfg->batch < 64 || cfg->hidden < 1 || cfg->n_hidden < 1 || cfg->epochs < 0)
+    return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "mlp_train: bad configuration");
+  const long long N = S->n;
+  const int din = S->din, dout = S->dout;
+  // statistics over the whole collected set
+  std::vector<float> stats(4 * (size_t)din + 2 * (size_t)dout);
+  float* dst = nullptr;
+  SG_TRY(hipMalloc((void**)&dst, 4 * stats.size()));
+  hipLaunchKernelGGL(k_col_stats, dim3(din), dim3(256), 0, h->stream, S->Xraw, (const float*)nullptr, N, din, dst,
+                     dst + din, dst + 2 * din, dst + 3 * din);
+  hipLaunchKernelGGL(k_col_stats, dim3(dout), dim3(256), 0, h->stream, S->Traw, S->Wraw, N, dout, dst + 4 * din,
+                     dst + 4 * din + dout, (float*)nullptr, (float*)nullptr);
+  SG_TRY(hipGetLastError());
+  SG_TRY(hipMemcpyAsync(stats.data(), dst, 4 * stats.size(), hipMemcpyDeviceToHost, h->stream));
+  SG_TRY(hipStreamSynchronize(h->stream));
+  if (S->d_stats) (void)hipFree(S->d_stats);
+  S->d_stats = dst;
+  S->fmean.assign(stats.begin(), stats.begin() + din);
+  S->fstd.assign(stats.begin() + din, stats.begin() + 2 * din);
+  S->fmin.assign(stats.begin() + 2 * din, stats.begin() + 3 * din);
+  S->fmax.assign(stats.begin() + 3 * din, stats.begin() + 4 * din);
+  S->tmean.assign(stats.begin() + 4 * din, stats.begin() + 4 * din + dout);
+  S->tstd.assign(stats.begin() + 4 * din + dout, stats.end());
+  // network
+  std::vector<int> widths{din};
+  for (int l = 0; l < cfg->n_hidden; ++l) widths.push_back(cfg->hidden);
+  widths.push_back(dout);
+  if (!S->have_net || S->widths != widths) {
+    int e = S->net.init(widths, h->stream);
+    if (e) return perr(e, "mlp_train: %s", S->net.err.c_str());
+    std::mt19937_64 rng((uint64_t)cfg->seed);
+    std::vector<float> p(S->net.n_public());
+    size_t q = 0;
+    for (size_t l = 0; l + 1 < widths.size(); ++l) {
+      const double lim = std::sqrt(6.0 / (widths[l] + widths[l + 1]));
+      std::uniform_real_distribution<double> u(-lim, lim);
+      for (long long i = 0; i < (long long)widths[l] * widths[l + 1]; ++i) p[q++] = (float)u(rng);
+      for (int i = 0; i < widths[l + 1]; ++i) p[q++] = 0.f;
+    }
+    e = S->net.set_params(p.data());
+    if (e) return perr(e, "mlp_train: %s", S->net.err.c_str());
+    S->widths = widths;
+    S->have_net = true;
+  }
+  mlp::Net& net = S->net;
+  const int ldx = net.pin[0], ldt = net.pout[net.nl - 1];
+  // normalised set
+  if (S->Xn) (void)hipFree(S->Xn);
+  if (S->Tn) (void)hipFree(S->Tn);
+  if (S->Wn) (void)hipFree(S->Wn);
+  if (S->idx) (void)hipFree(S->idx);
+  SG_TRY(hipMalloc((void**)&S->Xn, 2 * (size_t)N * ldx));
+  SG_TRY(hipMalloc((void**)&S->Tn, 4 * (size_t)N * ldt));
+  SG_TRY(hipMalloc((void**)&S->Wn, 4 * (size_t)N * ldt));
+  SG_TRY(hipMalloc((void**)&S->idx, 4 * (size_t)N));
+  hipLaunchKernelGGL(k_norm_features, gs(N * ldx, 256), dim3(256), 0, h->stream, S->Xraw, N, din, dst, dst + din,
+                     S->Xn, ldx);
+  hipLaunchKernelGGL(k_norm_targets, gs(N * ldt, 256), dim3(256), 0, h->stream, S->Traw, S->Wraw, N, dout,
+                     dst + 4 * din, dst + 4 * din + dout, S->Tn, S->Wn, ldt);
+  SG_TRY(hipGetLastError());
+  std::vector<float> rowsum((size_t)N);
+  SG_TRY(hipMemcpyAsync(rowsum.data(), S->rowsum, 4 * (size_t)N, hipMemcpyDeviceToHost, h->stream));
+  SG_TRY(hipStreamSynchronize(h->stream));
+  // split: a held-out share for validation
+  std::mt19937_64 rng((uint64_t)cfg->seed * 7919u + 17u);
+  std::vector<unsigned> perm((size_t)N);
+  for (long long i = 0; i < N; ++i) perm[(size_t)i] = (unsigned)i;
+  std::shuffle(perm.begin(), perm.end(), rng);
+  const long long n_val = std::min<long long>(N / 2, (long long)std::llround(cfg->val_fraction * (double)N));
+  std::vector<unsigned> val(perm.begin(), perm.begin() + n_val), train(perm.begin() + n_val, perm.end());
+  const int B = cfg->batch;
+  const int Mpad = mlp::round_up(B, 64);
+  int e = net.ensure_workspace(Mpad, true);
+  if (e) return perr(e, "mlp_train: %s", net.err.c_str());
+  const size_t nb_train = train.size() / (size_t)B, nb_val = (val.size() + (size_t)B - 1) / (size_t)B;
+  if (cfg->epochs > 0 && nb_train == 0)
+    return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "mlp_train: %zu training rows do not fill one batch of %d",
+                train.size(), B);
+  // per-batch loss slots on the device: one read-back per epoch
+  float* dl = nullptr;
+  SG_TRY(hipMalloc((void**)&dl, 4 * (nb_train + nb_val + 1)));
+  std::vector<float> lh(nb_train + nb_val);
+  std::vector<double> cnt(nb_train + nb_val);
+  // the validation rows sit after the training rows in S->idx
+  SG_TRY(hipMemcpyAsync(S->idx + train.size(), val.data(), 4 * val.size(), hipMemcpyHostToDevice, h->stream));
+  auto run_batch = [&](size_t off, const unsigned* rows, int nb, bool update, float* slot, double* count) -> int {
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)Mpad), dim3(128), 0, h->stream, S->Xn, ldx, S->Tn, S->Wn, ldt,
+                       S->idx + off, nb, net.X0, net.T, net.Wm);
+    double c = 0;
+    for (int b = 0; b < nb; ++b) c += rowsum[rows[b]];
+    *count = c;
+    int r = net.forward(Mpad, true);
+    if (r) return perr(r, "mlp_train: %s", net.err.c_str());
+    r = net.backward(Mpad, nb, c, nullptr, slot);  // also the loss (its gradient is unused for validation)
+    if (r) return perr(r, "mlp_train: %s", net.err.c_str());
+    if (update) {
+      r = net.adam((float)cfg->lr, (float)cfg->beta1, (float)cfg->beta2, (float)cfg->eps);
+      if (r) return perr(r, "mlp_train: %s", net.err.c_str());
+    }
+    return 0;
+  };
+  int done = 0;
+  for (int ep = 0; ep < cfg->epochs; ++ep) {
+    std::shuffle(train.begin(), train.end(), rng);
+    e = 0;
+    if (hipMemcpyAsync(S->idx, train.data(), 4 * train.size(), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemsetAsync(dl, 0, 4 * (nb_train + nb_val), h->stream) != hipSuccess)
+      e = perr(RDB_COLPHYS_ERR_SETUP, "mlp_train: batch index upload failed");
+    for (size_t q = 0; !e && q < nb_train; ++q)
+      e = run_batch(q * B, train.data() + q * B, B, true, dl + q, &cnt[q]);
+    for (size_t q = 0; !e && q < nb_val; ++q) {
+      const size_t b0 = q * B;
+      const int nb = (int)std::min<size_t>((size_t)B, val.size() - b0);
+      e = run_batch(train.size() + b0, val.data() + b0, nb, false, dl + nb_train + q, &cnt[nb_train + q]);
+    }
+    if (!e && (hipMemcpyAsync(lh.data(), dl, 4 * lh.size(), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+               hipStreamSynchronize(h->stream) != hipSuccess))
+      e = perr(RDB_COLPHYS_ERR_SETUP, "mlp_train: loss read-back failed");
+    if (e) {
+      (void)hipStreamSynchronize(h->stream);
+      (void)hipFree(dl);
+      return e;
+    }
+    double ls = 0, cs = 0, vs = 0, vc = 0;
+    for (size_t q = 0; q < nb_train; ++q) ls += (double)lh[q] * cnt[q], cs += cnt[q];
+    for (size_t q = nb_train; q < lh.size(); ++q) vs += (double)lh[q] * cnt[q], vc += cnt[q];
+    if (done < n_hist) {
+      if (train_loss) train_loss[done] = cs > 0 ? (float)(ls / cs) : 0.f;
+      if (val_loss) val_loss[done] = vc > 0 ? (float)(vs / vc) : 0.f;
+    }
+    ++done;
+  }
+  SG_TRY(hipStreamSynchronize(h->stream));
+  SG_TRY(hipFree(dl));
+  if (n_done) *n_done = done;
+  S->guard_mode = cfg->guard_mode;
+  return RDB_COLPHYS_OK;
+}
+
+int surr_step(rdb_colphys* h, double dt) {
+  SurrState* S = h->surr;
+  if (!S || !S->have_net || !S->d_stats) return perr(RDB_COLPHYS_ERR_NOT_INITIALISED, "surrogate: no trained network");
+  mlp::Net& net = S->net;
+  if (net.pin[0] > 384) return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "surrogate: input width exceeds the feature tile");
+  if (h->n_passive > 4) return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "surrogate: at most 4 passive tracers");
+  const int Mpad = mlp::round_up(std::max(h->n_wet, 1), 64);
+  if (S->inf_rows != Mpad) {
+    int e = net.ensure_workspace(Mpad, false);
+    if (e) return perr(e, "surrogate: %s", net.err.c_str());
+    SG_TRY(hipMemsetAsync(net.X0, 0, 2 * (size_t)Mpad * net.pin[0], h->stream));
+    if (S->guard) (void)hipFree(S->guard);
+    SG_TRY(hipMalloc((void**)&S->guard, (size_t)Mpad));
+    if (!S->nfall) SG_TRY(hipMalloc((void**)&S->nfall, 64));
+    S->inf_rows = Mpad;
+  }
+  SG_TRY(hipMemsetAsync(S->nfall, 0, 4, h->stream));
+  hipLaunchKernelGGL(k_features_infer, gs(h->n_wet, FT), dim3(FT), 0, h->stream, h->s, h->d_coriolis, h->n_wet,
+                     S->din, net.pin[0], S->d_stats, net.X0, S->guard, S->nfall, S->guard_mode, h->pr.rho0,
+                     h->pr.p_ref);
+  SG_TRY(hipGetLastError());
+  int e = net.forward(Mpad, false, true);
+  if (e) return perr(e, "surrogate: %s", net.err.c_str());
+  const SurrView sv{net.Y, (long long)Mpad, S->d_stats + 4 * S->din, S->d_stats + 4 * S->din + S->dout, S->guard};
+  const bool two = (h->cfg.kt_bg != h->cfg.ks_bg);
+  const int ntr = h->n_passive;
+  dim3 g((unsigned)((h->n_wet + h->block - 1) / h->block)), b((unsigned)h->block);
+  const int M = h->lds_levels;
+  const size_t shmem = (size_t)h->block * (size_t)M * (size_t)(3 + std::min(ntr, 4) + (two ? 1 : 0)) * 8;
+#define LAUNCH(N, TF)                                                                                              \
+  hipLaunchKernelGGL((k_column_step<double, N, TF, true>), g, b, shmem, h->stream, h->s, h->pr, dt, 0, h->n_wet, \
+                     h->d_guard, M, sv)
+#define BY_NTR(TF)                  \
+  switch (ntr) {                    \
+    case 0: LAUNCH(0, TF); break;   \
+    case 1: LAUNCH(1, TF); break;   \
+    case 2: LAUNCH(2, TF); break;   \
+    case 3: LAUNCH(3, TF); break;   \
+    default: LAUNCH(4, TF); break;  \
+  }
+  if (two) {
+    BY_NTR(true)
+  } else {
+    BY_NTR(false)
+  }
+#undef BY_NTR
+#undef LAUNCH
+  SG_TRY(hipGetLastError());
+  unsigned int nf = 0;
+  SG_TRY(hipMemcpyAsync(&nf, S->nfall, 4, hipMemcpyDeviceToHost, h->stream));
+  SG_TRY(hipStreamSynchronize(h->stream));
+  h->last_fallback = nf;
+  return RDB_COLPHYS_OK;
+}
+
+int surr_save(rdb_colphys* h, const char* path) {
+  SurrState* S = h->surr;
+  if (!S || !S->have_net || S->fmean.empty()) return perr(RDB_COLPHYS_ERR_NOT_INITIALISED, "mlp_save: no trained network");
+  std::vector<float> p(S->net.n_public());
+  if (S->net.get_params(p.data())) return perr(RDB_COLPHYS_ERR_SETUP, "mlp_save: %s", S->net.err.c_str());
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return perr(RDB_COLPHYS_ERR_IO, "mlp_save: cannot open %s", path);
+  const char magic[8] = {'R', 'D', 'B', 'S', 'U', 'R', 'R', '1'};
+  int hdr[4] = {S->nz, S->din, S->dout, (int)S->widths.size()};
+  bool ok = std::fwrite(magic, 1, 8, f) == 8 && std::fwrite(hdr, 4, 4, f) == 4 &&
+            std::fwrite(S->widths.data(), 4, S->widths.size(), f) == S->widths.size() &&
+            std::fwrite(p.data(), 4, p.size(), f) == p.size();
+  for (const std::vector<float>* v : {&S->fmean, &S->fstd, &S->fmin, &S->fmax, &S->tmean, &S->tstd})
+    ok = ok && std::fwrite(v->data(), 4, v->size(), f) == v->size();
+  ok = std::fclose(f) == 0 && ok;
+  return ok ? RDB_COLPHYS_OK : perr(RDB_COLPHYS_ERR_IO, "mlp_save: write to %s failed", path);
+}
+
+int surr_load(rdb_colphys* h, const char* path) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return perr(RDB_COLPHYS_ERR_IO, "mlp_load: cannot open %s", path);
+  char magic[8];
+  int hdr[4];
+  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "RDBSURR1", 8) == 0 && std::fread(hdr, 4, 4, f) == 4;
+  if (!ok || hdr[0] != h->nz || hdr[3] < 2 || hdr[3] > 16) {
+    std::fclose(f);
+    return perr(RDB_COLPHYS_ERR_BAD_SHAPE, "mlp_load: %s is not a network for nz = %d", path, h->nz);
+  }
+  SurrState* S = surr_get(h);
+  std::vector<int> widths((size_t)hdr[3]);
+  ok = std::fread(widths.data(), 4, widths.size(), f) == widths.size() && widths.front() == S->din &&
+       widths.back() == S->dout;
+  int e = ok ? S->net.init(widths, h->stream) : 2;
+  std::vector<float> p(ok && !e ? S->net.n_public() : 0);
+  ok = ok && !e && std::fread(p.data(), 4, p.size(), f) == p.size();
+  S->fmean.resize(S->din);
+  S->fstd.resize(S->din);
+  S->fmin.resize(S->din);
+  S->fmax.resize(S->din);
+  S->tmean.resize(S->dout);
+  S->tstd.resize(S->dout);
+  for (std::vector<float>* v : {&S->fmean, &S->fstd, &S->fmin, &S->fmax, &S->tmean, &S->tstd})
+    ok = ok && std::fread(v->data(), 4, v->size(), f) == v->size();
+  std::fclose(f);
+  if (!ok) return perr(RDB_COLPHYS_ERR_IO, "mlp_load: %s is truncated or malformed", path);
+  e = S->net.set_params(p.data());
+  if (e) return perr(e, "mlp_load: %s", S->net.err.c_str());
+  S->widths = widths;
+  S->have_net = true;
+  std::vector<float> stats;
+  for (const std::vector<float>* v : {&S->fmean, &S->fstd, &S->fmin, &S->fmax, &S->tmean, &S->tstd})
+    stats.insert(stats.end(), v->begin(), v->end());
+  if (S->d_stats) (void)hipFree(S->d_stats);
+  S->d_stats = nullptr;
+  SG_TRY(hipMalloc((void**)&S->d_stats, 4 * stats.size()));
+  SG_TRY(hipMemcpy(S->d_stats, stats.data(), 4 * stats.size(), hipMemcpyHostToDevice));
+  S->inf_rows = 0;
+  return RDB_COLPHYS_OK;
+}
+
+int surr_set_guard(rdb_colphys* h, int mode) {
+  if (mode < RDB_COLPHYS_GUARD_STRICT || mode > RDB_COLPHYS_GUARD_RANGE)
+    return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "set_guard: unknown mode %d", mode);
+  surr_get(h)->guard_mode = mode;
+  return RDB_COLPHYS_OK;
+}
+
+bool surr_ready(rdb_colphys* h) { return h->surr && h->surr->have_net && h->surr->d_stats; }
+
+}  // namespace colphys

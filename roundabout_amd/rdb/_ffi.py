@@ -89,6 +89,29 @@ class Config(ctypes.Structure):
     ]
 
 
+class MlpConfig(ctypes.Structure):
+    """ctypes mirror of ``rdb_colphys_mlp_config``."""
+
+    _fields_ = [
+        ("hidden", ctypes.c_int),
+        ("n_hidden", ctypes.c_int),
+        ("batch", ctypes.c_int),
+        ("epochs", ctypes.c_int),
+        ("lr", ctypes.c_double),
+        ("beta1", ctypes.c_double),
+        ("beta2", ctypes.c_double),
+        ("eps", ctypes.c_double),
+        ("seed", ctypes.c_int),
+        ("guard_mode", ctypes.c_int),
+        ("val_fraction", ctypes.c_double),
+    ]
+
+
+GUARD_STRICT = 0
+GUARD_SURFACE = 1
+GUARD_RANGE = 2
+
+
 class Counters(ctypes.Structure):
     """ctypes mirror of ``rdb_colphys_counters``."""
 
@@ -176,6 +199,14 @@ PROTOTYPES = [
     ("rdb_colphys_set_timing", ctypes.c_int, [_vp, ctypes.c_int]),
     ("rdb_colphys_stats", ctypes.c_int, [_vp, ctypes.POINTER(Counters)]),
     ("rdb_colphys_last_error", ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
+    ("rdb_colphys_mlp_config_default", ctypes.c_int, [ctypes.POINTER(MlpConfig)]),
+    ("rdb_colphys_mlp_collect", ctypes.c_int,
+     [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)]),
+    ("rdb_colphys_mlp_train", ctypes.c_int,
+     [_vp, ctypes.POINTER(MlpConfig), _fp, _fp, ctypes.c_int, _ip]),
+    ("rdb_colphys_mlp_save", ctypes.c_int, [_vp, ctypes.c_char_p]),
+    ("rdb_colphys_mlp_load", ctypes.c_int, [_vp, ctypes.c_char_p]),
+    ("rdb_colphys_set_guard", ctypes.c_int, [_vp, ctypes.c_int]),
     ("rdb_mlp_eval", ctypes.c_int,
      [_ip, ctypes.c_int, _fp, ctypes.c_int, _fp, _fp]),
     ("rdb_mlp_loss_grad", ctypes.c_int,

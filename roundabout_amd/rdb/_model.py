@@ -20,7 +20,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 
 LAYER_FIELDS = ("h_layer", "temperature", "salinity", "rho_layer")
 IFACE_FIELDS = ("kv", "kt", "ks", "gamma_t", "gamma_s")
-COLUMN_FIELDS = ("stress_shelf", "Q_heat", "Q_salt", "bl_depth", "stress_mag", "b0", "kbl")
+COLUMN_FIELDS = ("stress_shelf", "Q_heat", "Q_salt", "bl_depth", "stress_mag", "b0", "kbl", "coriolis")
 
 
 class ColumnPhysics:
@@ -71,6 +71,7 @@ class ColumnPhysics:
         self.set_field("Q_heat", ocean.Q_heat)
         self.set_field("Q_salt", ocean.Q_salt)
         self.set_field("bl_depth", ocean.bl_depth)
+        self.set_field("coriolis", ocean.coriolis)
 
     # ------------------------------------------------------------ lifecycle
     def close(self) -> None:
@@ -178,6 +179,39 @@ class ColumnPhysics:
 
     def set_timing(self, enable: bool) -> None:
         _ffi.check(self._lib.rdb_colphys_set_timing(self._handle(), int(bool(enable))), self._lib)
+
+    # ------------------------------------------------------- learned closure
+    def mlp_collect(self, dt: float, n_steps: int, stride: int = 1) -> int:
+        """Advance n_steps plain steps, recording training columns; returns
+        the number of samples collected so far."""
+        n = ctypes.c_longlong()
+        _ffi.check(self._lib.rdb_colphys_mlp_collect(self._handle(), float(dt), int(n_steps), int(stride),
+                                                     ctypes.byref(n)), self._lib)
+        return n.value
+
+    def mlp_train(self, **overrides) -> tuple[list, list]:
+        """Train the network (Adam on the GPU); returns per-epoch training and
+        held-out losses.  Keyword arguments override rdb_colphys_mlp_config."""
+        cfg = _ffi.MlpConfig()
+        _ffi.check(self._lib.rdb_colphys_mlp_config_default(ctypes.byref(cfg)), self._lib)
+        for k, v in overrides.items():
+            setattr(cfg, k, v)
+        n = max(cfg.epochs, 1)
+        tl = (ctypes.c_float * n)()
+        vl = (ctypes.c_float * n)()
+        done = ctypes.c_int()
+        _ffi.check(self._lib.rdb_colphys_mlp_train(self._handle(), ctypes.byref(cfg), tl, vl, n, ctypes.byref(done)),
+                   self._lib)
+        return list(tl)[:done.value], list(vl)[:done.value]
+
+    def mlp_save(self, path) -> None:
+        _ffi.check(self._lib.rdb_colphys_mlp_save(self._handle(), str(path).encode()), self._lib)
+
+    def mlp_load(self, path) -> None:
+        _ffi.check(self._lib.rdb_colphys_mlp_load(self._handle(), str(path).encode()), self._lib)
+
+    def set_guard(self, mode: int) -> None:
+        _ffi.check(self._lib.rdb_colphys_set_guard(self._handle(), int(mode)), self._lib)
 
     def stats(self) -> dict:
         c = _ffi.Counters()
