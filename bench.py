@@ -31,6 +31,7 @@ import numpy as np  # noqa: E402
 METRIC = json.loads((ROOT / "BASELINE.json").read_text())["metric"] if (ROOT / "BASELINE.json").exists() else \
     "column-steps/s"
 PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0  # dense BF16 MFMA
 DT = 3600.0
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_colphys_step_tile.txt"
 
@@ -84,6 +85,82 @@ def cpu_baseline(ocean, budget_s: float, crop_cols: tuple[int, int]):
     }
 
 
+def live_rms(ocean, a, b) -> float:
+    live = (ocean.h > 1.5e-4) & (ocean.wet > 0.5)[:, :, None]
+    d = (a - b)[live]
+    return float(np.sqrt(np.mean(d * d)))
+
+
+def surrogate_leg(ocean, cfg, args, plain_value: float) -> dict:
+    """The learned closure on the same tile: train on the GPU on columns the
+    plain path generates from an independent state draw on the same grid,
+    then run K guarded surrogate steps from the benchmark's initial state and
+    judge them against K plain steps from that state."""
+    from roundabout_amd import synth
+    from roundabout_amd.rdb import CLOSURE_SURROGATE, GUARD_RANGE, GUARD_STRICT, GUARD_SURFACE, ColumnPhysics
+
+    guard = {"strict": GUARD_STRICT, "surface": GUARD_SURFACE, "range": GUARD_RANGE}[args.guard]
+    K = args.steps
+    net_path = ROOT / "gpurun_out" / "bench_closure.bin"
+    net_path.parent.mkdir(exist_ok=True)
+    t0 = time.perf_counter()
+    with ColumnPhysics.from_ocean(synth.redraw(ocean, args.seed + 1000), config=cfg) as cp:
+        n = cp.mlp_collect(DT, args.train_steps)
+        cp.synchronize()
+        t_col = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        tl, vl = cp.mlp_train(epochs=args.epochs, guard_mode=guard)
+        cp.synchronize()
+        t_train = time.perf_counter() - t1
+        cp.mlp_save(net_path)
+    log(f"[bench] surrogate: {n} training columns in {t_col:.1f} s, {args.epochs} epochs in {t_train:.1f} s, "
+        f"val loss {vl[0]:.4f} -> {vl[-1]:.4f}")
+    with ColumnPhysics.from_ocean(ocean, config=cfg) as cp:
+        cp.step(DT, K)
+        T_ref, S_ref = cp.get_field("temperature"), cp.get_field("salinity")
+    with ColumnPhysics.from_ocean(ocean, config=cfg) as cp:
+        cp.mlp_load(net_path)
+        cp.set_guard(guard)
+        cp.set_closure(CLOSURE_SURROGATE)
+        cp.step(DT, 1)  # warm-up: workspace and first-touch allocations
+        cp.upload(ocean)
+        h0, s0 = cp.heat_salt_totals()
+        cp.set_timing(True)
+        cp.synchronize()
+        t2 = time.perf_counter()
+        cp.step(DT, K)
+        cp.synchronize()
+        wall = time.perf_counter() - t2
+        st = cp.stats()
+        h1, s1 = cp.heat_salt_totals()
+        T, S = cp.get_field("temperature"), cp.get_field("salinity")
+    n_wet = int(st["wet_columns"])
+    dep = K * DT / (cfg.rho0 * cfg.cp) * float((ocean.Q_heat * ocean.wet).sum())
+    value = n_wet * K / wall
+    t_rms, s_rms = live_rms(ocean, T, T_ref), live_rms(ocean, S, S_ref)
+    mlp_ms = st["mlp_forward_ms"]
+    tflops = st["mlp_flop_per_step"] / (mlp_ms * 1e-3) / 1e12 if mlp_ms > 0 else None
+    return {
+        "value": value,
+        "unit": "column-steps/s",
+        "ms_per_step": wall * 1e3 / K,
+        "speedup_vs_plain": value / plain_value,
+        "T_rms_K": t_rms,
+        "S_rms_psu": s_rms,
+        "meets_error_bar": bool(t_rms <= 0.01 and s_rms <= 0.002),
+        "heat_rel_residual": abs((h1 - h0) - dep) / abs(h0),
+        "salt_rel_residual": abs(s1 - s0) / abs(s0),
+        "fallback_fraction_last_step": st["fallback_fraction"],
+        "guard": args.guard,
+        "training": {"columns": n, "source": f"{args.train_steps} plain steps of an independent state draw "
+                     f"(seed {args.seed + 1000}) on the same grid", "epochs": args.epochs, "batch": 8192,
+                     "lr": 1e-3, "seconds": t_train, "val_loss_first": vl[0], "val_loss_last": vl[-1]},
+        "mlp": {"forward_ms": mlp_ms, "flop_per_step": st["mlp_flop_per_step"], "achieved_tflops": tflops,
+                "peak_tflops": PEAK_BF16_TFLOPS, "mfma_frac": (tflops / PEAK_BF16_TFLOPS) if tflops else None,
+                "widths": "381-256-256-256-256-149"},
+    }
+
+
 def copy_probe(nbytes: int = 1 << 30, reps: int = 3) -> None:
     """Device-to-device copies of a known size (the runtime's copy kernel),
     the calibration run for the FETCH_SIZE / WRITE_SIZE counters."""
@@ -115,6 +192,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-closure leg")
+    ap.add_argument("--guard", default="surface", choices=["strict", "surface", "range"])
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--train-steps", type=int, default=2, help="plain steps recorded as training columns")
     args = ap.parse_args()
 
     if args.copy_probe:
@@ -205,6 +286,9 @@ def main() -> int:
         "heat_salt_totals": [heat, salt],
     }
     cp.close()
+    if not args.no_surrogate and args.precision == "fp64":
+        log("[bench] learned closure leg ...")
+        out["surrogate"] = surrogate_leg(ocean, cfg, args, value)
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(ocean, args.cpu_seconds, (384, 256))

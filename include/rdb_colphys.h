@@ -130,6 +130,10 @@ typedef struct rdb_colphys_counters {
   int order_mode;              /* 0 natural order, 1 sorted by depth       */
   int lds_levels;              /* autotuned live levels kept in LDS        */
   int pad1;
+  double mlp_forward_ms;       /* surrogate closure, timing on: HIP-event
+                                  time of the network's forward GEMMs in the
+                                  last step                                 */
+  double mlp_flop_per_step;    /* 2 * sum of layer (in x out) * wet columns */
 } rdb_colphys_counters;
 
 /* ---- configuration ------------------------------------------------------ */
@@ -256,8 +260,9 @@ int rdb_colphys_stats(rdb_colphys *h, rdb_colphys_counters *out);
 /* ---- learned closure (no counterpart in the reference) -----------------
  * A multilayer perceptron predicts, per wet column, log10 kt and log10 kv at
  * the interior interfaces and the KPP boundary-layer depth from the column
- * state and forcing (T, S, centred u, v, log10 h at every level; |tau|,
- * Q_heat, Q_salt, Coriolis f, lagged boundary-layer depth, column depth).
+ * state and forcing (T, S, centred u, v as differences to the live level
+ * above and log10 h at every level; |tau|, Q_heat, Q_salt, Coriolis f,
+ * lagged boundary-layer depth, column depth).
  * BF16 weights and activations on MFMA with FP32 accumulation.  Its kv/kt go
  * through the same assembly clip and FP64 tridiagonal solve as the plain
  * path; the non-local transport uses the predicted depth.  A column falls

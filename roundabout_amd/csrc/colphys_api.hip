@@ -942,6 +942,8 @@ int rdb_colphys_stats(rdb_colphys* h, rdb_colphys_counters* out) {
       h->closure == RDB_COLPHYS_CLOSURE_PLAIN ? 1.0 : (double)h->last_fallback / (double)std::max(h->n_wet, 1);
   out->block_threads = h->block;
   out->order_mode = h->order_mode;
+  out->mlp_forward_ms = h->last_mlp_ms;
+  out->mlp_flop_per_step = h->mlp_flop;
   return RDB_COLPHYS_OK;
 }
 

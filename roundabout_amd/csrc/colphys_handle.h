@@ -69,4 +69,6 @@ struct rdb_colphys {
   double* d_coriolis = nullptr;
   colphys::SurrState* surr = nullptr;
   long long last_fallback = 0;
+  double last_mlp_ms = 0.0;
+  double mlp_flop = 0.0;  // per step, unpadded layer widths
 };

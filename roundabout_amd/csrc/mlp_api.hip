@@ -57,7 +57,7 @@ int rdb_mlp_eval(const int* widths, int n_widths, const float* params, int batch
   {
     mlp::Net net;
     st = build_net(net, widths, n_widths, params, s);
-    const int Mpad = mlp::round_up(batch, 64);
+    const int Mpad = mlp::round_up(batch, 128);  // the fused inference kernel's row block
     if (!st && net.ensure_workspace(Mpad, false)) st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, net.err.c_str());
     if (!st) st = upload_x(net, batch, Mpad, x);
     if (!st && net.forward(Mpad, false)) st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, net.err.c_str());

@@ -10,6 +10,20 @@ typedef uint16_t bf16;  // raw bfloat16 bits in memory
 
 enum { EPI_F32 = 0, EPI_BIAS_F32 = 1, EPI_BIAS_GELU = 2, EPI_BIAS_F32_T = 3 };
 
+// Fused inference over all layers (k_mlp_fused): per-layer weights (bf16,
+// pout x pin, k contiguous) and FP32 biases, padded widths.
+constexpr int FP_MAXL = 8;
+constexpr int FP_MAXK = 384;
+constexpr int FP_PITCH = FP_MAXK + 8;  // LDS row pitch (bf16): rows start 4 banks apart
+struct FusedNet {
+  const bf16* W[FP_MAXL];
+  const float* b[FP_MAXL];
+  int pin[FP_MAXL];
+  int pout[FP_MAXL];
+  int nl;
+};
+__global__ void k_mlp_fused(FusedNet fn, const bf16* X0, int ldx, float* Y, long long ldy, int out_major);
+
 template <int EPI>
 __global__ void k_gemm_bf16(const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, const float* bias,
                             float* Cf, int ldcf, bf16* Cb, int ldcb, int kc, long long split_stride);

@@ -137,6 +137,100 @@ template __global__ void k_gemm_bf16<EPI_BIAS_F32_T>(const bf16*, int, const bf1
 template __global__ void k_gemm_bf16<EPI_BIAS_GELU>(const bf16*, int, const bf16*, int, int, int, int, const float*,
                                                    float*, int, bf16*, int, int, long long);
 
+// The whole network on a block of 128 rows, activations resident in LDS.
+//
+// Workgroup of 4 wavefronts; the block's layer input (128 x pin bf16, row
+// pitch FP_PITCH) sits in LDS and every wave reads all of it, while wave w
+// owns output columns [w pout/4, (w+1) pout/4) and streams its rows of W
+// (k-contiguous bf16, L2-resident: the network is 0.66 MB) straight into
+// MFMA B fragments, one k-step ahead.  After a layer's k-loop all waves
+// synchronise and overwrite the block's LDS input with GELU(acc + b) in
+// bf16; the last layer adds the bias and stores FP32 to Y, row-major
+// (Y[m ldy + n]) or output-major (Y[n ldy + m]).  HBM sees only the input
+// features and the outputs: no activation round trips between layers.
+// Limits (checked by the host): pin <= FP_MAXK, pout <= 256, all multiples
+// of 64; rows a multiple of 128.
+__global__ void __launch_bounds__(256) k_mlp_fused(FusedNet fn, const bf16* __restrict__ X0, int ldx,
+                                                  float* __restrict__ Y, long long ldy, int out_major) {
+  extern __shared__ __align__(16) bf16 act[];  // 128 x FP_PITCH
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int r = lane & 15, q = lane >> 4;
+  const long long m0 = (long long)blockIdx.x * 128;
+  // stage the block's inputs: 16-byte chunks, rows contiguous
+  {
+    const int cpr = ldx / 8;  // chunks per row
+    for (int c = tid; c < 128 * cpr; c += 256) {
+      const int row = c / cpr, col = (c % cpr) * 8;
+      *(bf16x8*)(&act[row * FP_PITCH + col]) = *(const bf16x8*)(X0 + (size_t)(m0 + row) * (size_t)ldx + col);
+    }
+  }
+  __syncthreads();
+  for (int l = 0; l < fn.nl; ++l) {
+    const int K = fn.pin[l], N = fn.pout[l];
+    const int ntw = N / 64;            // 16-wide n-tiles per wave (<= 4)
+    const int nw0 = wave * (N / 4);    // this wave's first output column
+    const bf16* __restrict__ W = fn.W[l];
+    f32x4 acc[8][4];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bf16* Bp[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) Bp[j] = W + (size_t)(nw0 + 16 * (j < ntw ? j : 0) + r) * (size_t)K + q * 8;
+    bf16x8 b_cur[4], b_nxt[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (j < ntw) b_cur[j] = *(const bf16x8*)(Bp[j]);
+    for (int k0 = 0; k0 < K; k0 += 32) {
+      const bool more = k0 + 32 < K;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (more && j < ntw) b_nxt[j] = *(const bf16x8*)(Bp[j] + k0 + 32);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const bf16x8 a = *(const bf16x8*)(&act[(16 * i + r) * FP_PITCH + k0 + q * 8]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (j < ntw) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b_cur[j], acc[i][j], 0, 0, 0);
+      }
+      if (more) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b_cur[j] = b_nxt[j];
+      }
+    }
+    __syncthreads();  // every wave has read this layer's input
+    const float* __restrict__ bias = fn.b[l];
+    const bool last = (l == fn.nl - 1);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (j >= ntw) continue;
+      const int n = nw0 + 16 * j + r;
+      const float bn = bias[n];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int mr = 16 * i + 4 * q;  // rows mr .. mr + 3 of the block
+        if (!last) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) act[(mr + e) * FP_PITCH + n] = f2bf(gelu(acc[i][j][e] + bn));
+        } else if (out_major) {
+          f32x4 v = acc[i][j];
+          v[0] += bn;
+          v[1] += bn;
+          v[2] += bn;
+          v[3] += bn;
+          *(f32x4*)(Y + (size_t)n * (size_t)ldy + (size_t)(m0 + mr)) = v;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) Y[(size_t)(m0 + mr + e) * (size_t)ldy + n] = acc[i][j][e] + bn;
+        }
+      }
+    }
+    __syncthreads();  // the next layer's input is complete
+  }
+}
+
 // out[c][r] = in[r][c] for an R x C bf16 matrix (row pitches ldi, ldo).
 __global__ void k_transpose_bf16(const bf16* __restrict__ in, int ldi, bf16* __restrict__ out, int ldo, int R, int C) {
   __shared__ bf16 tile[32][33];

@@ -59,7 +59,10 @@ struct Net {
   size_t n_public() const;
   // X0 (Mpad x pin[0] bf16) must hold the inputs; Y receives the outputs,
   // row-major (Mpad x pout) or, with out_major, output-major (pout x Mpad)
+  // Inference (keep_pre false) with Mpad a multiple of 128 and widths within
+  // the fused kernel's limits runs as one fused launch (k_mlp_fused).
   int forward(int Mpad, bool keep_pre, bool out_major = false);
+  bool fused_ok(int Mpad) const;
   // T, Wm (Mpad x pout[last]) hold targets and weights; rows >= Mvalid
   // ignored.  The loss is added to *loss_dev (device, no synchronisation)
   // when given, else returned in *loss_out (synchronises) when that is given.

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <utility>
 
@@ -226,7 +227,41 @@ int Net::ensure_workspace(int Mpad, bool training) {
   return 0;
 }
 
+bool Net::fused_ok(int Mpad) const {
+  // RDB_MLP_UNFUSED=1 keeps inference on the per-layer GEMMs (A/B timing)
+  static const bool unfused = [] {
+    const char* e = std::getenv("RDB_MLP_UNFUSED");
+    return e && e[0] == '1';
+  }();
+  if (unfused || Mpad % 128 != 0 || nl > FP_MAXL) return false;
+  for (int l = 0; l < nl; ++l)
+    if (pin[l] > FP_MAXK || pout[l] > 256) return false;
+  return true;
+}
+
 int Net::forward(int Mpad, bool keep_pre, bool out_major) {
+  if (!keep_pre && fused_ok(Mpad)) {
+    // inference: one launch over all layers, activations in LDS
+    static bool lds_set = false;
+    const size_t shmem = (size_t)128 * FP_PITCH * sizeof(bf16);
+    if (!lds_set) {
+      NT_TRY(hipFuncSetAttribute((const void*)k_mlp_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+      lds_set = true;
+    }
+    FusedNet fn{};
+    fn.nl = nl;
+    for (int l = 0; l < nl; ++l) {
+      fn.W[l] = Wb + offWb[l];
+      fn.b[l] = p + offB[l];
+      fn.pin[l] = pin[l];
+      fn.pout[l] = pout[l];
+    }
+    const long long ldy = out_major ? (long long)Mpad : (long long)pout[nl - 1];
+    hipLaunchKernelGGL(k_mlp_fused, dim3((unsigned)(Mpad / 128)), dim3(256), shmem, stream, fn, (const bf16*)X0,
+                       pin[0], Y, ldy, out_major ? 1 : 0);
+    NT_TRY(hipGetLastError());
+    return 0;
+  }
   const bf16* x = X0;
   int ldx = pin[0];
   for (int l = 0; l < nl; ++l) {

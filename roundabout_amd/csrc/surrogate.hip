@@ -56,6 +56,7 @@ struct SurrState {
   unsigned int* nfall = nullptr;
   int guard_mode = RDB_COLPHYS_GUARD_STRICT;
   int inf_rows = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};  // around the forward GEMMs (timing on)
   ~SurrState();
   void free_dataset();
 };
@@ -308,6 +309,8 @@ SurrState::~SurrState() {
   if (d_stats) (void)hipFree(d_stats);
   if (guard) (void)hipFree(guard);
   if (nfall) (void)hipFree(nfall);
+  for (hipEvent_t e : ev)
+    if (e) (void)hipEventDestroy(e);
 }
 
 void SurrState::free_dataset() {
@@ -529,7 +532,7 @@ int surr_step(rdb_colphys* h, double dt) {
   mlp::Net& net = S->net;
   if (net.pin[0] > 384) return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "surrogate: input width exceeds the feature tile");
   if (h->n_passive > 4) return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "surrogate: at most 4 passive tracers");
-  const int Mpad = mlp::round_up(std::max(h->n_wet, 1), 64);
+  const int Mpad = mlp::round_up(std::max(h->n_wet, 1), 128);  // the fused inference kernel's row block
   if (S->inf_rows != Mpad) {
     int e = net.ensure_workspace(Mpad, false);
     if (e) return perr(e, "surrogate: %s", net.err.c_str());
@@ -544,8 +547,17 @@ int surr_step(rdb_colphys* h, double dt) {
                      S->din, net.pin[0], S->d_stats, net.X0, S->guard, S->nfall, S->guard_mode, h->pr.rho0,
                      h->pr.p_ref);
   SG_TRY(hipGetLastError());
+  if (h->timing) {
+    if (!S->ev[0]) SG_TRY(hipEventCreate(&S->ev[0]));
+    if (!S->ev[1]) SG_TRY(hipEventCreate(&S->ev[1]));
+    SG_TRY(hipEventRecord(S->ev[0], h->stream));
+  }
   int e = net.forward(Mpad, false, true);
   if (e) return perr(e, "surrogate: %s", net.err.c_str());
+  if (h->timing) SG_TRY(hipEventRecord(S->ev[1], h->stream));
+  double flop = 0.0;
+  for (int l = 0; l < net.nl; ++l) flop += 2.0 * (double)net.din[l] * (double)net.dout[l];
+  h->mlp_flop = flop * (double)h->n_wet;
   const SurrView sv{net.Y, (long long)Mpad, S->d_stats + 4 * S->din, S->d_stats + 4 * S->din + S->dout, S->guard};
   const bool two = (h->cfg.kt_bg != h->cfg.ks_bg);
   const int ntr = h->n_passive;
@@ -575,6 +587,11 @@ int surr_step(rdb_colphys* h, double dt) {
   SG_TRY(hipMemcpyAsync(&nf, S->nfall, 4, hipMemcpyDeviceToHost, h->stream));
   SG_TRY(hipStreamSynchronize(h->stream));
   h->last_fallback = nf;
+  if (h->timing) {
+    float ms = 0.f;
+    SG_TRY(hipEventElapsedTime(&ms, S->ev[0], S->ev[1]));
+    h->last_mlp_ms = ms;
+  }
   return RDB_COLPHYS_OK;
 }
 

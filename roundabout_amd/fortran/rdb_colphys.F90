@@ -9,7 +9,7 @@
 !> staged entry points with the same state arrays.  Arrays are the reference
 !> layout: column-major (i fastest), k = 1 at the bed; passed by address.
 module rdb_colphys
-   use, intrinsic :: iso_c_binding, only: c_int, c_double, c_char, c_ptr, c_null_ptr, &
+   use, intrinsic :: iso_c_binding, only: c_int, c_double, c_float, c_char, c_ptr, c_null_ptr, &
                                           c_long_long, c_null_char, c_loc, c_associated
    implicit none
    private
@@ -23,6 +23,12 @@ module rdb_colphys
    integer(c_int), parameter, public :: RDB_COLPHYS_ERR_BAD_SHAPE = 14
    integer(c_int), parameter, public :: RDB_COLPHYS_PREC_FP64 = 0
    integer(c_int), parameter, public :: RDB_COLPHYS_PREC_SPLIT = 1
+   integer(c_int), parameter, public :: RDB_COLPHYS_GUARD_TRIPPED = 1
+   integer(c_int), parameter, public :: RDB_COLPHYS_CLOSURE_PLAIN = 0
+   integer(c_int), parameter, public :: RDB_COLPHYS_CLOSURE_SURROGATE = 1
+   integer(c_int), parameter, public :: RDB_COLPHYS_GUARD_STRICT = 0
+   integer(c_int), parameter, public :: RDB_COLPHYS_GUARD_SURFACE = 1
+   integer(c_int), parameter, public :: RDB_COLPHYS_GUARD_RANGE = 2
 
    !> Mirror of rdb_colphys_config (field order is the ABI).
    type, bind(c), public :: rdb_colphys_config_t
@@ -41,6 +47,16 @@ module rdb_colphys
       integer(c_int) :: use_harmonic, precision
    end type rdb_colphys_config_t
 
+   !> Mirror of rdb_colphys_mlp_config (learned closure training).
+   type, bind(c), public :: rdb_colphys_mlp_config_t
+      integer(c_int) :: hidden, n_hidden, batch, epochs
+      real(c_double) :: lr, beta1, beta2, eps
+      integer(c_int) :: seed, guard_mode
+      real(c_double) :: val_fraction
+   end type rdb_colphys_mlp_config_t
+
+   public :: rdb_colphys_mlp_config_default, rdb_colphys_mlp_collect, rdb_colphys_mlp_train
+   public :: rdb_colphys_mlp_save, rdb_colphys_mlp_load, rdb_colphys_set_guard
    public :: rdb_colphys_config_default, rdb_colphys_working_precision
    public :: rdb_colphys_create, rdb_colphys_destroy
    public :: rdb_colphys_set_field, rdb_colphys_get_field
@@ -142,6 +158,42 @@ module rdb_colphys
          import :: c_int, c_double, c_ptr
          type(c_ptr), value :: handle
          real(c_double), intent(out) :: heat, salt
+      end function
+      integer(c_int) function rdb_colphys_mlp_config_default(cfg) bind(c, name="rdb_colphys_mlp_config_default")
+         import :: c_int, rdb_colphys_mlp_config_t
+         type(rdb_colphys_mlp_config_t), intent(out) :: cfg
+      end function
+      integer(c_int) function rdb_colphys_mlp_collect(handle, dt, n_steps, stride, n_samples) &
+         bind(c, name="rdb_colphys_mlp_collect")
+         import :: c_int, c_double, c_ptr, c_long_long
+         type(c_ptr), value :: handle
+         real(c_double), value :: dt
+         integer(c_int), value :: n_steps, stride
+         integer(c_long_long), intent(out) :: n_samples
+      end function
+      integer(c_int) function rdb_colphys_mlp_train(handle, cfg, train_loss, val_loss, n_hist, n_done) &
+         bind(c, name="rdb_colphys_mlp_train")
+         import :: c_int, c_float, c_ptr, rdb_colphys_mlp_config_t
+         type(c_ptr), value :: handle
+         type(rdb_colphys_mlp_config_t), intent(in) :: cfg
+         real(c_float), intent(out) :: train_loss(*), val_loss(*)
+         integer(c_int), value :: n_hist
+         integer(c_int), intent(out) :: n_done
+      end function
+      integer(c_int) function rdb_colphys_mlp_save(handle, path) bind(c, name="rdb_colphys_mlp_save")
+         import :: c_int, c_char, c_ptr
+         type(c_ptr), value :: handle
+         character(kind=c_char), intent(in) :: path(*)
+      end function
+      integer(c_int) function rdb_colphys_mlp_load(handle, path) bind(c, name="rdb_colphys_mlp_load")
+         import :: c_int, c_char, c_ptr
+         type(c_ptr), value :: handle
+         character(kind=c_char), intent(in) :: path(*)
+      end function
+      integer(c_int) function rdb_colphys_set_guard(handle, mode) bind(c, name="rdb_colphys_set_guard")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int), value :: mode
       end function
       integer(c_int) function rdb_colphys_last_error(idx, buf, cap) bind(c, name="rdb_colphys_last_error")
          import :: c_int, c_char

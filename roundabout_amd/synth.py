@@ -156,7 +156,15 @@ def make_ocean(nx: int, ny: int, nz: int, seed: int = 1, *, land_frac: float = 0
                 hflat[c, mm] = 10.0 ** rng.uniform(-3.0, -1.0)
         h_sf = hflat.reshape(nx, ny, nz, order="F")
     h = np.asfortranarray(h_sf[:, :, ::-1])  # k = 0 bed
+    return _with_state(rng, h, wet, n_live, n_passive)
 
+
+def _with_state(rng: np.random.Generator, h: np.ndarray, wet: np.ndarray, n_live: np.ndarray,
+                n_passive: int) -> Ocean:
+    """Draw T, S, passives, face velocities and forcing for a given geometry."""
+    nx, ny, nz = h.shape
+    wetb = wet > 0.5
+    h_sf = h[:, :, ::-1]
     # Layer-centre depth from the surface (live layers only matter).
     cum = np.cumsum(h_sf, axis=2) - 0.5 * h_sf
     z_c = np.asfortranarray(cum[:, :, ::-1])
@@ -197,6 +205,14 @@ def make_ocean(nx: int, ny: int, nz: int, seed: int = 1, *, land_frac: float = 0
                  np.asfortranarray(v), np.asfortranarray(T), np.asfortranarray(S),
                  [np.asfortranarray(p) for p in passive], tau_x, tau_y, Q_heat, Q_salt,
                  np.asfortranarray(wet), k_top, np.asfortranarray(n_live), cor, bl)
+
+
+def redraw(o: Ocean, seed: int) -> Ocean:
+    """Another state on the same grid: the geometry (h, wet mask, live
+    layers) of ``o`` with T, S, passives, velocities and forcing drawn anew
+    from the same distributions with an independent seed."""
+    return _with_state(np.random.default_rng(seed), o.h.copy(order="F"), o.wet.copy(order="F"), o.n_live.copy(),
+                       o.n_passive)
 
 
 def crop(o: Ocean, i0: int, i1: int, j0: int, j1: int) -> Ocean:
