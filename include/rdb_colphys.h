@@ -317,6 +317,25 @@ int rdb_mlp_eval(const int *widths, int n_widths, const float *params, int batch
 int rdb_mlp_loss_grad(const int *widths, int n_widths, const float *params, int batch, const float *x,
                       const float *t, const float *w, float *loss, float *grad);
 
+/* ---- batched ensemble (no counterpart in the reference) -----------------
+ * Several independent ocean members, each with its own grid (nx x ny x nz),
+ * wet mask and optional depth sort key, in one handle.  rdb_colphys_step
+ * advances all of them with one persistent launch per step (captured in the
+ * hipGraph like the single-grid step); the learned closure, when enabled,
+ * runs one network over every member's columns (its levels are the largest
+ * member nz, with the members' surfaces aligned) and the guard sends columns
+ * back to the plain path per column.  Fields are reached per member; the
+ * single-grid entry points (set_field/get_field and the staged routines)
+ * refuse an ensemble handle with RDB_COLPHYS_ERR_BAD_HANDLE.  Totals and
+ * counters cover all members.  Counts as the one live handle. */
+int rdb_colphys_ensemble_create(const rdb_colphys_config *cfg, int n_members, const int *nx, const int *ny,
+                                const int *nz, int n_passive, const double *const *wet, const int *const *sort_key,
+                                rdb_colphys **out);
+int rdb_colphys_member_count(rdb_colphys *h, int *n_members);
+int rdb_colphys_member_set_field(rdb_colphys *h, int member, const char *name, const double *data, int n1, int n2,
+                                 int n3);
+int rdb_colphys_member_get_field(rdb_colphys *h, int member, const char *name, double *out, int n1, int n2, int n3);
+
 /* ---- error ring (mirrors rdb_ocean_last_error, rdb_ocean.h:302-314) ---- */
 int rdb_colphys_last_error(int idx, char *buf, int cap);
 

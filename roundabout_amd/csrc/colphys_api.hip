@@ -176,6 +176,11 @@ int dalloc(rdb_colphys* h, void** p, size_t bytes) {
 }
 
 void release(rdb_colphys* h) {
+  for (rdb_colphys* m : h->members) {
+    release(m);
+    delete m;
+  }
+  h->members.clear();
   colphys::surr_free(h->surr);
   h->surr = nullptr;
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
@@ -184,7 +189,9 @@ void release(rdb_colphys* h) {
   h->events.clear();
   for (void* p : h->allocations) (void)hipFree(p);
   h->allocations.clear();
-  if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->d_work) (void)hipFree(h->d_work);
+  h->d_work = nullptr;
+  if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
   h->stream = nullptr;
 }
 
@@ -231,6 +238,76 @@ int launch_step(rdb_colphys* h, double dt, int p0, int p1) {
   return launch_check("k_column_step");
 }
 
+// Every wet column of the handle: the single-grid launch, or for an ensemble
+// one persistent launch over the (member, column chunk) work list.  With surr
+// the members' learned-closure views (d_svs) must be current.
+// (Re)build the ensemble's work list for the current block size; outside
+// any stream capture (it allocates and copies).
+int ensure_work(rdb_colphys* h) {
+  const int B = h->block;
+  if (h->members.empty() || (h->work_block == B && h->d_work)) return RDB_COLPHYS_OK;
+  {
+    // chunks of B packed columns; members interleaved so that the deep
+    // columns (each member is sorted deepest first) spread over the grid
+    std::vector<int2> work;
+    int maxc = 0;
+    for (rdb_colphys* m : h->members) maxc = std::max(maxc, (m->n_wet + B - 1) / B);
+    for (int c = 0; c < maxc; ++c)
+      for (size_t m = 0; m < h->members.size(); ++m)
+        if (c * B < h->members[m]->n_wet) work.push_back(make_int2((int)m, c * B));
+    if (h->d_work) {
+      HIP_TRY(hipStreamSynchronize(h->stream));
+      HIP_TRY(hipFree(h->d_work));
+      h->d_work = nullptr;
+    }
+    HIP_TRY(hipMalloc((void**)&h->d_work, sizeof(int2) * std::max<size_t>(work.size(), 1)));
+    HIP_TRY(hipMemcpy(h->d_work, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice));
+    h->n_work = (int)work.size();
+    h->work_block = B;
+  }
+  int dev = 0, n_cu = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+  // persistent grid: up to 2048 lanes per CU, never more workgroups than items
+  h->persist_grid = std::max(1, std::min(h->n_work, std::max(n_cu, 1) * std::max(1, 2048 / B)));
+  return RDB_COLPHYS_OK;
+}
+
+int launch_step_all(rdb_colphys* h, double dt, bool surr) {
+  if (h->members.empty()) return surr ? push_error(RDB_COLPHYS_ERR_SETUP, "internal: single-grid surrogate launch")
+                                      : launch_step(h, dt, 0, h->n_wet);
+  int st = ensure_work(h);
+  if (st) return st;
+  if (h->n_work == 0) return RDB_COLPHYS_OK;
+  const int B = h->block, grid = h->persist_grid;
+  const bool two = (h->cfg.kt_bg != h->cfg.ks_bg);
+  const bool split = (h->precision == RDB_COLPHYS_PREC_SPLIT);
+  const int ntr = h->n_passive;
+  const int M = h->lds_levels;
+  const size_t shmem = (size_t)B * (size_t)M * (size_t)(3 + std::min(ntr, 4) + (two ? 1 : 0)) * 8;
+#define LAUNCH(R, N, TF, SU)                                                                                      \
+  hipLaunchKernelGGL((colphys::k_column_step_ens<R, N, TF, SU>), dim3(grid), dim3(B), shmem, h->stream, h->d_views, \
+                     h->d_svs, h->d_nwet, h->pr, dt, h->d_work, h->n_work, h->d_guard, M)
+#define BY_NTR(R, TF, SU)                \
+  switch (ntr) {                         \
+    case 0: LAUNCH(R, 0, TF, SU); break; \
+    case 1: LAUNCH(R, 1, TF, SU); break; \
+    case 2: LAUNCH(R, 2, TF, SU); break; \
+    case 3: LAUNCH(R, 3, TF, SU); break; \
+    default: LAUNCH(R, 4, TF, SU); break; \
+  }
+  if (surr) {
+    if (two) { BY_NTR(double, true, true) } else { BY_NTR(double, false, true) }
+  } else if (split) {
+    if (two) { BY_NTR(float, true, false) } else { BY_NTR(float, false, false) }
+  } else {
+    if (two) { BY_NTR(double, true, false) } else { BY_NTR(double, false, false) }
+  }
+#undef BY_NTR
+#undef LAUNCH
+  return launch_check("k_column_step_ens");
+}
+
 // Staged sequence (one kernel per reference routine) over [p0, p1).
 int staged_coefficients(rdb_colphys* h, int p0, int p1) {
   if (p1 <= p0) return RDB_COLPHYS_OK;
@@ -252,6 +329,10 @@ int read_guard(rdb_colphys* h, unsigned int* out) {
 // Dry columns (no live layer) have outputs that depend only on h, u, v, tau
 // and Q: refresh them with one launch whenever one of those changed.
 int refresh_static(rdb_colphys* h, double dt) {
+  for (rdb_colphys* m : h->members) {
+    int st = refresh_static(m, dt);
+    if (st) return st;
+  }
   if (!h->static_dirty) return RDB_COLPHYS_OK;
   hipLaunchKernelGGL(colphys::k_prepare, grid1(h->n_all, 256), dim3(256), 0, h->stream, h->s, h->pr, 0, h->n_all);
   int st = launch_check("k_prepare");
@@ -291,9 +372,11 @@ int ensure_events(rdb_colphys* h, int n) {
 // into a graph.
 int build_graph(rdb_colphys* h, double dt, int n) {
   drop_graph(h);
+  int st0 = colphys::ensure_work(h);
+  if (st0) return st0;
   HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   int st = RDB_COLPHYS_OK;
-  for (int i = 0; i < n && st == RDB_COLPHYS_OK; ++i) st = launch_step(h, dt, 0, h->n_wet);
+  for (int i = 0; i < n && st == RDB_COLPHYS_OK; ++i) st = colphys::launch_step_all(h, dt, false);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(h->stream, &g);
   if (st) {
@@ -350,12 +433,15 @@ int rdb_colphys_config_default(rdb_colphys_config* c) {
 
 int rdb_colphys_working_precision(void) { return 8; }
 
-int rdb_colphys_create(const rdb_colphys_config* cfg, int nx, int ny, int nz, int n_passive, const double* wet,
-                       const int* k_top, const int* sort_key, rdb_colphys** out) {
-  if (!out) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "create: NULL handle_out");
+}  // extern "C"
+
+namespace {
+
+// One grid's plan: validation, column packing, device buffers.  With a
+// non-null `stream` the plan borrows it (an ensemble member).
+int create_plan(const rdb_colphys_config* cfg, int nx, int ny, int nz, int n_passive, const double* wet,
+                const int* k_top, const int* sort_key, hipStream_t stream, rdb_colphys** out) {
   *out = nullptr;
-  if (g_live)
-    return push_error(RDB_COLPHYS_ERR_ALREADY_EXISTS, "create: a column-physics handle is already live");
   if (!cfg || !wet) return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "create: NULL config or wet mask");
   if (nx < 1 || ny < 1 || nz < 1 || n_passive < 0 || n_passive > 16)
     return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "create: bad extents nx=%d ny=%d nz=%d n_passive=%d", nx, ny, nz,
@@ -421,10 +507,15 @@ int rdb_colphys_create(const rdb_colphys_config* cfg, int nx, int ny, int nz, in
   h->col_host = wet_idx;
   h->col_host.insert(h->col_host.end(), dry_idx.begin(), dry_idx.end());
 
-  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) {
-    delete h;
-    return push_error(RDB_COLPHYS_ERR_SETUP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+  if (stream) {
+    h->stream = stream;
+    h->own_stream = false;
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete h;
+      return push_error(RDB_COLPHYS_ERR_SETUP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
   }
   const size_t P = (size_t)h->P;
   const size_t L = (size_t)nz * P, I = (size_t)(nz + 1) * P;
@@ -508,8 +599,122 @@ int rdb_colphys_create(const rdb_colphys_config* cfg, int nx, int ny, int nz, in
     delete h;
     return push_error(RDB_COLPHYS_ERR_SETUP, "create: stream synchronize failed");
   }
+  *out = h;
+  return RDB_COLPHYS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rdb_colphys_create(const rdb_colphys_config* cfg, int nx, int ny, int nz, int n_passive, const double* wet,
+                       const int* k_top, const int* sort_key, rdb_colphys** out) {
+  if (!out) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "create: NULL handle_out");
+  *out = nullptr;
+  if (g_live)
+    return push_error(RDB_COLPHYS_ERR_ALREADY_EXISTS, "create: a column-physics handle is already live");
+  int st = create_plan(cfg, nx, ny, nz, n_passive, wet, k_top, sort_key, nullptr, out);
+  if (st) return st;
+  g_live = true;
+  return RDB_COLPHYS_OK;
+}
+
+int rdb_colphys_ensemble_create(const rdb_colphys_config* cfg, int n_members, const int* nx, const int* ny,
+                                const int* nz, int n_passive, const double* const* wet, const int* const* sort_key,
+                                rdb_colphys** out) {
+  if (!out) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "ensemble_create: NULL handle_out");
+  *out = nullptr;
+  if (g_live)
+    return push_error(RDB_COLPHYS_ERR_ALREADY_EXISTS, "ensemble_create: a column-physics handle is already live");
+  if (n_members < 1 || n_members > 256 || !nx || !ny || !nz || !wet)
+    return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "ensemble_create: bad member count %d or NULL extents", n_members);
+  if (n_passive > 4)
+    return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "ensemble_create: at most 4 passive tracers");
+  // the parent: a 1 x 1 x 1 all-land grid that owns the stream, the graph,
+  // the counters and the learned closure; its members hold the columns
+  const double land = 0.0;
+  rdb_colphys* h = nullptr;
+  int st = create_plan(cfg, 1, 1, 1, n_passive, &land, nullptr, nullptr, nullptr, &h);
+  if (st) return st;
+  long long row = 0;
+  for (int m = 0; m < n_members && !st; ++m) {
+    rdb_colphys* mp = nullptr;
+    st = create_plan(cfg, nx[m], ny[m], nz[m], n_passive, wet[m], nullptr, sort_key ? sort_key[m] : nullptr,
+                     h->stream, &mp);
+    if (st) {
+      push_error(st, "ensemble_create: member %d (%d x %d x %d) failed", m, nx[m], ny[m], nz[m]);
+      break;
+    }
+    mp->row_off = row;
+    row += mp->n_wet;
+    h->members.push_back(mp);
+  }
+  int nz_max = 1;
+  for (rdb_colphys* mp : h->members) nz_max = std::max(nz_max, mp->nz);
+  for (rdb_colphys* mp : h->members) mp->lev_off = nz_max - mp->nz;
+  // the parent's own 1 x 1 x 1 grid keeps nz = 1 and no wet column, so any
+  // single-grid code path that reaches it touches nothing
+  h->nz_net = nz_max;
+  h->ens_wet = row;
+  if (!st && row > 0x7fffffffLL) st = push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "ensemble_create: too many wet columns");
+  if (!st) {
+    const size_t nm = h->members.size();
+    std::vector<StateView> views(nm);
+    std::vector<int> nw(nm);
+    for (size_t m = 0; m < nm; ++m) {
+      views[m] = h->members[m]->s;
+      nw[m] = h->members[m]->n_wet;
+      h->live_levels += h->members[m]->live_levels;
+    }
+    st = dalloc(h, (void**)&h->d_views, sizeof(StateView) * nm);
+    if (!st) st = dalloc(h, (void**)&h->d_svs, sizeof(colphys::SurrView) * nm);
+    if (!st) st = dalloc(h, (void**)&h->d_nwet, sizeof(int) * nm);
+    if (!st && (hipMemcpy(h->d_views, views.data(), sizeof(StateView) * nm, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(h->d_nwet, nw.data(), sizeof(int) * nm, hipMemcpyHostToDevice) != hipSuccess))
+      st = push_error(RDB_COLPHYS_ERR_SETUP, "ensemble_create: upload of the member table failed");
+  }
+  if (st) {
+    release(h);
+    delete h;
+    return st;
+  }
   g_live = true;
   *out = h;
+  return RDB_COLPHYS_OK;
+}
+
+int rdb_colphys_member_set_field(rdb_colphys* h, int member, const char* name, const double* data, int n1, int n2,
+                                 int n3) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "member_set_field: bad handle");
+  if (h->members.empty()) {
+    if (member != 0) return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "member_set_field: member %d of a single grid", member);
+    return rdb_colphys_set_field(h, name, data, n1, n2, n3);
+  }
+  if (member < 0 || member >= (int)h->members.size())
+    return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "member_set_field: no member %d", member);
+  int st = rdb_colphys_set_field(h->members[(size_t)member], name, data, n1, n2, n3);
+  if (!st && name && !std::strcmp(name, "h_layer")) {
+    h->live_levels = 0;
+    for (rdb_colphys* m : h->members) h->live_levels += m->live_levels;
+  }
+  return st;
+}
+
+int rdb_colphys_member_get_field(rdb_colphys* h, int member, const char* name, double* out, int n1, int n2, int n3) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "member_get_field: bad handle");
+  if (h->members.empty()) {
+    if (member != 0) return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "member_get_field: member %d of a single grid", member);
+    return rdb_colphys_get_field(h, name, out, n1, n2, n3);
+  }
+  if (member < 0 || member >= (int)h->members.size())
+    return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "member_get_field: no member %d", member);
+  return rdb_colphys_get_field(h->members[(size_t)member], name, out, n1, n2, n3);
+}
+
+int rdb_colphys_member_count(rdb_colphys* h, int* n) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "member_count: bad handle");
+  if (!n) return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "member_count: NULL output");
+  *n = h->members.empty() ? 1 : (int)h->members.size();
   return RDB_COLPHYS_OK;
 }
 
@@ -528,6 +733,9 @@ int rdb_colphys_destroy(rdb_colphys** hp) {
 
 int rdb_colphys_set_field(rdb_colphys* h, const char* name, const double* data, int n1, int n2, int n3) {
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "set_field: bad handle");
+  if (!h->members.empty())
+    return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "set_field: an ensemble handle is reached per member (rdb_colphys_member_*) "
+                      "and stepped with rdb_colphys_step");
   if (!name || !data) return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "set_field: NULL name or data");
   FieldSpec spec;
   int tracer;
@@ -626,6 +834,9 @@ int rdb_colphys_set_field(rdb_colphys* h, const char* name, const double* data, 
 
 int rdb_colphys_get_field(rdb_colphys* h, const char* name, double* out, int n1, int n2, int n3) {
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "get_field: bad handle");
+  if (!h->members.empty())
+    return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "get_field: an ensemble handle is reached per member (rdb_colphys_member_*) "
+                      "and stepped with rdb_colphys_step");
   if (!name || !out) return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "get_field: NULL name or buffer");
   FieldSpec spec;
   int tracer;
@@ -671,6 +882,9 @@ int rdb_colphys_get_field(rdb_colphys* h, const char* name, double* out, int n1,
 
 int rdb_colphys_eos(rdb_colphys* h) {
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "eos: bad handle");
+  if (!h->members.empty())
+    return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "eos: an ensemble handle is reached per member (rdb_colphys_member_*) "
+                      "and stepped with rdb_colphys_step");
   dim3 g(grid1(h->n_all, h->block).x, (unsigned)h->nz);
   hipLaunchKernelGGL(colphys::k_eos, g, dim3(h->block), 0, h->stream, h->s, h->pr, 0, h->n_all);
   return launch_check("k_eos");
@@ -679,6 +893,9 @@ int rdb_colphys_eos(rdb_colphys* h) {
 int rdb_colphys_coefficients(rdb_colphys* h, double dt) {
   (void)dt;
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "coefficients: bad handle");
+  if (!h->members.empty())
+    return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "coefficients: an ensemble handle is reached per member (rdb_colphys_member_*) "
+                      "and stepped with rdb_colphys_step");
   int st = staged_coefficients(h, 0, h->n_all);
   if (st) return st;
   h->static_dirty = true;  // the staged pass rewrote every interface
@@ -696,6 +913,9 @@ int rdb_colphys_coefficients(rdb_colphys* h, double dt) {
 
 int rdb_colphys_surface_flux(rdb_colphys* h, double dt) {
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "surface_flux: bad handle");
+  if (!h->members.empty())
+    return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "surface_flux: an ensemble handle is reached per member (rdb_colphys_member_*) "
+                      "and stepped with rdb_colphys_step");
   if (!h->cfg.has_heat && !h->cfg.has_salt) return RDB_COLPHYS_OK;
   hipLaunchKernelGGL(colphys::k_surface_flux, grid1(h->n_all, h->block), dim3(h->block), 0, h->stream, h->s, h->pr,
                      dt, 0, h->n_all);
@@ -704,6 +924,9 @@ int rdb_colphys_surface_flux(rdb_colphys* h, double dt) {
 
 int rdb_colphys_diffuse_tracers(rdb_colphys* h, double dt) {
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "diffuse_tracers: bad handle");
+  if (!h->members.empty())
+    return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "diffuse_tracers: an ensemble handle is reached per member (rdb_colphys_member_*) "
+                      "and stepped with rdb_colphys_step");
   hipLaunchKernelGGL(colphys::k_vdiff, grid1(h->n_all, h->block), dim3(h->block), 0, h->stream, h->s, h->pr, dt, 0,
                      h->n_all);
   return launch_check("k_vdiff");
@@ -711,6 +934,9 @@ int rdb_colphys_diffuse_tracers(rdb_colphys* h, double dt) {
 
 int rdb_colphys_nonlocal(rdb_colphys* h, double dt) {
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "nonlocal: bad handle");
+  if (!h->members.empty())
+    return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "nonlocal: an ensemble handle is reached per member (rdb_colphys_member_*) "
+                      "and stepped with rdb_colphys_step");
   if (!h->cfg.use_kpp) return RDB_COLPHYS_OK;
   dim3 g(grid1(h->n_all, h->block).x, (unsigned)h->nz);
   hipLaunchKernelGGL(colphys::k_nonlocal, g, dim3(h->block), 0, h->stream, h->s, dt, 0, h->n_all);
@@ -799,23 +1025,25 @@ int rdb_colphys_autotune(rdb_colphys* h, double dt) {
   if (h->block_pinned) return RDB_COLPHYS_OK;
   int st = refresh_static(h, dt);
   if (st) return st;
-  // back up the evolving state (T, S, passives, bl_depth)
-  const size_t L = (size_t)h->nz * (size_t)h->P;
-  const size_t nstate = L * (size_t)(2 + h->n_passive) + (size_t)h->P;
+  // back up the evolving state (T, S, passives, bl_depth) of every grid
+  const std::vector<rdb_colphys*> ps = colphys::plans(h);
+  size_t nstate = 0;
+  for (rdb_colphys* m : ps) nstate += (size_t)m->nz * (size_t)m->P * (size_t)(2 + m->n_passive) + (size_t)m->P;
   double* bak = nullptr;
-  HIP_TRY(hipMalloc((void**)&bak, 8 * nstate));
+  HIP_TRY(hipMalloc((void**)&bak, 8 * std::max<size_t>(nstate, 1)));
   auto save = [&](bool restore) {
-    double* parts[3] = {h->s.T, h->s.S, h->s.tr};
-    size_t sizes[3] = {L, L, L * (size_t)h->n_passive};
     size_t off = 0;
-    for (int q = 0; q < 3; ++q) {
-      if (sizes[q])
-        (void)hipMemcpyAsync(restore ? parts[q] : bak + off, restore ? bak + off : parts[q], 8 * sizes[q],
-                       hipMemcpyDeviceToDevice, h->stream);
-      off += sizes[q];
+    for (rdb_colphys* m : ps) {
+      const size_t L = (size_t)m->nz * (size_t)m->P;
+      double* parts[4] = {m->s.T, m->s.S, m->s.tr, m->s.bl};
+      size_t sizes[4] = {L, L, L * (size_t)m->n_passive, (size_t)m->P};
+      for (int q = 0; q < 4; ++q) {
+        if (sizes[q])
+          (void)hipMemcpyAsync(restore ? parts[q] : bak + off, restore ? bak + off : parts[q], 8 * sizes[q],
+                               hipMemcpyDeviceToDevice, h->stream);
+        off += sizes[q];
+      }
     }
-    (void)hipMemcpyAsync(restore ? h->s.bl : bak + off, restore ? bak + off : h->s.bl, 8 * (size_t)h->P,
-                   hipMemcpyDeviceToDevice, h->stream);
   };
   save(false);
   // candidates: launch block x live levels of Thomas state held in LDS
@@ -836,10 +1064,11 @@ int rdb_colphys_autotune(rdb_colphys* h, double dt) {
   for (auto [b, mm] : cands) {
     h->block = b;
     h->lds_levels = mm;
-    st = launch_step(h, dt, 0, h->n_wet);  // warm-up
+    st = colphys::ensure_work(h);
+    if (!st) st = colphys::launch_step_all(h, dt, false);  // warm-up
     if (st) break;
     (void)hipEventRecord(e0, h->stream);
-    for (int r = 0; r < 3 && !st; ++r) st = launch_step(h, dt, 0, h->n_wet);
+    for (int r = 0; r < 3 && !st; ++r) st = colphys::launch_step_all(h, dt, false);
     (void)hipEventRecord(e1, h->stream);
     (void)hipEventSynchronize(e1);
     float ms = 0.f;
@@ -858,7 +1087,7 @@ int rdb_colphys_autotune(rdb_colphys* h, double dt) {
   h->block = best_b;
   h->lds_levels = best_m;
   if (st) return st;
-  return RDB_COLPHYS_OK;
+  return colphys::ensure_work(h);
 }
 
 int rdb_colphys_set_precision(rdb_colphys* h, int mode) {
@@ -867,6 +1096,10 @@ int rdb_colphys_set_precision(rdb_colphys* h, int mode) {
     return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "set_precision: unknown mode %d", mode);
   if (mode != h->precision) h->static_dirty = true;
   h->precision = mode;
+  for (rdb_colphys* m : h->members) {
+    if (mode != m->precision) m->static_dirty = true;
+    m->precision = mode;
+  }
   return RDB_COLPHYS_OK;
 }
 
@@ -892,10 +1125,13 @@ int rdb_colphys_heat_salt_totals(rdb_colphys* h, double* heat, double* salt) {
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "heat_salt_totals: bad handle");
   if (!heat || !salt) return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "heat_salt_totals: NULL output");
   HIP_TRY(hipMemsetAsync(h->d_acc, 0, 4 * sizeof(unsigned long long), h->stream));
-  int nb = std::min(2048, std::max(1, (h->n_wet + 255) / 256));
-  hipLaunchKernelGGL(colphys::k_totals, dim3(nb), dim3(256), 0, h->stream, h->s, 0, h->n_wet, h->d_acc);
-  int st = launch_check("k_totals");
-  if (st) return st;
+  for (rdb_colphys* m : colphys::plans(h)) {
+    // integer bins: the members' sums add exactly, in any order
+    int nb = std::min(2048, std::max(1, (m->n_wet + 255) / 256));
+    hipLaunchKernelGGL(colphys::k_totals, dim3(nb), dim3(256), 0, h->stream, m->s, 0, m->n_wet, h->d_acc);
+    int st = launch_check("k_totals");
+    if (st) return st;
+  }
   unsigned long long acc[4];
   HIP_TRY(hipMemcpyAsync(acc, h->d_acc, sizeof acc, hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -923,23 +1159,28 @@ int rdb_colphys_stats(rdb_colphys* h, rdb_colphys_counters* out) {
   if (!out) return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "stats: NULL output");
   std::memset(out, 0, sizeof *out);
   out->steps = h->steps;
-  out->wet_columns = h->n_wet;
-  out->land_columns = h->n_all - h->n_wet;
   out->last_kernel_ms = h->last_ms;
   out->sum_kernel_ms = h->sum_ms;
   out->timed_launches = h->timed;
-  const double L = (double)h->nz;
   // B_col = 8 (11 L + 3 (L + 1)) + 44 per wet column: reads h, u, v, T, S and
   // two passives, writes T, S, passives and kv/kt/ks, plus column scalars.
   // The step touches only live levels (bed fillers are static), so its
   // algorithmic figure uses each column's live-level count L_c.
-  out->reference_bytes_per_step = (8.0 * (11.0 * L + 3.0 * (L + 1.0)) + 44.0) * (double)h->n_wet;
-  const double Lsum = (double)h->live_levels;
-  out->algorithmic_bytes_per_step = 8.0 * (14.0 * Lsum + 3.0 * (double)h->n_wet) + 44.0 * (double)h->n_wet;
-  out->live_levels = h->live_levels;
+  long long wet = 0, land = 0, live = 0;
+  for (rdb_colphys* m : colphys::plans(h)) {
+    const double L = (double)m->nz;
+    out->reference_bytes_per_step += (8.0 * (11.0 * L + 3.0 * (L + 1.0)) + 44.0) * (double)m->n_wet;
+    wet += m->n_wet;
+    land += m->n_all - m->n_wet;
+    live += m->live_levels;
+  }
+  out->wet_columns = wet;
+  out->land_columns = land;
+  out->algorithmic_bytes_per_step = 8.0 * (14.0 * (double)live + 3.0 * (double)wet) + 44.0 * (double)wet;
+  out->live_levels = live;
   out->lds_levels = h->lds_levels;
   out->fallback_fraction =
-      h->closure == RDB_COLPHYS_CLOSURE_PLAIN ? 1.0 : (double)h->last_fallback / (double)std::max(h->n_wet, 1);
+      h->closure == RDB_COLPHYS_CLOSURE_PLAIN ? 1.0 : (double)h->last_fallback / (double)std::max(wet, 1LL);
   out->block_threads = h->block;
   out->order_mode = h->order_mode;
   out->mlp_forward_ms = h->last_mlp_ms;

@@ -71,4 +71,28 @@ struct rdb_colphys {
   long long last_fallback = 0;
   double last_mlp_ms = 0.0;
   double mlp_flop = 0.0;  // per step, unpadded layer widths
+  // batched ensemble: member plans (each its own grid and packed layout,
+  // sharing this handle's stream), stepped together by one persistent launch
+  std::vector<rdb_colphys*> members;
+  bool own_stream = true;                  // members borrow the parent's stream
+  colphys::StateView* d_views = nullptr;   // [members]
+  colphys::SurrView* d_svs = nullptr;      // [members]
+  int* d_nwet = nullptr;                   // [members]
+  int2* d_work = nullptr;                  // (member, first packed column) per work item
+  int n_work = 0;
+  int work_block = 0;                      // block size the work list was cut for
+  int persist_grid = 0;                    // workgroups of the persistent launch
+  int nz_net = 0;                          // ensemble: the largest member nz (network levels)
+  long long ens_wet = 0;                   // ensemble: wet columns over all members
+  long long row_off = 0;                   // member: first row in the ensemble's network batch
+  int lev_off = 0;                         // member: network levels below this grid's bed
 };
+
+namespace colphys {
+// the member plans of a handle (itself when it is a single grid)
+inline std::vector<rdb_colphys*> plans(rdb_colphys* h) {
+  return h->members.empty() ? std::vector<rdb_colphys*>{h} : h->members;
+}
+int launch_step_all(rdb_colphys* h, double dt, bool surr);
+int ensure_work(rdb_colphys* h);
+}  // namespace colphys

@@ -56,18 +56,28 @@ __global__ void k_nonlocal(StateView s, double dt, int p0, int p1);
 // Learned-closure inputs of the fused step: the network's outputs stored
 // output-major (Yt[n * ldy + p], p the packed wet column), their
 // normalisation, and a per-column guard flag (non-zero: plain path).
+// Learned-closure predictions for the fused step: Yt is output-major
+// (output n of packed column p at Yt[n * ldy + p], normalised by mean/stdv),
+// guard[p] != 0 sends column p down the plain path.  The network is laid out
+// on nz + lev_off levels with the surfaces aligned: interface kf of this
+// grid is the network's interface kf + lev_off.
 struct SurrView {
   const float* Yt;
   long long ldy;
   const float* mean;
   const float* stdv;
   const unsigned char* guard;
+  int lev_off;
+  int pad;
 };
 
 __global__ void k_prepare(StateView s, Params pr, int p0, int p1);
 template <typename R, int NTR, bool TWO_FACTOR, bool SURR>
 __global__ void k_column_step(StateView s, Params pr, double dt, int p0, int p1, unsigned int* guard_count, int M,
                               SurrView sv);
+template <typename R, int NTR, bool TWO_FACTOR, bool SURR>
+__global__ void k_column_step_ens(const StateView* views, const SurrView* svs, const int* n_wet, Params pr, double dt,
+                                  const int2* work, int n_work, unsigned int* guard_count, int M);
 __global__ void k_totals(StateView s, int p0, int p1, unsigned long long* acc);
 
 }  // namespace colphys

@@ -376,11 +376,8 @@ __global__ void k_prepare(StateView s, Params pr, int p0, int p1) {
 // identical arithmetic on its bitwise copy ks = kt).
 // ---------------------------------------------------------------------------
 template <typename R, int NTR, bool TWO_FACTOR, bool SURR>
-__global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, double dt, int p0, int p1,
-                                                     unsigned int* guard_count, int M, SurrView sv) {
-  extern __shared__ double lds[];
-  int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= p1) return;
+__device__ __forceinline__ void column_step(const StateView& s, const Params& pr, double dt, int p,
+                                            unsigned int* guard_count, int M, const SurrView& sv, double* lds) {
   const int P = s.P, nz = s.nz;
   const double HV = H_VANISHED;
   const R rho0 = R(pr.rho0);
@@ -413,7 +410,7 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
     qT_r = R(qh) / (rho0 * R(pr.cp));
     qS_r = R(qsv) / rho0;
     R B0 = kpp_b0<R>(pr, qT_r, qS_r);
-    R h_b = fmx(R(pred(2 * (nz - 1))), R(0));
+    R h_b = fmx(R(pred(2 * (nz + sv.lev_off - 1))), R(0));
     s.b0[p] = (double)B0;
     destab = (B0 < R(0));
     R w_star = cbrt_kahan(fmx(R(0), -B0) * h_b);
@@ -549,8 +546,9 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
     double kvv, ktv;
     if (use_s) {
       // predicted log10 kt (outputs 0..nz-2) and log10 kv (nz-1..2nz-3)
-      ktv = exp2(pred(kf - 1) * 3.321928094887362);
-      kvv = exp2(pred(nz - 1 + kf - 1) * 3.321928094887362);
+      const int i_net = kf + sv.lev_off;  // the network's interface
+      ktv = exp2(pred(i_net - 1) * 3.321928094887362);
+      kvv = exp2(pred(nz + sv.lev_off - 1 + i_net - 1) * 3.321928094887362);
     } else {
       R kv_r, kt_r;
       pp81_iface<R>(pr, ub_lo, vb_lo, ub_hi, vb_hi, R(h_lo), R(h_hi), rho_hi - rho_lo, kv_r, kt_r);
@@ -747,6 +745,36 @@ __global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, dou
 #undef LSLOT
 }
 
+template <typename R, int NTR, bool TWO_FACTOR, bool SURR>
+__global__ void __launch_bounds__(256) k_column_step(StateView s, Params pr, double dt, int p0, int p1,
+                                                     unsigned int* guard_count, int M, SurrView sv) {
+  extern __shared__ double lds[];
+  const int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < p1) column_step<R, NTR, TWO_FACTOR, SURR>(s, pr, dt, p, guard_count, M, sv, lds);
+}
+
+// The ensemble form: one persistent launch over the wet columns of several
+// members (each with its own grid, nz and packed layout).  Work item w is the
+// column chunk [ck.y, ck.y + blockDim.x) of member ck.x; workgroups take
+// items w = blockIdx.x, blockIdx.x + gridDim.x, ... so every workgroup
+// leaves after a bounded number of items.
+template <typename R, int NTR, bool TWO_FACTOR, bool SURR>
+__global__ void __launch_bounds__(256) k_column_step_ens(const StateView* __restrict__ views,
+                                                         const SurrView* __restrict__ svs, const int* __restrict__ n_wet,
+                                                         Params pr, double dt, const int2* __restrict__ work,
+                                                         int n_work, unsigned int* guard_count, int M) {
+  extern __shared__ double lds[];
+  for (int w = blockIdx.x; w < n_work; w += gridDim.x) {
+    const int2 ck = work[w];
+    const int p = ck.y + threadIdx.x;
+    if (p < n_wet[ck.x]) {
+      const StateView s = views[ck.x];
+      const SurrView sv = SURR ? svs[ck.x] : SurrView{};
+      column_step<R, NTR, TWO_FACTOR, SURR>(s, pr, dt, p, guard_count, M, sv, lds);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // A17 order-invariant integrals: each h*c product is rounded to a multiple of
 // 2^-44 and split into a signed high word and an unsigned low 32-bit word;
@@ -786,9 +814,11 @@ __global__ void k_totals(StateView s, int p0, int p1, unsigned long long* acc) {
 }
 
 // Explicit instantiations of the fused step.
-#define INST(R, N, TF, SU)                                                                             \
+#define INST(R, N, TF, SU)                                                                                      \
   template __global__ void k_column_step<R, N, TF, SU>(StateView, Params, double, int, int, unsigned int*, int, \
-                                                       SurrView);
+                                                       SurrView);                                               \
+  template __global__ void k_column_step_ens<R, N, TF, SU>(const StateView*, const SurrView*, const int*, Params, \
+                                                           double, const int2*, int, unsigned int*, int);
 INST(double, 0, false, false) INST(double, 1, false, false) INST(double, 2, false, false)
 INST(double, 3, false, false) INST(double, 4, false, false)
 INST(double, 0, true, false) INST(double, 1, true, false) INST(double, 2, true, false)

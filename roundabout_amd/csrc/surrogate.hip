@@ -86,10 +86,15 @@ __device__ __forceinline__ bf16 f2bf(float f) {
   return (bf16)(u >> 16);
 }
 
-// feature f of packed column p (raw, un-normalised)
-__device__ __forceinline__ float feature(const StateView& s, const double* cor, int p, int f, double depth) {
+// feature f of packed column p (raw, un-normalised).  The network has
+// nzn >= nz levels with the surfaces aligned: network level j is this
+// grid's level j - lev_off; the lev_off lowest network levels do not exist
+// here and read as bed fillers (h = 1e-4 m, no tracer).
+__device__ __forceinline__ float feature(const StateView& s, const double* cor, int p, int f, double depth, int nzn,
+                                         int lev_off) {
   const int P = s.P, nz = s.nz;
-  const int blk = f / nz, k = f % nz;
+  const int blk = f / nzn, k = f % nzn - lev_off;
+  if (blk < 5 && k < 0) return blk == 4 ? -4.0f : 0.f;
   if (blk < 5) {
     const double hk = LV(s.h, k);
     if (blk == 4) return (float)log10(fmax(hk, 1e-30));
@@ -99,7 +104,7 @@ __device__ __forceinline__ float feature(const StateView& s, const double* cor, 
     if (k + 1 < nz && LV(s.h, k + 1) > H_VANISHED) v -= LV(a, k + 1);
     return (float)v;
   }
-  switch (f - 5 * nz) {
+  switch (f - 5 * nzn) {
     case 0: return (float)s.tau_tot[p];
     case 1: return (float)s.Qh[p];
     case 2: return (float)s.Qs[p];
@@ -120,37 +125,42 @@ __device__ __forceinline__ double live_depth(const StateView& s, int p) {
 }
 
 // Raw features of packed columns c = i * stride (i < ncols) into rows row0 + i.
-__global__ void k_features_raw(StateView s, const double* cor, int ncols, int stride, int din, float* X, long long row0) {
+__global__ void k_features_raw(StateView s, const double* cor, int ncols, int stride, int din, float* X, long long row0,
+                               int nzn, int lev_off) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ncols) return;
   int p = i * stride;
   double depth = live_depth(s, p);
   float* row = X + (size_t)(row0 + i) * (size_t)din;
-  for (int f = 0; f < din; ++f) row[f] = feature(s, cor, p, f, depth);
+  for (int f = 0; f < din; ++f) row[f] = feature(s, cor, p, f, depth, nzn, lev_off);
 }
 
 // Targets of the same columns after the plain step: log10 kt, log10 kv at
-// interfaces 1..nz-1, bl_depth; weights 1 on interfaces >= kb.
+// the network's interfaces 1..nzn-1, bl_depth; weight 1 on interfaces at or
+// above this grid's first live layer kb, 0 below it and on interfaces the
+// grid does not have.
 __global__ void k_targets_raw(StateView s, int ncols, int stride, int dout, float* T, float* W, float* rowsum,
-                              long long row0) {
+                              long long row0, int nzn, int lev_off) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ncols) return;
-  const int P = s.P, nz = s.nz;
+  const int P = s.P;
   int p = i * stride;
   const int kb = s.kb[p];
   float* t = T + (size_t)(row0 + i) * (size_t)dout;
   float* w = W + (size_t)(row0 + i) * (size_t)dout;
   float sum = 0.f;
-  for (int k = 1; k < nz; ++k) {
-    float wk = (k >= kb) ? 1.f : 0.f;
-    t[k - 1] = (float)log10(fmax(LV(s.kt, k), 1e-30));
-    t[nz - 1 + k - 1] = (float)log10(fmax(LV(s.kv, k), 1e-30));
-    w[k - 1] = wk;
-    w[nz - 1 + k - 1] = wk;
+  for (int j = 1; j < nzn; ++j) {
+    const int k = j - lev_off;  // this grid's interface
+    const bool has = k >= 1;
+    float wk = (has && k >= kb) ? 1.f : 0.f;
+    t[j - 1] = has ? (float)log10(fmax(LV(s.kt, k), 1e-30)) : 0.f;
+    t[nzn - 1 + j - 1] = has ? (float)log10(fmax(LV(s.kv, k), 1e-30)) : 0.f;
+    w[j - 1] = wk;
+    w[nzn - 1 + j - 1] = wk;
     sum += 2.f * wk;
   }
-  t[2 * (nz - 1)] = (float)s.bl[p];
-  w[2 * (nz - 1)] = 1.f;
+  t[2 * (nzn - 1)] = (float)s.bl[p];
+  w[2 * (nzn - 1)] = 1.f;
   rowsum[row0 + i] = sum + 1.f;
 }
 
@@ -240,7 +250,8 @@ __global__ void k_gather(const bf16* Xn, int ldx, const float* Tn, const float* 
 constexpr int FT = 64;
 __global__ void __launch_bounds__(FT) k_features_infer(StateView s, const double* cor, int n, int din, int ld,
                                                      const float* st, bf16* X0, unsigned char* guard,
-                                                     unsigned int* nfall, int guard_mode, double rho0, double p_ref) {
+                                                     unsigned int* nfall, int guard_mode, double rho0, double p_ref,
+                                                     int nzn, int lev_off) {
   __shared__ bf16 tile[384][FT + 1];
   const float* mean = st;
   const float* stdv = st + din;
@@ -253,7 +264,7 @@ __global__ void __launch_bounds__(FT) k_features_infer(StateView s, const double
     double depth = live_depth(s, p);
     bool out_of_range = false;
     for (int f = 0; f < din; ++f) {
-      float v = feature(s, cor, p, f, depth);
+      float v = feature(s, cor, p, f, depth, nzn, lev_off);
       out_of_range |= (v < mn[f] || v > mx[f]);
       tile[f][tid] = f2bf((v - mean[f]) / stdv[f]);
     }
@@ -324,23 +335,31 @@ void SurrState::free_dataset() {
 
 void surr_free(SurrState* s) { delete s; }
 
+// levels of the network: the grid's nz, or the largest member nz
+static int net_levels(rdb_colphys* h) { return h->members.empty() ? h->nz : h->nz_net; }
+static long long wet_rows(rdb_colphys* h) { return h->members.empty() ? (long long)h->n_wet : h->ens_wet; }
+
 static SurrState* surr_get(rdb_colphys* h) {
   if (!h->surr) {
+    const int nzn = net_levels(h);
     h->surr = new SurrState();
-    h->surr->nz = h->nz;
-    h->surr->din = 5 * h->nz + 6;
-    h->surr->dout = 2 * (h->nz - 1) + 1;
+    h->surr->nz = nzn;
+    h->surr->din = 5 * nzn + 6;
+    h->surr->dout = 2 * (nzn - 1) + 1;
   }
   return h->surr;
 }
 
 int surr_collect(rdb_colphys* h, double dt, int n_steps, int stride, long long* n_out) {
-  if (h->nz < 2) return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "mlp_collect: needs nz >= 2");
-  if (5 * h->nz + 6 > 384) return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "mlp_collect: nz > 75 exceeds the feature tile");
+  const int nzn = net_levels(h);
+  if (nzn < 2) return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "mlp_collect: needs nz >= 2");
+  if (5 * nzn + 6 > 384) return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "mlp_collect: nz > 75 exceeds the feature tile");
   if (stride < 1) stride = 1;
   SurrState* S = surr_get(h);
-  const int ncols = (h->n_wet + stride - 1) / stride;
-  const long long need = S->n + (long long)ncols * n_steps;
+  const std::vector<rdb_colphys*> ps = plans(h);
+  long long ncols = 0;
+  for (rdb_colphys* m : ps) ncols += (m->n_wet + stride - 1) / stride;
+  const long long need = S->n + ncols * n_steps;
   if (need > S->cap) {
     // grow (copying what is already collected)
     const long long cap = std::max(need, S->cap * 2);
@@ -368,12 +387,24 @@ int surr_collect(rdb_colphys* h, double dt, int n_steps, int stride, long long* 
   int st = refresh_static(h, dt);
   if (st) return st;
   for (int i = 0; i < n_steps; ++i) {
-    hipLaunchKernelGGL(k_features_raw, gs(ncols, 128), dim3(128), 0, h->stream, h->s, h->d_coriolis, ncols, stride,
-                       S->din, S->Xraw, S->n);
-    st = launch_step(h, dt, 0, h->n_wet);
+    long long row = S->n;
+    for (rdb_colphys* m : ps) {
+      const int nc = (m->n_wet + stride - 1) / stride;
+      if (nc > 0)
+        hipLaunchKernelGGL(k_features_raw, gs(nc, 128), dim3(128), 0, h->stream, m->s, m->d_coriolis, nc, stride,
+                           S->din, S->Xraw, row, nzn, nzn - m->nz);
+      row += nc;
+    }
+    st = launch_step_all(h, dt, false);
     if (st) return st;
-    hipLaunchKernelGGL(k_targets_raw, gs(ncols, 128), dim3(128), 0, h->stream, h->s, ncols, stride, S->dout, S->Traw,
-                       S->Wraw, S->rowsum, S->n);
+    row = S->n;
+    for (rdb_colphys* m : ps) {
+      const int nc = (m->n_wet + stride - 1) / stride;
+      if (nc > 0)
+        hipLaunchKernelGGL(k_targets_raw, gs(nc, 128), dim3(128), 0, h->stream, m->s, nc, stride, S->dout, S->Traw,
+                           S->Wraw, S->rowsum, row, nzn, nzn - m->nz);
+      row += nc;
+    }
     SG_TRY(hipGetLastError());
     S->n += ncols;
     h->steps += 1;
@@ -532,7 +563,11 @@ int surr_step(rdb_colphys* h, double dt) {
   mlp::Net& net = S->net;
   if (net.pin[0] > 384) return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "surrogate: input width exceeds the feature tile");
   if (h->n_passive > 4) return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "surrogate: at most 4 passive tracers");
-  const int Mpad = mlp::round_up(std::max(h->n_wet, 1), 128);  // the fused inference kernel's row block
+  const int nzn = net_levels(h);
+  if (S->din != 5 * nzn + 6) return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "surrogate: network built for another nz");
+  const long long rows = wet_rows(h);
+  if (rows > 0x7fffff00LL) return perr(RDB_COLPHYS_ERR_BAD_SHAPE, "surrogate: too many wet columns");
+  const int Mpad = mlp::round_up((int)std::max<long long>(rows, 1), 128);  // the fused inference kernel's row block
   if (S->inf_rows != Mpad) {
     int e = net.ensure_workspace(Mpad, false);
     if (e) return perr(e, "surrogate: %s", net.err.c_str());
@@ -542,10 +577,15 @@ int surr_step(rdb_colphys* h, double dt) {
     if (!S->nfall) SG_TRY(hipMalloc((void**)&S->nfall, 64));
     S->inf_rows = Mpad;
   }
+  const std::vector<rdb_colphys*> ps = plans(h);
   SG_TRY(hipMemsetAsync(S->nfall, 0, 4, h->stream));
-  hipLaunchKernelGGL(k_features_infer, gs(h->n_wet, FT), dim3(FT), 0, h->stream, h->s, h->d_coriolis, h->n_wet,
-                     S->din, net.pin[0], S->d_stats, net.X0, S->guard, S->nfall, S->guard_mode, h->pr.rho0,
-                     h->pr.p_ref);
+  const int ld = net.pin[0];
+  for (rdb_colphys* m : ps) {
+    if (m->n_wet == 0) continue;
+    hipLaunchKernelGGL(k_features_infer, gs(m->n_wet, FT), dim3(FT), 0, h->stream, m->s, m->d_coriolis, m->n_wet,
+                       S->din, ld, S->d_stats, net.X0 + (size_t)m->row_off * (size_t)ld, S->guard + m->row_off,
+                       S->nfall, S->guard_mode, h->pr.rho0, h->pr.p_ref, nzn, nzn - m->nz);
+  }
   SG_TRY(hipGetLastError());
   if (h->timing) {
     if (!S->ev[0]) SG_TRY(hipEventCreate(&S->ev[0]));
@@ -557,13 +597,19 @@ int surr_step(rdb_colphys* h, double dt) {
   if (h->timing) SG_TRY(hipEventRecord(S->ev[1], h->stream));
   double flop = 0.0;
   for (int l = 0; l < net.nl; ++l) flop += 2.0 * (double)net.din[l] * (double)net.dout[l];
-  h->mlp_flop = flop * (double)h->n_wet;
-  const SurrView sv{net.Y, (long long)Mpad, S->d_stats + 4 * S->din, S->d_stats + 4 * S->din + S->dout, S->guard};
-  const bool two = (h->cfg.kt_bg != h->cfg.ks_bg);
-  const int ntr = h->n_passive;
-  dim3 g((unsigned)((h->n_wet + h->block - 1) / h->block)), b((unsigned)h->block);
-  const int M = h->lds_levels;
-  const size_t shmem = (size_t)h->block * (size_t)M * (size_t)(3 + std::min(ntr, 4) + (two ? 1 : 0)) * 8;
+  h->mlp_flop = flop * (double)rows;
+  const float* tmean = S->d_stats + 4 * S->din;
+  const float* tstd = tmean + S->dout;
+  auto view_of = [&](rdb_colphys* m) {
+    return SurrView{net.Y + m->row_off, (long long)Mpad, tmean, tstd, S->guard + m->row_off, nzn - m->nz, 0};
+  };
+  if (h->members.empty()) {
+    const SurrView sv = view_of(h);
+    const bool two = (h->cfg.kt_bg != h->cfg.ks_bg);
+    const int ntr = h->n_passive;
+    dim3 g((unsigned)((h->n_wet + h->block - 1) / h->block)), b((unsigned)h->block);
+    const int M = h->lds_levels;
+    const size_t shmem = (size_t)h->block * (size_t)M * (size_t)(3 + std::min(ntr, 4) + (two ? 1 : 0)) * 8;
 #define LAUNCH(N, TF)                                                                                              \
   hipLaunchKernelGGL((k_column_step<double, N, TF, true>), g, b, shmem, h->stream, h->s, h->pr, dt, 0, h->n_wet, \
                      h->d_guard, M, sv)
@@ -575,17 +621,26 @@ int surr_step(rdb_colphys* h, double dt) {
     case 3: LAUNCH(3, TF); break;   \
     default: LAUNCH(4, TF); break;  \
   }
-  if (two) {
-    BY_NTR(true)
-  } else {
-    BY_NTR(false)
-  }
+    if (h->n_wet > 0) {
+      if (two) {
+        BY_NTR(true)
+      } else {
+        BY_NTR(false)
+      }
+    }
 #undef BY_NTR
 #undef LAUNCH
-  SG_TRY(hipGetLastError());
+    SG_TRY(hipGetLastError());
+  } else {
+    std::vector<SurrView> svs;
+    for (rdb_colphys* m : ps) svs.push_back(view_of(m));
+    SG_TRY(hipMemcpyAsync(h->d_svs, svs.data(), sizeof(SurrView) * svs.size(), hipMemcpyHostToDevice, h->stream));
+    int st = launch_step_all(h, dt, true);
+    if (st) return st;
+  }
   unsigned int nf = 0;
   SG_TRY(hipMemcpyAsync(&nf, S->nfall, 4, hipMemcpyDeviceToHost, h->stream));
-  SG_TRY(hipStreamSynchronize(h->stream));
+  SG_TRY(hipStreamSynchronize(h->stream));  // also keeps svs alive until its copy is done
   h->last_fallback = nf;
   if (h->timing) {
     float ms = 0.f;
@@ -619,9 +674,9 @@ int surr_load(rdb_colphys* h, const char* path) {
   char magic[8];
   int hdr[4];
   bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "RDBSURR1", 8) == 0 && std::fread(hdr, 4, 4, f) == 4;
-  if (!ok || hdr[0] != h->nz || hdr[3] < 2 || hdr[3] > 16) {
+  if (!ok || hdr[0] != net_levels(h) || hdr[3] < 2 || hdr[3] > 16) {
     std::fclose(f);
-    return perr(RDB_COLPHYS_ERR_BAD_SHAPE, "mlp_load: %s is not a network for nz = %d", path, h->nz);
+    return perr(RDB_COLPHYS_ERR_BAD_SHAPE, "mlp_load: %s is not a network for nz = %d", path, net_levels(h));
   }
   SurrState* S = surr_get(h);
   std::vector<int> widths((size_t)hdr[3]);

@@ -209,6 +209,14 @@ PROTOTYPES = [
     ("rdb_colphys_mlp_save", ctypes.c_int, [_vp, ctypes.c_char_p]),
     ("rdb_colphys_mlp_load", ctypes.c_int, [_vp, ctypes.c_char_p]),
     ("rdb_colphys_set_guard", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("rdb_colphys_ensemble_create", ctypes.c_int,
+     [ctypes.POINTER(Config), ctypes.c_int, _ip, _ip, _ip, ctypes.c_int, ctypes.POINTER(_dp),
+      ctypes.POINTER(_ip), _hp]),
+    ("rdb_colphys_member_count", ctypes.c_int, [_vp, _ip]),
+    ("rdb_colphys_member_set_field", ctypes.c_int,
+     [_vp, ctypes.c_int, ctypes.c_char_p, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("rdb_colphys_member_get_field", ctypes.c_int,
+     [_vp, ctypes.c_int, ctypes.c_char_p, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("rdb_mlp_eval", ctypes.c_int,
      [_ip, ctypes.c_int, _fp, ctypes.c_int, _fp, _fp]),
     ("rdb_mlp_loss_grad", ctypes.c_int,

@@ -217,3 +217,111 @@ class ColumnPhysics:
         c = _ffi.Counters()
         _ffi.check(self._lib.rdb_colphys_stats(self._handle(), ctypes.byref(c)), self._lib)
         return {f: getattr(c, f) for f, _ in _ffi.Counters._fields_}
+
+
+class ColumnEnsemble(ColumnPhysics):
+    """Several independent members stepped together
+    (``rdb_colphys_ensemble_create``): one persistent launch per step over
+    every member's wet columns.  Fields are reached per member; stepping,
+    the learned closure, totals and counters cover all members."""
+
+    def __init__(self, grids, *, n_passive: int = 0, config=None, closures=(), sort_keys=None, lib=None):
+        """``grids``: sequence of (nx, ny, nz, wet) tuples."""
+        self._lib = lib or _ffi.load()
+        cfg = config if config is not None else _ffi.default_config()
+        for c in closures:
+            c.apply(cfg)
+        self.config = cfg
+        self.n_passive = int(n_passive)
+        self.dims = [(int(nx), int(ny), int(nz)) for nx, ny, nz, _ in grids]
+        n = len(self.dims)
+        wets = []
+        for (nx, ny, _), (_, _, _, wet) in zip(self.dims, grids):
+            w = np.asfortranarray(wet, dtype=np.float64)
+            if w.shape != (nx, ny):
+                raise ValueError(f"wet mask shape {w.shape} != ({nx}, {ny})")
+            wets.append(w)
+        keys = None
+        if sort_keys is not None:
+            keys = [np.asfortranarray(k, dtype=np.int32) for k in sort_keys]
+        self._keep = (wets, keys)  # the library copies them; keep until create returns
+        IntN = ctypes.c_int * n
+        wet_ptrs = (_dp * n)(*[w.ctypes.data_as(_dp) for w in wets])
+        key_ptrs = (_ip * n)(*[k.ctypes.data_as(_ip) for k in keys]) if keys is not None else None
+        handle = ctypes.c_void_p()
+        st = self._lib.rdb_colphys_ensemble_create(
+            ctypes.byref(cfg), n, IntN(*[d[0] for d in self.dims]), IntN(*[d[1] for d in self.dims]),
+            IntN(*[d[2] for d in self.dims]), self.n_passive, wet_ptrs, key_ptrs, ctypes.byref(handle))
+        _ffi.check(st, self._lib)
+        self._h = handle
+        self._keep = None
+
+    @classmethod
+    def from_oceans(cls, oceans, config=None, closures=(), sort: bool = True, lib=None) -> "ColumnEnsemble":
+        n_passive = {o.n_passive for o in oceans}
+        if len(n_passive) != 1:
+            raise ValueError("ensemble members must carry the same number of passive tracers")
+        ens = cls([(o.nx, o.ny, o.nz, o.wet) for o in oceans], n_passive=n_passive.pop(), config=config,
+                  closures=closures, sort_keys=[o.n_live for o in oceans] if sort else None, lib=lib)
+        for m, o in enumerate(oceans):
+            ens.upload_member(m, o)
+        return ens
+
+    @property
+    def n_members(self) -> int:
+        return len(self.dims)
+
+    def member_shape(self, member: int, name: str) -> tuple[int, int, int]:
+        if not 0 <= member < len(self.dims):
+            return 1, 1, 1  # the library reports the unknown member (RDB_COLPHYS_ERR_NOT_FOUND)
+        nx, ny, nz = self.dims[member]
+        self.nx, self.ny, self.nz = nx, ny, nz
+        return ColumnPhysics.shape_of(self, name)
+
+    def member_set_field(self, member: int, name: str, data) -> None:
+        n1, n2, n3 = self.member_shape(member, name)
+        arr = np.asfortranarray(data, dtype=np.float64)
+        if arr.size != n1 * n2 * n3:
+            raise ValueError(f"{name}: {arr.shape} does not hold {(n1, n2, n3)}")
+        st = self._lib.rdb_colphys_member_set_field(self._handle(), int(member), name.encode(), arr.ctypes.data_as(_dp),
+                                                    n1, n2, n3)
+        _ffi.check(st, self._lib)
+
+    def member_get_field(self, member: int, name: str) -> np.ndarray:
+        n1, n2, n3 = self.member_shape(member, name)
+        out = np.empty((n1, n2, n3), order="F")
+        st = self._lib.rdb_colphys_member_get_field(self._handle(), int(member), name.encode(),
+                                                    out.ctypes.data_as(_dp), n1, n2, n3)
+        _ffi.check(st, self._lib)
+        if name in COLUMN_FIELDS or name in ("tau_x", "tau_y"):
+            out = out[:, :, 0]
+        if name == "kbl":
+            out = out.astype(np.int32)
+        return out
+
+    def upload_member(self, member: int, ocean) -> None:
+        for name, data in (("h_layer", ocean.h), ("u_face_x_layer", ocean.u), ("v_face_y_layer", ocean.v),
+                           ("temperature", ocean.T), ("salinity", ocean.S), ("tau_x", ocean.tau_x),
+                           ("tau_y", ocean.tau_y), ("Q_heat", ocean.Q_heat), ("Q_salt", ocean.Q_salt),
+                           ("bl_depth", ocean.bl_depth), ("coriolis", ocean.coriolis)):
+            self.member_set_field(member, name, data)
+        for i, p in enumerate(ocean.passive):
+            self.member_set_field(member, f"tracer_{i + 1}", p)
+
+    def upload(self, oceans) -> None:
+        for m, o in enumerate(oceans):
+            self.upload_member(m, o)
+
+    # the single-grid field calls go to the library, which refuses them on an
+    # ensemble handle (RDB_COLPHYS_ERR_BAD_HANDLE): use the member_* forms
+    def set_field(self, name: str, data) -> None:
+        arr = np.asfortranarray(data, dtype=np.float64)
+        n = arr.shape + (1,) * (3 - arr.ndim)
+        st = self._lib.rdb_colphys_set_field(self._handle(), name.encode(), arr.ctypes.data_as(_dp), *n[:3])
+        _ffi.check(st, self._lib)
+
+    def get_field(self, name: str) -> np.ndarray:
+        out = np.empty((1, 1, 1), order="F")
+        st = self._lib.rdb_colphys_get_field(self._handle(), name.encode(), out.ctypes.data_as(_dp), 1, 1, 1)
+        _ffi.check(st, self._lib)
+        return out

@@ -1,0 +1,125 @@
+"""Batched ensembles on the GPU: several members of different grids and nz
+stepped by one persistent launch per step.
+
+The plain ensemble must reproduce, bit for bit, each member stepped alone
+(which the single-grid parity tests tie to the oracle); the guarded learned
+closure over an uneven-nz ensemble must conserve heat and salt and is judged
+against the all-plain ensemble.
+"""
+
+from __future__ import annotations
+
+import json
+
+import numpy as np
+import pytest
+
+from roundabout_amd import synth
+from roundabout_amd.rdb import (CLOSURE_SURROGATE, GUARD_SURFACE, ColumnEnsemble, ColumnPhysics, RdbError,
+                                default_config)
+
+pytestmark = pytest.mark.gpu
+DT = 3600.0
+
+
+def members(seed=40):
+    return [
+        synth.small(seed),                                                        # 64 x 64 x 20 z-star
+        synth.tile(seed + 1, nx=56, ny=40),                                       # z-level, nz 75
+        synth.make_ocean(48, 36, 50, seed + 2, land_frac=0.3, topo="zlevel", dmin=50.0, dmax=5500.0),
+        synth.thin(seed + 3, nx=40, ny=32),                                       # thin layers, nz 75
+        synth.make_ocean(24, 20, 50, seed + 4, land_frac=0.3, topo="zlevel", dmin=50.0, dmax=5500.0),
+    ]
+
+
+FIELDS = ("temperature", "salinity", "tracer_1", "tracer_2", "kv", "kt", "ks", "bl_depth", "kbl", "b0")
+
+
+@pytest.mark.parametrize("nsteps", [1, 24])
+def test_ensemble_matches_members_alone_bitwise(gpu_lib, nsteps):
+    print(f"[case] ensemble of 5 (nz 20/75/50/75/50), {nsteps} steps, bitwise vs members alone", flush=True)
+    oceans = members()
+    cfg = default_config()
+    alone = []
+    for o in oceans:
+        with ColumnPhysics.from_ocean(o, config=cfg) as cp:
+            cp.step(DT, nsteps)
+            alone.append({f: cp.get_field(f) for f in FIELDS})
+    with ColumnEnsemble.from_oceans(oceans, config=cfg) as ens:
+        assert ens.n_members == 5
+        ens.step(DT, nsteps)
+        for m in range(len(oceans)):
+            for f in FIELDS:
+                got = ens.member_get_field(m, f)
+                assert np.array_equal(got, alone[m][f]), (m, f, float(np.nanmax(np.abs(got - alone[m][f]))))
+        st = ens.stats()
+        assert st["wet_columns"] == sum(o.n_wet for o in oceans)
+
+
+def test_ensemble_autotune_and_graph_replay(gpu_lib):
+    print("[case] ensemble autotune + repeated graph replays, bitwise vs untuned", flush=True)
+    oceans = members(50)
+    cfg = default_config()
+    with ColumnEnsemble.from_oceans(oceans, config=cfg) as ens:
+        ens.step(DT, 6)
+        ref = [ens.member_get_field(m, "temperature") for m in range(len(oceans))]
+    with ColumnEnsemble.from_oceans(oceans, config=cfg) as ens:
+        ens.autotune(DT)  # restores the state it times on
+        for _ in range(3):
+            ens.step(DT, 2)  # one cached graph, replayed
+        for m in range(len(oceans)):
+            assert np.array_equal(ens.member_get_field(m, "temperature"), ref[m])
+
+
+def test_ensemble_refuses_single_grid_entry_points(gpu_lib):
+    print("[case] ensemble error paths", flush=True)
+    oceans = members(60)[:2]
+    with ColumnEnsemble.from_oceans(oceans) as ens:
+        with pytest.raises(RdbError):
+            ens.set_field("temperature", oceans[0].T)
+        with pytest.raises(RdbError):
+            ens.eos()
+        with pytest.raises(RdbError):
+            ens.member_get_field(2, "temperature")
+        with pytest.raises(RdbError):
+            ColumnPhysics.from_ocean(oceans[0])  # one live handle
+
+
+def test_ensemble_guarded_surrogate(gpu_lib, tmp_path):
+    print("[case] ensemble surrogate: train on redrawn members, 24 guarded steps vs all-plain", flush=True)
+    oceans = members(70)
+    cfg = default_config()
+    with ColumnEnsemble.from_oceans(oceans, config=cfg) as ens:
+        ens.step(DT, 24)
+        T_ref = [ens.member_get_field(m, "temperature") for m in range(len(oceans))]
+    net = tmp_path / "ens.bin"
+    with ColumnEnsemble.from_oceans([synth.redraw(o, 700 + i) for i, o in enumerate(oceans)], config=cfg) as ens:
+        n = ens.mlp_collect(DT, 12)
+        tl, vl = ens.mlp_train(epochs=60, batch=4096, guard_mode=GUARD_SURFACE)
+        ens.mlp_save(net)
+    assert n == 12 * sum(o.n_wet for o in oceans)
+    assert vl[-1] < 0.5 * vl[0]
+    with ColumnEnsemble.from_oceans(oceans, config=cfg) as ens:
+        ens.mlp_load(net)
+        ens.set_guard(GUARD_SURFACE)
+        ens.set_closure(CLOSURE_SURROGATE)
+        h0, s0 = ens.heat_salt_totals()
+        ens.step(DT, 24)
+        h1, s1 = ens.heat_salt_totals()
+        fb = ens.stats()["fallback_fraction"]
+        T = [ens.member_get_field(m, "temperature") for m in range(len(oceans))]
+        kbl = [ens.member_get_field(m, "kbl") for m in range(len(oceans))]
+    dep = 24 * DT / (cfg.rho0 * cfg.cp) * sum(float((o.Q_heat * o.wet).sum()) for o in oceans)
+    assert abs((h1 - h0) - dep) <= 1e-12 * abs(h0)
+    assert abs(s1 - s0) <= 1e-13 * abs(s0)
+    rms = []
+    for o, a, b in zip(oceans, T, T_ref):
+        live = (o.h > 1.5e-4) & (o.wet > 0.5)[:, :, None]
+        d = (a - b)[live]
+        rms.append(float(np.sqrt(np.mean(d * d))))
+    used = sum(int((k[o.wet > 0.5] == -1).sum()) for o, k in zip(oceans, kbl))
+    print("ensemble-surrogate " + json.dumps({"T_rms_K_per_member": rms, "fallback_fraction": fb,
+                                              "surrogate_columns_last_step": used, "val_loss": [vl[0], vl[-1]]}),
+          flush=True)
+    assert all(np.isfinite(r) for r in rms) and max(rms) < 0.2
+    assert 0.0 <= fb < 1.0 and used > 0
