@@ -28,18 +28,23 @@ __device__ __forceinline__ bf16 f2bf(float f) {
   return (bf16)(u >> 16);
 }
 
-// GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))).
-__device__ __forceinline__ float gelu(float x) {
+// GELU, tanh form: 0.5 x (1 + tanh(u)), u = sqrt(2/pi) (x + 0.044715 x^3),
+// evaluated as x * sigma(2 u) with sigma(z) = 1 / (1 + e^-z) (the identity
+// 0.5 (1 + tanh(u)) = sigma(2 u)): one hardware exp and one divide.  Large
+// |u| saturates correctly (e^-2u -> inf gives -0, -> 0 gives x).
+__device__ __forceinline__ float gelu_sigma(float x, float* u_prime) {
   const float c = 0.7978845608028654f;
-  float u = c * (x + 0.044715f * x * x * x);
-  return 0.5f * x * (1.0f + tanhf(u));
+  const float x2 = x * x;
+  const float u = c * (x + 0.044715f * x2 * x);
+  if (u_prime) *u_prime = c * (1.0f + 3.0f * 0.044715f * x2);
+  return 1.0f / (1.0f + __expf(-2.0f * u));
 }
+__device__ __forceinline__ float gelu(float x) { return x * gelu_sigma(x, nullptr); }
+// d/dx [x sigma(2u)] = sigma + 2 x sigma (1 - sigma) u'
 __device__ __forceinline__ float gelu_grad(float x) {
-  const float c = 0.7978845608028654f;
-  float x2 = x * x;
-  float u = c * (x + 0.044715f * x2 * x);
-  float t = tanhf(u);
-  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * x2);
+  float up;
+  const float sg = gelu_sigma(x, &up);
+  return sg + 2.0f * x * sg * (1.0f - sg) * up;
 }
 
 template <int EPI>
