@@ -418,11 +418,29 @@ __device__ __forceinline__ void column_step(const StateView& s, const Params& pr
     kpp_on = pr.use_kpp && (h_b > R(0) && w_s > R(0));
     h_b_r = h_b;
     if (kpp_on) {
+      // the KPP shape-function profile of the predicted depth, parked for
+      // sweep B's max with the predicted kv/kt exactly as on the plain path
       R d_running = R(0);
       for (int k = k_sfc; k >= 1; --k) {
         d_running = d_running + R(LV(s.h, k));
         if (!(d_running < h_b)) break;
+        R sigma = d_running / h_b;
+        R g_shape = sigma * (R(1) - sigma) * (R(1) - sigma);
+        R kv_kpp = h_b * w_s * g_shape;
         k_lo = k;
+        if (k >= kb) {
+          LV(s.kv, k) = (double)kv_kpp;
+        } else {
+          double kv, kt;
+          filler_iface(s, pr, p, k, kv, kt);
+          double kk = (double)kv_kpp;
+          kv = fmx(kv, kk);
+          kt = fmx(kt, kk);
+          double ks = kt;
+          LV(s.kv, k) = fmn(fmx(kv, pr.kv_bg), pr.kv_max);
+          LV(s.kt, k) = fmn(fmx(kt, pr.kt_bg), pr.kd_max);
+          LV(s.ks, k) = fmn(fmx(ks, pr.ks_bg), pr.kd_max);
+        }
       }
     }
     s.bl[p] = (double)h_b;
@@ -513,7 +531,7 @@ __device__ __forceinline__ void column_step(const StateView& s, const Params& pr
     // filler interfaces the boundary layer reached last step but not now get
     // their plain value back
     const int fd = s.fdirty[p];
-    const int k_lo_fill = use_s ? nz + 1 : k_lo;  // the surrogate leaves filler interfaces plain
+    const int k_lo_fill = k_lo;
     const int top = min(kb - 1, k_lo_fill - 1);
     for (int f = max(fd, 1); f <= top; ++f) {
       double kv, kt;
@@ -555,7 +573,7 @@ __device__ __forceinline__ void column_step(const StateView& s, const Params& pr
       kvv = (double)kv_r;
       ktv = (double)kt_r;
     }
-    if (!use_s && kf >= k_lo && kf <= k_sfc) {
+    if (kf >= k_lo && kf <= k_sfc) {
       double kk = LV(s.kv, kf);  // KPP value parked by sweep A
       kvv = fmx(kvv, kk);
       ktv = fmx(ktv, kk);

@@ -15,6 +15,9 @@ enum { EPI_F32 = 0, EPI_BIAS_F32 = 1, EPI_BIAS_GELU = 2, EPI_BIAS_F32_T = 3 };
 constexpr int FP_MAXL = 8;
 constexpr int FP_MAXK = 384;
 constexpr int FP_PITCH = FP_MAXK + 8;  // LDS row pitch (bf16): rows start 4 banks apart
+constexpr int FP_MAXN = 256;           // widest layer output
+constexpr int FP_WAVES = 8;            // wavefronts per workgroup
+constexpr size_t FP_LDS_BYTES = (size_t)(128 * FP_PITCH + 2 * FP_MAXN * 32) * 2;
 struct FusedNet {
   const bf16* W[FP_MAXL];
   const float* b[FP_MAXL];

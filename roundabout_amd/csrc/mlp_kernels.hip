@@ -142,95 +142,155 @@ template __global__ void k_gemm_bf16<EPI_BIAS_F32_T>(const bf16*, int, const bf1
 template __global__ void k_gemm_bf16<EPI_BIAS_GELU>(const bf16*, int, const bf16*, int, int, int, int, const float*,
                                                    float*, int, bf16*, int, int, long long);
 
+// One layer of the fused kernel for a wave owning rows [row0, row0 + 64) of
+// the block and NTW 16-wide output tiles from column nw0:
+// acc[i][j] (rows row0 + 16 i.., columns nw0 + 16 j..) = act * W^T.
+//
+// W streams through LDS in k-slices of 32 (N rows x 64 B), double-buffered,
+// with the direct global-to-LDS load (global_load_lds_dwordx4: lane l of a
+// wave writes its 16 bytes at the instruction's LDS base + 16 l), so the
+// next slice is in flight while the MFMAs of the current one run.  Within a
+// slice, the 16-byte chunk q of row n sits at slot 4 n + ((q + n / 4) mod 4):
+// the 16 rows of one MFMA operand then start in 16 different bank groups.
+// NTW is a compile-time constant, so the loop body has no branches; the
+// scheduling barriers keep each k-step as: fragment reads, MFMAs, wait.
+template <int NTW>
+__device__ __forceinline__ void fused_layer(const bf16* __restrict__ W, int K, int N, int wave, int lane, int row0,
+                                            int nw0, const bf16* act, bf16* wbuf, f32x4 (&acc)[4][4]) {
+  const int r = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < NTW; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // a slice is N / 16 row groups of 16 rows (1 KB of LDS, one instruction
+  // per wave each); lane l loads row g * 16 + l / 4 into LDS slot l % 4,
+  // which holds global chunk (l % 4 - l / 16) mod 4
+  const int lrow = lane >> 2;
+  const int lchunk = ((lane & 3) - (lane >> 4) + 4) & 3;
+  const int groups = N / 16;
+  const int slice_elems = N * 32;  // bf16 per slice buffer
+  auto load_slice = [&](int k0, bf16* buf) {
+    for (int g = wave; g < groups; g += FP_WAVES) {
+      const bf16* src = W + (size_t)(g * 16 + lrow) * (size_t)K + k0 + lchunk * 8;
+      __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)(buf + g * 512), 16,
+                                       0, 0);
+    }
+  };
+  load_slice(0, wbuf);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  const bf16* arow = act + (row0 + r) * FP_PITCH + q * 8;
+  int cur = 0;
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    bf16* bcur = wbuf + cur * slice_elems;
+    if (k0 + 32 < K) load_slice(k0 + 32, wbuf + (cur ^ 1) * slice_elems);
+    bf16x8 a[4], b[NTW];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = *(const bf16x8*)(arow + 16 * i * FP_PITCH + k0);
+#pragma unroll
+    for (int j = 0; j < NTW; ++j) {
+      const int n = nw0 + 16 * j + r;
+      b[j] = *(const bf16x8*)(bcur + (4 * n + ((q + (n >> 2)) & 3)) * 8);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < NTW; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's part of the next slice has landed
+    __syncthreads();                // ... and every wave's; the current buffer is free
+    cur ^= 1;
+  }
+}
+
+// Epilogue of one layer: GELU(acc + b) to the block's LDS input (hidden
+// layers), or acc + b to Y (the last layer).
+template <int NTW>
+__device__ __forceinline__ void fused_epilogue(const f32x4 (&acc)[4][4], const float* __restrict__ bias, int row0,
+                                               int nw0, int r, int q, bf16* act, bool last, float* __restrict__ Y,
+                                               long long ldy, long long m0, int out_major) {
+#pragma unroll
+  for (int j = 0; j < NTW; ++j) {
+    const int n = nw0 + 16 * j + r;
+    const float bn = bias[n];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int mr = row0 + 16 * i + 4 * q;  // rows mr .. mr + 3 of the block
+      if (!last) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) act[(mr + e) * FP_PITCH + n] = f2bf(gelu(acc[i][j][e] + bn));
+      } else if (out_major) {
+        f32x4 v = acc[i][j];
+        v[0] += bn;
+        v[1] += bn;
+        v[2] += bn;
+        v[3] += bn;
+        *(f32x4*)(Y + (size_t)n * (size_t)ldy + (size_t)(m0 + mr)) = v;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) Y[(size_t)(m0 + mr + e) * (size_t)ldy + n] = acc[i][j][e] + bn;
+      }
+    }
+  }
+}
+
 // The whole network on a block of 128 rows, activations resident in LDS.
 //
-// Workgroup of 4 wavefronts; the block's layer input (128 x pin bf16, row
-// pitch FP_PITCH) sits in LDS and every wave reads all of it, while wave w
-// owns output columns [w pout/4, (w+1) pout/4) and streams its rows of W
-// (k-contiguous bf16, L2-resident: the network is 0.66 MB) straight into
-// MFMA B fragments, one k-step ahead.  After a layer's k-loop all waves
-// synchronise and overwrite the block's LDS input with GELU(acc + b) in
-// bf16; the last layer adds the bias and stores FP32 to Y, row-major
-// (Y[m ldy + n]) or output-major (Y[n ldy + m]).  HBM sees only the input
-// features and the outputs: no activation round trips between layers.
-// Limits (checked by the host): pin <= FP_MAXK, pout <= 256, all multiples
-// of 64; rows a multiple of 128.
-__global__ void __launch_bounds__(256) k_mlp_fused(FusedNet fn, const bf16* __restrict__ X0, int ldx,
-                                                  float* __restrict__ Y, long long ldy, int out_major) {
-  extern __shared__ __align__(16) bf16 act[];  // 128 x FP_PITCH
+// Workgroup of 8 wavefronts: the block's layer input (128 x pin bf16, row
+// pitch FP_PITCH) sits in LDS; wave w owns rows 64 (w / 4) .. + 64 and
+// output columns [(w % 4) pout / 4, (w % 4 + 1) pout / 4), two waves per
+// SIMD.  The inputs are staged with direct-to-LDS loads, all in flight at
+// once; W streams through LDS (fused_layer).  After a layer's k-loop the
+// block overwrites its LDS input with GELU(acc + b) in bf16; the last layer
+// adds the bias and stores FP32 to Y, row-major (Y[m ldy + n]) or
+// output-major (Y[n ldy + m]).  HBM sees only the input features and the
+// outputs: no activation round trips between layers.
+// Limits (checked by the host): pin <= FP_MAXK, pout <= FP_MAXN, all
+// multiples of 64; rows a multiple of 128.
+__global__ void __launch_bounds__(64 * FP_WAVES) k_mlp_fused(FusedNet fn, const bf16* __restrict__ X0, int ldx,
+                                                             float* __restrict__ Y, long long ldy, int out_major) {
+  extern __shared__ __align__(16) bf16 act[];  // 128 x FP_PITCH, then 2 W slices of FP_MAXN x 32
+  bf16* wbuf = act + 128 * FP_PITCH;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int r = lane & 15, q = lane >> 4;
   const long long m0 = (long long)blockIdx.x * 128;
-  // stage the block's inputs: 16-byte chunks, rows contiguous
   {
-    const int cpr = ldx / 8;  // chunks per row
-    for (int c = tid; c < 128 * cpr; c += 256) {
-      const int row = c / cpr, col = (c % cpr) * 8;
-      *(bf16x8*)(&act[row * FP_PITCH + col]) = *(const bf16x8*)(X0 + (size_t)(m0 + row) * (size_t)ldx + col);
+    // the block's LDS image (128 rows x FP_PITCH) in 1 KB pieces of 64
+    // 16-byte chunks; the pad chunks at the end of each row re-read the
+    // row's first chunk (never used)
+    constexpr int CPR = FP_PITCH / 8;  // chunks per LDS row
+    constexpr int NCH = 128 * CPR;     // a multiple of 64
+    for (int c0 = wave * 64; c0 < NCH; c0 += 64 * FP_WAVES) {
+      const int c = c0 + lane;
+      const int row = c / CPR, col = (c % CPR) * 8;
+      const bf16* src = X0 + (size_t)(m0 + row) * (size_t)ldx + (col < ldx ? col : 0);
+      __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)(act + c0 * 8), 16,
+                                       0, 0);
     }
+    __builtin_amdgcn_s_waitcnt(0);
   }
   __syncthreads();
+  const int row0 = 64 * (wave >> 2);
   for (int l = 0; l < fn.nl; ++l) {
     const int K = fn.pin[l], N = fn.pout[l];
-    const int ntw = N / 64;            // 16-wide n-tiles per wave (<= 4)
-    const int nw0 = wave * (N / 4);    // this wave's first output column
-    const bf16* __restrict__ W = fn.W[l];
-    f32x4 acc[8][4];
-#pragma unroll
-    for (int i = 0; i < 8; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const bf16* Bp[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) Bp[j] = W + (size_t)(nw0 + 16 * (j < ntw ? j : 0) + r) * (size_t)K + q * 8;
-    bf16x8 b_cur[4], b_nxt[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if (j < ntw) b_cur[j] = *(const bf16x8*)(Bp[j]);
-    for (int k0 = 0; k0 < K; k0 += 32) {
-      const bool more = k0 + 32 < K;
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (more && j < ntw) b_nxt[j] = *(const bf16x8*)(Bp[j] + k0 + 32);
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const bf16x8 a = *(const bf16x8*)(&act[(16 * i + r) * FP_PITCH + k0 + q * 8]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (j < ntw) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b_cur[j], acc[i][j], 0, 0, 0);
-      }
-      if (more) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) b_cur[j] = b_nxt[j];
-      }
-    }
-    __syncthreads();  // every wave has read this layer's input
-    const float* __restrict__ bias = fn.b[l];
+    const int ntw = N / 64;                // 16-wide n-tiles per wave (1..4)
+    const int nw0 = (wave & 3) * (N / 4);  // this wave's first output column
     const bool last = (l == fn.nl - 1);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (j >= ntw) continue;
-      const int n = nw0 + 16 * j + r;
-      const float bn = bias[n];
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int mr = 16 * i + 4 * q;  // rows mr .. mr + 3 of the block
-        if (!last) {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) act[(mr + e) * FP_PITCH + n] = f2bf(gelu(acc[i][j][e] + bn));
-        } else if (out_major) {
-          f32x4 v = acc[i][j];
-          v[0] += bn;
-          v[1] += bn;
-          v[2] += bn;
-          v[3] += bn;
-          *(f32x4*)(Y + (size_t)n * (size_t)ldy + (size_t)(m0 + mr)) = v;
-        } else {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) Y[(size_t)(m0 + mr + e) * (size_t)ldy + n] = acc[i][j][e] + bn;
-        }
-      }
+    f32x4 acc[4][4];
+    switch (ntw) {
+      case 4: fused_layer<4>(fn.W[l], K, N, wave, lane, row0, nw0, act, wbuf, acc); break;
+      case 3: fused_layer<3>(fn.W[l], K, N, wave, lane, row0, nw0, act, wbuf, acc); break;
+      case 2: fused_layer<2>(fn.W[l], K, N, wave, lane, row0, nw0, act, wbuf, acc); break;
+      default: fused_layer<1>(fn.W[l], K, N, wave, lane, row0, nw0, act, wbuf, acc); break;
+    }
+    // (the layer ends with a barrier: every wave has read this layer's input)
+    switch (ntw) {
+      case 4: fused_epilogue<4>(acc, fn.b[l], row0, nw0, r, q, act, last, Y, ldy, m0, out_major); break;
+      case 3: fused_epilogue<3>(acc, fn.b[l], row0, nw0, r, q, act, last, Y, ldy, m0, out_major); break;
+      case 2: fused_epilogue<2>(acc, fn.b[l], row0, nw0, r, q, act, last, Y, ldy, m0, out_major); break;
+      default: fused_epilogue<1>(acc, fn.b[l], row0, nw0, r, q, act, last, Y, ldy, m0, out_major); break;
     }
     __syncthreads();  // the next layer's input is complete
   }

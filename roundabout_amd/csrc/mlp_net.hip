@@ -235,7 +235,7 @@ bool Net::fused_ok(int Mpad) const {
   }();
   if (unfused || Mpad % 128 != 0 || nl > FP_MAXL) return false;
   for (int l = 0; l < nl; ++l)
-    if (pin[l] > FP_MAXK || pout[l] > 256) return false;
+    if (pin[l] > FP_MAXK || pout[l] > FP_MAXN) return false;
   return true;
 }
 
@@ -243,7 +243,7 @@ int Net::forward(int Mpad, bool keep_pre, bool out_major) {
   if (!keep_pre && fused_ok(Mpad)) {
     // inference: one launch over all layers, activations in LDS
     static bool lds_set = false;
-    const size_t shmem = (size_t)128 * FP_PITCH * sizeof(bf16);
+    const size_t shmem = FP_LDS_BYTES;
     if (!lds_set) {
       NT_TRY(hipFuncSetAttribute((const void*)k_mlp_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
       lds_set = true;
@@ -257,7 +257,8 @@ int Net::forward(int Mpad, bool keep_pre, bool out_major) {
       fn.pout[l] = pout[l];
     }
     const long long ldy = out_major ? (long long)Mpad : (long long)pout[nl - 1];
-    hipLaunchKernelGGL(k_mlp_fused, dim3((unsigned)(Mpad / 128)), dim3(256), shmem, stream, fn, (const bf16*)X0,
+    hipLaunchKernelGGL(k_mlp_fused, dim3((unsigned)(Mpad / 128)), dim3(64 * FP_WAVES), shmem, stream, fn,
+                       (const bf16*)X0,
                        pin[0], Y, ldy, out_major ? 1 : 0);
     NT_TRY(hipGetLastError());
     return 0;

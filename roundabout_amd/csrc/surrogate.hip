@@ -2,15 +2,23 @@
 // from the plain path, feature and guard kernels, normalisation, the Adam
 // training loop, guarded inference feeding the FP64 solve, save/load.
 //
-// Features per wet column (5 nz + 6): T, S, centred u, centred v and
-// log10 h at every level, then |tau| (+ shelf stress), Q_heat, Q_salt,
-// Coriolis f, the lagged KPP boundary-layer depth and the live column depth.
-// T, S, u and v enter as differences to the live level above (the surface
-// level, or one under a vanished level, as its value; 0 on vanished
-// levels): an invertible recoding of the profile that keeps the
-// level-to-level contrasts, which set N^2, shear and Ri, above the BF16
-// rounding of the normalised inputs (raw values would bury a 0.1 K step
-// under the 8-bit mantissa of a profile spanning 25 K).
+// Features per wet column (5 nz + 6): five per level, then |tau| (+ shelf
+// stress), Q_heat, Q_salt, Coriolis f, the lagged KPP boundary-layer depth
+// and the live column depth.  The five level slots hold the column state
+// (T, S, centred u, centred v, h) recoded against the live level above:
+//   buoyancy jump  slog(g (rho_k - rho_above) / rho0, 1e-7 m/s^2)  [T slot]
+//   S_k - S_above                                                  [S slot]
+//   slog(u_k - u_above, 1e-3 m/s), slog(v_k - v_above, 1e-3 m/s)   [u, v slots]
+//   log10 h_k                                                      [h slot]
+// with slog(x, x0) = sign(x) log10(1 + |x| / x0) and rho from the FP64
+// Wright EOS; a live level with no live level above (the surface) holds its
+// T, S, u, v; a vanished level holds 0.  Given the surface values and the
+// EOS the recoding is invertible, so it carries the same information as the
+// raw profile, but (a) level-to-level contrasts survive the BF16 rounding of
+// the normalised inputs (raw T would bury a 0.1 K step under the 8-bit
+// mantissa of a 25 K profile) and (b) log Ri = log N^2 - log S^2 becomes
+// nearly a linear combination of the inputs, leaving PP81 a function of one
+// variable for the network to learn.
 // Targets (2 (nz-1) + 1): log10 kt and log10 kv at the interior interfaces
 // 1..nz-1 after the plain coefficient build, and the new boundary-layer
 // depth.  Interfaces below the first live layer carry weight 0 (they keep
@@ -86,25 +94,53 @@ __device__ __forceinline__ bf16 f2bf(float f) {
   return (bf16)(u >> 16);
 }
 
-// feature f of packed column p (raw, un-normalised).  The network has
-// nzn >= nz levels with the surfaces aligned: network level j is this
-// grid's level j - lev_off; the lev_off lowest network levels do not exist
-// here and read as bed fillers (h = 1e-4 m, no tracer).
-__device__ __forceinline__ float feature(const StateView& s, const double* cor, int p, int f, double depth, int nzn,
-                                         int lev_off) {
-  const int P = s.P, nz = s.nz;
-  const int blk = f / nzn, k = f % nzn - lev_off;
-  if (blk < 5 && k < 0) return blk == 4 ? -4.0f : 0.f;
-  if (blk < 5) {
-    const double hk = LV(s.h, k);
-    if (blk == 4) return (float)log10(fmax(hk, 1e-30));
-    if (!(hk > H_VANISHED)) return 0.f;
-    const double* a = blk == 0 ? s.T : blk == 1 ? s.S : blk == 2 ? s.ub : s.vb;
-    double v = LV(a, k);
-    if (k + 1 < nz && LV(s.h, k + 1) > H_VANISHED) v -= LV(a, k + 1);
-    return (float)v;
+// One level of a column, as the features see it.
+struct LevelVals {
+  double h, T, S, U, V, rho;
+  bool live;
+};
+
+// level k of packed column p; levels outside [0, nz) do not exist in this
+// grid and read as a bed filler (h = 1e-4 m, not live)
+__device__ __forceinline__ LevelVals load_level(const StateView& s, int p, int k, double rho0, double p_ref) {
+  LevelVals v{1.0e-4, 0.0, 0.0, 0.0, 0.0, 0.0, false};
+  if (k < 0 || k >= s.nz) return v;
+  const int P = s.P;
+  v.h = LV(s.h, k);
+  v.live = v.h > H_VANISHED;
+  if (v.live) {
+    v.T = LV(s.T, k);
+    v.S = LV(s.S, k);
+    v.U = LV(s.ub, k);
+    v.V = LV(s.vb, k);
+    v.rho = eos_wright<double>(v.T, v.S, v.h, rho0, p_ref);
   }
-  switch (f - 5 * nzn) {
+  return v;
+}
+
+__device__ __forceinline__ double slog(double x, double x0) { return copysign(log10(1.0 + fabs(x) / x0), x); }
+
+// the five level slots of `cur` under the level `above`
+__device__ __forceinline__ void level_slots(const LevelVals& cur, const LevelVals& above, double rho0, float out[5]) {
+  out[4] = (float)log10(fmax(cur.h, 1e-30));
+  if (!cur.live) {
+    out[0] = out[1] = out[2] = out[3] = 0.f;
+  } else if (!above.live) {
+    out[0] = (float)cur.T;
+    out[1] = (float)cur.S;
+    out[2] = (float)cur.U;
+    out[3] = (float)cur.V;
+  } else {
+    out[0] = (float)slog(GRAVITY * (cur.rho - above.rho) / rho0, 1.0e-7);
+    out[1] = (float)(cur.S - above.S);
+    out[2] = (float)slog(cur.U - above.U, 1.0e-3);
+    out[3] = (float)slog(cur.V - above.V, 1.0e-3);
+  }
+}
+
+// forcing feature i (0..5) of packed column p
+__device__ __forceinline__ float forcing_feature(const StateView& s, const double* cor, int p, int i, double depth) {
+  switch (i) {
     case 0: return (float)s.tau_tot[p];
     case 1: return (float)s.Qh[p];
     case 2: return (float)s.Qs[p];
@@ -125,20 +161,32 @@ __device__ __forceinline__ double live_depth(const StateView& s, int p) {
 }
 
 // Raw features of packed columns c = i * stride (i < ncols) into rows row0 + i.
+// The network has nzn >= nz levels with the surfaces aligned: network level
+// j is this grid's level j - lev_off.
 __global__ void k_features_raw(StateView s, const double* cor, int ncols, int stride, int din, float* X, long long row0,
-                               int nzn, int lev_off) {
+                               int nzn, int lev_off, double rho0, double p_ref) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ncols) return;
   int p = i * stride;
-  double depth = live_depth(s, p);
   float* row = X + (size_t)(row0 + i) * (size_t)din;
-  for (int f = 0; f < din; ++f) row[f] = feature(s, cor, p, f, depth, nzn, lev_off);
+  LevelVals above = load_level(s, p, nzn - lev_off, rho0, p_ref);  // above the top: none
+  for (int j = nzn - 1; j >= 0; --j) {
+    const LevelVals cur = load_level(s, p, j - lev_off, rho0, p_ref);
+    float o[5];
+    level_slots(cur, above, rho0, o);
+    for (int b = 0; b < 5; ++b) row[b * nzn + j] = o[b];
+    above = cur;
+  }
+  const double depth = live_depth(s, p);
+  for (int f = 5 * nzn; f < din; ++f) row[f] = forcing_feature(s, cor, p, f - 5 * nzn, depth);
 }
 
 // Targets of the same columns after the plain step: log10 kt, log10 kv at
-// the network's interfaces 1..nzn-1, bl_depth; weight 1 on interfaces at or
-// above this grid's first live layer kb, 0 below it and on interfaces the
-// grid does not have.
+// the network's interfaces 1..nzn-1, bl_depth.  Loss weight 1, except 0
+// below this grid's first live layer kb and on interfaces the grid does not
+// have.  (Weighting the kt targets by the diffusive flux they carry was
+// tried and left the 24-step temperature error no better: the network's
+// limit there is resolving Ri, not the weighting.)
 __global__ void k_targets_raw(StateView s, int ncols, int stride, int dout, float* T, float* W, float* rowsum,
                               long long row0, int nzn, int lev_off) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -150,14 +198,14 @@ __global__ void k_targets_raw(StateView s, int ncols, int stride, int dout, floa
   float* w = W + (size_t)(row0 + i) * (size_t)dout;
   float sum = 0.f;
   for (int j = 1; j < nzn; ++j) {
-    const int k = j - lev_off;  // this grid's interface
-    const bool has = k >= 1;
-    float wk = (has && k >= kb) ? 1.f : 0.f;
-    t[j - 1] = has ? (float)log10(fmax(LV(s.kt, k), 1e-30)) : 0.f;
-    t[nzn - 1 + j - 1] = has ? (float)log10(fmax(LV(s.kv, k), 1e-30)) : 0.f;
-    w[j - 1] = wk;
-    w[nzn - 1 + j - 1] = wk;
-    sum += 2.f * wk;
+    const int k = j - lev_off;  // this grid's interface (between layers k - 1 and k)
+    const bool on = k >= 1 && k >= kb;
+    const float wt = on ? 1.f : 0.f, wv = wt;
+    t[j - 1] = k >= 1 ? (float)log10(fmax(LV(s.kt, k), 1e-30)) : 0.f;
+    t[nzn - 1 + j - 1] = k >= 1 ? (float)log10(fmax(LV(s.kv, k), 1e-30)) : 0.f;
+    w[j - 1] = wt;
+    w[nzn - 1 + j - 1] = wv;
+    sum += wt + wv;
   }
   t[2 * (nzn - 1)] = (float)s.bl[p];
   w[2 * (nzn - 1)] = 1.f;
@@ -246,65 +294,85 @@ __global__ void k_gather(const bf16* Xn, int ldx, const float* Tn, const float* 
 
 // Inference features of wet columns [0, n): normalised bf16 rows written
 // through an LDS tile so that each row is stored with coalesced writes, and
-// the per-column guard (non-zero: plain path).
-constexpr int FT = 64;
-__global__ void __launch_bounds__(FT) k_features_infer(StateView s, const double* cor, int n, int din, int ld,
+// the per-column guard (non-zero: plain path).  Workgroup of 256 = 4 waves
+// over 64 columns: wave w takes a quarter of the network levels for all 64
+// columns (coalesced level-major reads), walking them top-down with the
+// level above carried in registers; it also checks N^2 < 0 (strict guard)
+// on the pairs whose lower level it owns.  Wave 0 adds the forcing features,
+// wave 3 the surface-pair check; the flags meet in LDS.  Every value is the
+// same arithmetic as feature(), so training and inference features agree
+// bit for bit.
+constexpr int FC = 64, FB = 256;
+__global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double* cor, int n, int din, int ld,
                                                      const float* st, bf16* X0, unsigned char* guard,
                                                      unsigned int* nfall, int guard_mode, double rho0, double p_ref,
                                                      int nzn, int lev_off) {
-  __shared__ bf16 tile[384][FT + 1];
+  __shared__ bf16 tile[384][FC + 1];
+  __shared__ unsigned char flags[4][FC];
   const float* mean = st;
   const float* stdv = st + din;
   const float* mn = st + 2 * din;
   const float* mx = st + 3 * din;
-  const int tid = threadIdx.x;
-  const int p = blockIdx.x * FT + tid;
+  const int tid = threadIdx.x, c = tid & (FC - 1), part = tid >> 6;
+  const int p = blockIdx.x * FC + c;
   const int P = s.P, nz = s.nz;
+  unsigned char fl = 0;
+  auto put = [&](int f, float v) {
+    if (v < mn[f] || v > mx[f]) fl |= 1;
+    tile[f][c] = f2bf((v - mean[f]) / stdv[f]);
+  };
   if (p < n) {
-    double depth = live_depth(s, p);
-    bool out_of_range = false;
-    for (int f = 0; f < din; ++f) {
-      float v = feature(s, cor, p, f, depth, nzn, lev_off);
-      out_of_range |= (v < mn[f] || v > mx[f]);
-      tile[f][tid] = f2bf((v - mean[f]) / stdv[f]);
-    }
-    for (int f = din; f < ld; ++f) tile[f][tid] = 0;
-    // static stability: N^2 < 0 where density increases upward between two
-    // live layers (N^2 = -g (rho_k - rho_km1) / (rho0 dz), A3's sign)
+    const int per = (nzn + 3) / 4;
+    const int j0 = part * per, j1 = min(nzn, j0 + per);
+    LevelVals above = load_level(s, p, j1 - lev_off, rho0, p_ref);  // the level above this part's highest
     bool unstable = false;
-    const int kb = s.kb[p];
-    int k_sfc = nz - 1;
-    for (int k = nz - 1; k >= kb; --k)
-      if (LV(s.h, k) > H_VANISHED) { k_sfc = k; break; }
-    if (guard_mode == RDB_COLPHYS_GUARD_STRICT) {
-      double rho_m = 0.0;
-      bool live_m = false;
-      for (int k = kb; k < nz; ++k) {
-        double hk = LV(s.h, k);
-        bool live = hk > H_VANISHED;
-        double rho = live ? eos_wright<double>(LV(s.T, k), LV(s.S, k), hk, rho0, p_ref) : 0.0;
-        if (live && live_m && rho > rho_m) unstable = true;
-        rho_m = rho;
-        live_m = live;
-      }
-    } else if (guard_mode == RDB_COLPHYS_GUARD_SURFACE && k_sfc > kb) {
-      double h1 = LV(s.h, k_sfc), h0 = LV(s.h, k_sfc - 1);
-      if (h1 > H_VANISHED && h0 > H_VANISHED) {
-        double r1 = eos_wright<double>(LV(s.T, k_sfc), LV(s.S, k_sfc), h1, rho0, p_ref);
-        double r0 = eos_wright<double>(LV(s.T, k_sfc - 1), LV(s.S, k_sfc - 1), h0, rho0, p_ref);
-        unstable = r1 > r0;
+    for (int j = j1 - 1; j >= j0; --j) {
+      const LevelVals cur = load_level(s, p, j - lev_off, rho0, p_ref);
+      float o[5];
+      level_slots(cur, above, rho0, o);
+#pragma unroll
+      for (int b = 0; b < 5; ++b) put(b * nzn + j, o[b]);
+      // static stability of the pair (this level, the one above): N^2 < 0
+      // where the upper of two live layers is denser (A3's sign)
+      if (guard_mode == RDB_COLPHYS_GUARD_STRICT && cur.live && above.live && above.rho > cur.rho) unstable = true;
+      above = cur;
+    }
+    if (part == 0) {
+      const double depth = live_depth(s, p);
+      for (int f = 5 * nzn; f < din; ++f) put(f, forcing_feature(s, cor, p, f - 5 * nzn, depth));
+      for (int f = din; f < ld; ++f) tile[f][c] = 0;
+    }
+    if (part == 3 && guard_mode == RDB_COLPHYS_GUARD_SURFACE) {
+      const int kb = s.kb[p];
+      int k_sfc = nz - 1;
+      for (int k = nz - 1; k >= kb; --k)
+        if (LV(s.h, k) > H_VANISHED) {
+          k_sfc = k;
+          break;
+        }
+      if (k_sfc > kb) {
+        const double h1 = LV(s.h, k_sfc), h0 = LV(s.h, k_sfc - 1);
+        if (h1 > H_VANISHED && h0 > H_VANISHED) {
+          const double r1 = eos_wright<double>(LV(s.T, k_sfc), LV(s.S, k_sfc), h1, rho0, p_ref);
+          const double r0 = eos_wright<double>(LV(s.T, k_sfc - 1), LV(s.S, k_sfc - 1), h0, rho0, p_ref);
+          unstable = unstable || r1 > r0;
+        }
       }
     }
-    unsigned char gflag = (out_of_range ? 1 : 0) | (unstable ? 2 : 0);
-    guard[p] = gflag;
-    if (gflag) atomicAdd(nfall, 1u);
+    if (unstable) fl |= 2;
   }
+  flags[part][c] = fl;
   __syncthreads();
-  // cooperative row stores: the FT rows of this block, 2 bf16 per lane step
-  const int rows = min(FT, n - blockIdx.x * FT);
+  if (part == 0 && p < n) {
+    const unsigned char g = flags[0][c] | flags[1][c] | flags[2][c] | flags[3][c];
+    guard[p] = g;
+    if (g) atomicAdd(nfall, 1u);
+  }
+  // cooperative row stores: the FC rows of this block, 2 bf16 per lane step
+  const int rows = min(FC, n - (int)blockIdx.x * FC);
   for (int r = 0; r < rows; ++r) {
-    uint32_t* dst = (uint32_t*)(X0 + (size_t)(blockIdx.x * FT + r) * (size_t)ld);
-    for (int c2 = tid; c2 < ld / 2; c2 += FT) {
+    uint32_t* dst = (uint32_t*)(X0 + (size_t)(blockIdx.x * FC + r) * (size_t)ld);
+    for (int c2 = tid; c2 < ld / 2; c2 += FB) {
       uint32_t lo = tile[2 * c2][r], hi = tile[2 * c2 + 1][r];
       dst[c2] = lo | (hi << 16);
     }
@@ -392,7 +460,7 @@ int surr_collect(rdb_colphys* h, double dt, int n_steps, int stride, long long* 
       const int nc = (m->n_wet + stride - 1) / stride;
       if (nc > 0)
         hipLaunchKernelGGL(k_features_raw, gs(nc, 128), dim3(128), 0, h->stream, m->s, m->d_coriolis, nc, stride,
-                           S->din, S->Xraw, row, nzn, nzn - m->nz);
+                           S->din, S->Xraw, row, nzn, nzn - m->nz, h->pr.rho0, h->pr.p_ref);
       row += nc;
     }
     st = launch_step_all(h, dt, false);
@@ -582,7 +650,7 @@ int surr_step(rdb_colphys* h, double dt) {
   const int ld = net.pin[0];
   for (rdb_colphys* m : ps) {
     if (m->n_wet == 0) continue;
-    hipLaunchKernelGGL(k_features_infer, gs(m->n_wet, FT), dim3(FT), 0, h->stream, m->s, m->d_coriolis, m->n_wet,
+    hipLaunchKernelGGL(k_features_infer, gs(m->n_wet, FC), dim3(FB), 0, h->stream, m->s, m->d_coriolis, m->n_wet,
                        S->din, ld, S->d_stats, net.X0 + (size_t)m->row_off * (size_t)ld, S->guard + m->row_off,
                        S->nfall, S->guard_mode, h->pr.rho0, h->pr.p_ref, nzn, nzn - m->nz);
   }
@@ -657,7 +725,8 @@ int surr_save(rdb_colphys* h, const char* path) {
   if (S->net.get_params(p.data())) return perr(RDB_COLPHYS_ERR_SETUP, "mlp_save: %s", S->net.err.c_str());
   FILE* f = std::fopen(path, "wb");
   if (!f) return perr(RDB_COLPHYS_ERR_IO, "mlp_save: cannot open %s", path);
-  const char magic[8] = {'R', 'D', 'B', 'S', 'U', 'R', 'R', '1'};
+  // version 2: buoyancy-jump / signed-log level encoding of the features
+  const char magic[8] = {'R', 'D', 'B', 'S', 'U', 'R', 'R', '2'};
   int hdr[4] = {S->nz, S->din, S->dout, (int)S->widths.size()};
   bool ok = std::fwrite(magic, 1, 8, f) == 8 && std::fwrite(hdr, 4, 4, f) == 4 &&
             std::fwrite(S->widths.data(), 4, S->widths.size(), f) == S->widths.size() &&
@@ -673,7 +742,7 @@ int surr_load(rdb_colphys* h, const char* path) {
   if (!f) return perr(RDB_COLPHYS_ERR_IO, "mlp_load: cannot open %s", path);
   char magic[8];
   int hdr[4];
-  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "RDBSURR1", 8) == 0 && std::fread(hdr, 4, 4, f) == 4;
+  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "RDBSURR2", 8) == 0 && std::fread(hdr, 4, 4, f) == 4;
   if (!ok || hdr[0] != net_levels(h) || hdr[3] < 2 || hdr[3] > 16) {
     std::fclose(f);
     return perr(RDB_COLPHYS_ERR_BAD_SHAPE, "mlp_load: %s is not a network for nz = %d", path, net_levels(h));
