@@ -17,7 +17,9 @@ constexpr int FP_MAXK = 384;
 constexpr int FP_PITCH = FP_MAXK + 8;  // LDS row pitch (bf16): rows start 4 banks apart
 constexpr int FP_MAXN = 256;           // widest layer output
 constexpr int FP_WAVES = 8;            // wavefronts per workgroup
-constexpr size_t FP_LDS_BYTES = (size_t)(128 * FP_PITCH + 2 * FP_MAXN * 32) * 2;
+// dynamic LDS of k_mlp_fused (the activation tile); three weight slices of
+// FP_MAXN x 32 bf16 (48 KB) are static LDS on top
+constexpr size_t FP_LDS_BYTES = (size_t)128 * FP_PITCH * 2;
 struct FusedNet {
   const bf16* W[FP_MAXL];
   const float* b[FP_MAXL];

@@ -142,21 +142,33 @@ template __global__ void k_gemm_bf16<EPI_BIAS_F32_T>(const bf16*, int, const bf1
 template __global__ void k_gemm_bf16<EPI_BIAS_GELU>(const bf16*, int, const bf16*, int, int, int, int, const float*,
                                                    float*, int, bf16*, int, int, long long);
 
+// s_waitcnt vmcnt(n) (n = 0, 1, 2, 3) without waiting on the LDS or export
+// counters; the gfx9 immediate holds vmcnt in bits 3:0, expcnt in 6:4 and
+// lgkmcnt in 11:8.
+__device__ __forceinline__ void wait_vm_except(int n) {
+  switch (n) {
+    case 0: __builtin_amdgcn_s_waitcnt(0xF70); break;
+    case 1: __builtin_amdgcn_s_waitcnt(0xF71); break;
+    case 2: __builtin_amdgcn_s_waitcnt(0xF72); break;
+    default: __builtin_amdgcn_s_waitcnt(0xF73); break;
+  }
+}
+
 // One layer of the fused kernel for a wave owning rows [row0, row0 + 64) of
 // the block and NTW 16-wide output tiles from column nw0:
 // acc[i][j] (rows row0 + 16 i.., columns nw0 + 16 j..) = act * W^T.
 //
-// W streams through LDS in k-slices of 32 (N rows x 64 B), double-buffered,
-// with the direct global-to-LDS load (global_load_lds_dwordx4: lane l of a
-// wave writes its 16 bytes at the instruction's LDS base + 16 l), so the
-// next slice is in flight while the MFMAs of the current one run.  Within a
+// W streams through LDS in k-slices of 32 (N rows x 64 B) in 3 buffers, with the direct global-to-LDS load (global_load_lds_dwordx4:
+// lane l of a wave writes its 16 bytes at the instruction's LDS base + 16 l),
+// two slices ahead of the MFMAs that use them.  Within a
 // slice, the 16-byte chunk q of row n sits at slot 4 n + ((q + n / 4) mod 4):
 // the 16 rows of one MFMA operand then start in 16 different bank groups.
 // NTW is a compile-time constant, so the loop body has no branches; the
 // scheduling barriers keep each k-step as: fragment reads, MFMAs, wait.
 template <int NTW>
 __device__ __forceinline__ void fused_layer(const bf16* __restrict__ W, int K, int N, int wave, int lane, int row0,
-                                            int nw0, const bf16* act, bf16* wbuf, f32x4 (&acc)[4][4]) {
+                                            int nw0, const bf16* act, bf16* w0, bf16* w1, bf16* w2,
+                                            f32x4 (&acc)[4][4]) {
   const int r = lane & 15, q = lane >> 4;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -168,7 +180,6 @@ __device__ __forceinline__ void fused_layer(const bf16* __restrict__ W, int K, i
   const int lrow = lane >> 2;
   const int lchunk = ((lane & 3) - (lane >> 4) + 4) & 3;
   const int groups = N / 16;
-  const int slice_elems = N * 32;  // bf16 per slice buffer
   auto load_slice = [&](int k0, bf16* buf) {
     for (int g = wave; g < groups; g += FP_WAVES) {
       const bf16* src = W + (size_t)(g * 16 + lrow) * (size_t)K + k0 + lchunk * 8;
@@ -176,21 +187,22 @@ __device__ __forceinline__ void fused_layer(const bf16* __restrict__ W, int K, i
                                        0, 0);
     }
   };
-  load_slice(0, wbuf);
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
+  // this wave's loads per slice (the vmcnt a slice still in flight holds)
+  const int per_wave = wave < groups ? (groups - wave + FP_WAVES - 1) / FP_WAVES : 0;
+  const int nsteps = K / 32;
   const bf16* arow = act + (row0 + r) * FP_PITCH + q * 8;
-  int cur = 0;
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    bf16* bcur = wbuf + cur * slice_elems;
-    if (k0 + 32 < K) load_slice(k0 + 32, wbuf + (cur ^ 1) * slice_elems);
+  // k-step t: slice t from `cur`; slice t + 2 starts into `ahead_buf`
+  auto step = [&](int t, const bf16* cur, bf16* ahead_buf) {
+    const int k0 = 32 * t;
+    const bool ahead = t + 2 < nsteps;
+    if (ahead) load_slice(k0 + 64, ahead_buf);
     bf16x8 a[4], b[NTW];
 #pragma unroll
     for (int i = 0; i < 4; ++i) a[i] = *(const bf16x8*)(arow + 16 * i * FP_PITCH + k0);
 #pragma unroll
     for (int j = 0; j < NTW; ++j) {
       const int n = nw0 + 16 * j + r;
-      b[j] = *(const bf16x8*)(bcur + (4 * n + ((q + (n >> 2)) & 3)) * 8);
+      b[j] = *(const bf16x8*)(cur + (4 * n + ((q + (n >> 2)) & 3)) * 8);
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -198,9 +210,25 @@ __device__ __forceinline__ void fused_layer(const bf16* __restrict__ W, int K, i
 #pragma unroll
       for (int j = 0; j < NTW; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     __builtin_amdgcn_sched_barrier(0);
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's part of the next slice has landed
-    __syncthreads();                // ... and every wave's; the current buffer is free
-    cur ^= 1;
+    // slice t + 1 has landed (slice t + 2 may still be in flight); the
+    // barrier also frees `cur` for slice t + 3.  A bare s_barrier: the loop
+    // writes no LDS itself, the reads of `cur` are complete (their values
+    // fed the MFMAs) and the direct loads are covered by the wait above, so
+    // the full fence of __syncthreads -- which would drain slice t + 2 as
+    // well -- is not needed.
+    wait_vm_except(ahead ? per_wave : 0);
+    __builtin_amdgcn_s_barrier();
+  };
+  load_slice(0, w0);
+  if (nsteps > 1) load_slice(32, w1);
+  wait_vm_except(nsteps > 1 ? per_wave : 0);  // slice 0 has landed
+  __syncthreads();
+  // unrolled by the buffer rotation, so that every LDS access names its
+  // buffer and the fragment reads need not wait for the loads in flight
+  for (int t = 0; t < nsteps; t += 3) {
+    step(t, w0, w2);
+    if (t + 1 < nsteps) step(t + 1, w1, w0);
+    if (t + 2 < nsteps) step(t + 2, w2, w1);
   }
 }
 
@@ -250,10 +278,12 @@ __device__ __forceinline__ void fused_epilogue(const f32x4 (&acc)[4][4], const f
 // multiples of 64; rows a multiple of 128.
 __global__ void __launch_bounds__(64 * FP_WAVES) k_mlp_fused(FusedNet fn, const bf16* __restrict__ X0, int ldx,
                                                              float* __restrict__ Y, long long ldy, int out_major) {
-  extern __shared__ __align__(16) bf16 act[];  // 128 x FP_PITCH, then 2 W slices of FP_MAXN x 32
-  bf16* wbuf = act + 128 * FP_PITCH;
+  extern __shared__ __align__(16) bf16 act[];  // 128 x FP_PITCH (dynamic LDS)
+  // weight slices: separate objects from `act`, so that reading one never
+  // has to wait for the direct loads into another
+  __shared__ __align__(16) bf16 w0[FP_MAXN * 32], w1[FP_MAXN * 32], w2[FP_MAXN * 32];
   const int tid = threadIdx.x;
-  const int wave = tid >> 6, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;  // wave index in a scalar register
   const int r = lane & 15, q = lane >> 4;
   const long long m0 = (long long)blockIdx.x * 128;
   {
@@ -280,10 +310,10 @@ __global__ void __launch_bounds__(64 * FP_WAVES) k_mlp_fused(FusedNet fn, const 
     const bool last = (l == fn.nl - 1);
     f32x4 acc[4][4];
     switch (ntw) {
-      case 4: fused_layer<4>(fn.W[l], K, N, wave, lane, row0, nw0, act, wbuf, acc); break;
-      case 3: fused_layer<3>(fn.W[l], K, N, wave, lane, row0, nw0, act, wbuf, acc); break;
-      case 2: fused_layer<2>(fn.W[l], K, N, wave, lane, row0, nw0, act, wbuf, acc); break;
-      default: fused_layer<1>(fn.W[l], K, N, wave, lane, row0, nw0, act, wbuf, acc); break;
+      case 4: fused_layer<4>(fn.W[l], K, N, wave, lane, row0, nw0, act, w0, w1, w2, acc); break;
+      case 3: fused_layer<3>(fn.W[l], K, N, wave, lane, row0, nw0, act, w0, w1, w2, acc); break;
+      case 2: fused_layer<2>(fn.W[l], K, N, wave, lane, row0, nw0, act, w0, w1, w2, acc); break;
+      default: fused_layer<1>(fn.W[l], K, N, wave, lane, row0, nw0, act, w0, w1, w2, acc); break;
     }
     // (the layer ends with a barrier: every wave has read this layer's input)
     switch (ntw) {

@@ -102,19 +102,20 @@ struct LevelVals {
 
 // level k of packed column p; levels outside [0, nz) do not exist in this
 // grid and read as a bed filler (h = 1e-4 m, not live)
+// (the loads and the EOS do not depend on liveness, so that a walk over
+// levels has no branches and keeps several levels' loads in flight; a
+// vanished level's values are never used)
 __device__ __forceinline__ LevelVals load_level(const StateView& s, int p, int k, double rho0, double p_ref) {
   LevelVals v{1.0e-4, 0.0, 0.0, 0.0, 0.0, 0.0, false};
   if (k < 0 || k >= s.nz) return v;
   const int P = s.P;
   v.h = LV(s.h, k);
   v.live = v.h > H_VANISHED;
-  if (v.live) {
-    v.T = LV(s.T, k);
-    v.S = LV(s.S, k);
-    v.U = LV(s.ub, k);
-    v.V = LV(s.vb, k);
-    v.rho = eos_wright<double>(v.T, v.S, v.h, rho0, p_ref);
-  }
+  v.T = LV(s.T, k);
+  v.S = LV(s.S, k);
+  v.U = LV(s.ub, k);
+  v.V = LV(s.vb, k);
+  v.rho = eos_wright<double>(v.T, v.S, v.h, rho0, p_ref);
   return v;
 }
 
@@ -294,21 +295,22 @@ __global__ void k_gather(const bf16* Xn, int ldx, const float* Tn, const float* 
 
 // Inference features of wet columns [0, n): normalised bf16 rows written
 // through an LDS tile so that each row is stored with coalesced writes, and
-// the per-column guard (non-zero: plain path).  Workgroup of 256 = 4 waves
-// over 64 columns: wave w takes a quarter of the network levels for all 64
+// the per-column guard (non-zero: plain path).  Workgroup of FW waves over
+// 64 columns: wave w takes an FW-th of the network levels for all 64
 // columns (coalesced level-major reads), walking them top-down with the
 // level above carried in registers; it also checks N^2 < 0 (strict guard)
 // on the pairs whose lower level it owns.  Wave 0 adds the forcing features,
-// wave 3 the surface-pair check; the flags meet in LDS.  Every value is the
+// the last wave (the surface levels) the surface-pair check; the flags meet
+// in LDS.  Every value is the
 // same arithmetic as feature(), so training and inference features agree
 // bit for bit.
-constexpr int FC = 64, FB = 256;
+constexpr int FC = 64, FW = 8, FB = 64 * FW;
 __global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double* cor, int n, int din, int ld,
                                                      const float* st, bf16* X0, unsigned char* guard,
                                                      unsigned int* nfall, int guard_mode, double rho0, double p_ref,
                                                      int nzn, int lev_off) {
   __shared__ bf16 tile[384][FC + 1];
-  __shared__ unsigned char flags[4][FC];
+  __shared__ unsigned char flags[FW][FC];
   const float* mean = st;
   const float* stdv = st + din;
   const float* mn = st + 2 * din;
@@ -322,10 +324,11 @@ __global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double
     tile[f][c] = f2bf((v - mean[f]) / stdv[f]);
   };
   if (p < n) {
-    const int per = (nzn + 3) / 4;
+    const int per = (nzn + FW - 1) / FW;
     const int j0 = part * per, j1 = min(nzn, j0 + per);
     LevelVals above = load_level(s, p, j1 - lev_off, rho0, p_ref);  // the level above this part's highest
     bool unstable = false;
+#pragma unroll 4
     for (int j = j1 - 1; j >= j0; --j) {
       const LevelVals cur = load_level(s, p, j - lev_off, rho0, p_ref);
       float o[5];
@@ -342,7 +345,7 @@ __global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double
       for (int f = 5 * nzn; f < din; ++f) put(f, forcing_feature(s, cor, p, f - 5 * nzn, depth));
       for (int f = din; f < ld; ++f) tile[f][c] = 0;
     }
-    if (part == 3 && guard_mode == RDB_COLPHYS_GUARD_SURFACE) {
+    if (part == FW - 1 && guard_mode == RDB_COLPHYS_GUARD_SURFACE) {
       const int kb = s.kb[p];
       int k_sfc = nz - 1;
       for (int k = nz - 1; k >= kb; --k)
@@ -364,18 +367,19 @@ __global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double
   flags[part][c] = fl;
   __syncthreads();
   if (part == 0 && p < n) {
-    const unsigned char g = flags[0][c] | flags[1][c] | flags[2][c] | flags[3][c];
+    unsigned char g = 0;
+    for (int w = 0; w < FW; ++w) g |= flags[w][c];
     guard[p] = g;
     if (g) atomicAdd(nfall, 1u);
   }
-  // cooperative row stores: the FC rows of this block, 2 bf16 per lane step
+  // cooperative row stores: the FC rows of this block, 2 bf16 per lane,
+  // consecutive lanes along a row
   const int rows = min(FC, n - (int)blockIdx.x * FC);
-  for (int r = 0; r < rows; ++r) {
-    uint32_t* dst = (uint32_t*)(X0 + (size_t)(blockIdx.x * FC + r) * (size_t)ld);
-    for (int c2 = tid; c2 < ld / 2; c2 += FB) {
-      uint32_t lo = tile[2 * c2][r], hi = tile[2 * c2 + 1][r];
-      dst[c2] = lo | (hi << 16);
-    }
+  const int half = ld / 2;
+  for (int idx = tid; idx < rows * half; idx += FB) {
+    const int r = idx / half, c2 = idx - r * half;
+    const uint32_t lo = tile[2 * c2][r], hi = tile[2 * c2 + 1][r];
+    ((uint32_t*)(X0 + (size_t)(blockIdx.x * FC + r) * (size_t)ld))[c2] = lo | (hi << 16);
   }
 }
 
