@@ -190,12 +190,20 @@ def ensemble_bench(args) -> dict:
     K = args.steps
     workers = max(1, min(8, os.cpu_count() or 1))
     t0 = time.perf_counter()
-    with ProcessPoolExecutor(workers) as pool:
-        oceans = list(pool.map(_member, [(sz, args.seed + 100 * i, None) for i, sz in enumerate(sizes)]))
-    log(f"[bench] ensemble: generated {len(oceans)} members in {time.perf_counter() - t0:.0f} s")
+    def generate(jobs, what):
+        out = []
+        with ProcessPoolExecutor(workers) as pool:
+            for i, o in enumerate(pool.map(_member, jobs)):
+                out.append(o)
+                log(f"[bench] ensemble: {what} member {i} {o.nx}x{o.ny}x{o.nz} ({o.n_wet} wet) "
+                    f"at {time.perf_counter() - t0:.0f} s")
+        return out
+
+    oceans = generate([(sz, args.seed + 100 * i, None) for i, sz in enumerate(sizes)], "generated")
     cfg = default_config()
     n_wet = sum(o.n_wet for o in oceans)
     with ColumnEnsemble.from_oceans(oceans, config=cfg) as ens:
+        log(f"[bench] ensemble: {n_wet} wet columns uploaded at {time.perf_counter() - t0:.0f} s")
         ens.autotune(DT)
         ens.step(DT, K)  # builds and instantiates the graph
         ens.upload(oceans)
@@ -213,9 +221,8 @@ def ensemble_bench(args) -> dict:
     guard = {"strict": GUARD_STRICT, "surface": GUARD_SURFACE, "range": GUARD_RANGE}[args.guard]
     net_path = ROOT / "gpurun_out" / "bench_ensemble_closure.bin"
     net_path.parent.mkdir(exist_ok=True)
-    with ProcessPoolExecutor(workers) as pool:
-        train_oceans = list(pool.map(_member, [(sz, args.seed + 100 * i, args.seed + 5000 + i)
-                                               for i, sz in enumerate(sizes)]))
+    train_oceans = generate([(sz, args.seed + 100 * i, args.seed + 5000 + i) for i, sz in enumerate(sizes)],
+                            "training state of")
     with ColumnEnsemble.from_oceans(train_oceans, config=cfg) as ens:
         del train_oceans
         n = ens.mlp_collect(DT, args.train_steps, stride=4)

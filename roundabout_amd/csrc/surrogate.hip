@@ -119,11 +119,16 @@ __device__ __forceinline__ LevelVals load_level(const StateView& s, int p, int k
   return v;
 }
 
-__device__ __forceinline__ double slog(double x, double x0) { return copysign(log10(1.0 + fabs(x) / x0), x); }
+// sign(x) log10(1 + |x| / x0), in single precision: the argument is formed
+// in FP64 (a difference that would cancel in FP32) and the result is a
+// network input, rounded to BF16 anyway
+__device__ __forceinline__ float slog(double x, float inv_x0) {
+  return copysignf(log10f(1.0f + fabsf((float)x) * inv_x0), (float)x);
+}
 
 // the five level slots of `cur` under the level `above`
 __device__ __forceinline__ void level_slots(const LevelVals& cur, const LevelVals& above, double rho0, float out[5]) {
-  out[4] = (float)log10(fmax(cur.h, 1e-30));
+  out[4] = log10f(fmaxf((float)cur.h, 1e-30f));
   if (!cur.live) {
     out[0] = out[1] = out[2] = out[3] = 0.f;
   } else if (!above.live) {
@@ -132,10 +137,10 @@ __device__ __forceinline__ void level_slots(const LevelVals& cur, const LevelVal
     out[2] = (float)cur.U;
     out[3] = (float)cur.V;
   } else {
-    out[0] = (float)slog(GRAVITY * (cur.rho - above.rho) / rho0, 1.0e-7);
+    out[0] = slog(GRAVITY * (cur.rho - above.rho) / rho0, 1.0e7f);
     out[1] = (float)(cur.S - above.S);
-    out[2] = (float)slog(cur.U - above.U, 1.0e-3);
-    out[3] = (float)slog(cur.V - above.V, 1.0e-3);
+    out[2] = slog(cur.U - above.U, 1.0e3f);
+    out[3] = slog(cur.V - above.V, 1.0e3f);
   }
 }
 
@@ -328,7 +333,6 @@ __global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double
     const int j0 = part * per, j1 = min(nzn, j0 + per);
     LevelVals above = load_level(s, p, j1 - lev_off, rho0, p_ref);  // the level above this part's highest
     bool unstable = false;
-#pragma unroll 4
     for (int j = j1 - 1; j >= j0; --j) {
       const LevelVals cur = load_level(s, p, j - lev_off, rho0, p_ref);
       float o[5];
@@ -729,8 +733,9 @@ int surr_save(rdb_colphys* h, const char* path) {
   if (S->net.get_params(p.data())) return perr(RDB_COLPHYS_ERR_SETUP, "mlp_save: %s", S->net.err.c_str());
   FILE* f = std::fopen(path, "wb");
   if (!f) return perr(RDB_COLPHYS_ERR_IO, "mlp_save: cannot open %s", path);
-  // version 2: buoyancy-jump / signed-log level encoding of the features
-  const char magic[8] = {'R', 'D', 'B', 'S', 'U', 'R', 'R', '2'};
+  // version 3: buoyancy-jump / signed-log level encoding of the features,
+  // logarithms in single precision
+  const char magic[8] = {'R', 'D', 'B', 'S', 'U', 'R', 'R', '3'};
   int hdr[4] = {S->nz, S->din, S->dout, (int)S->widths.size()};
   bool ok = std::fwrite(magic, 1, 8, f) == 8 && std::fwrite(hdr, 4, 4, f) == 4 &&
             std::fwrite(S->widths.data(), 4, S->widths.size(), f) == S->widths.size() &&
@@ -746,7 +751,7 @@ int surr_load(rdb_colphys* h, const char* path) {
   if (!f) return perr(RDB_COLPHYS_ERR_IO, "mlp_load: cannot open %s", path);
   char magic[8];
   int hdr[4];
-  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "RDBSURR2", 8) == 0 && std::fread(hdr, 4, 4, f) == 4;
+  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "RDBSURR3", 8) == 0 && std::fread(hdr, 4, 4, f) == 4;
   if (!ok || hdr[0] != net_levels(h) || hdr[3] < 2 || hdr[3] > 16) {
     std::fclose(f);
     return perr(RDB_COLPHYS_ERR_BAD_SHAPE, "mlp_load: %s is not a network for nz = %d", path, net_levels(h));
