@@ -105,7 +105,11 @@ def surrogate_leg(ocean, cfg, args, plain_value: float) -> dict:
     net_path.parent.mkdir(exist_ok=True)
     t0 = time.perf_counter()
     with ColumnPhysics.from_ocean(synth.redraw(ocean, args.seed + 1000), config=cfg) as cp:
-        n = cp.mlp_collect(DT, args.train_steps)
+        for s in range(args.train_states):
+            if s:
+                cp.upload(synth.redraw(ocean, args.seed + 1000 + s))
+            n = cp.mlp_collect(DT, args.train_steps)
+            log(f"[bench] surrogate: training state {s}: {n} columns at {time.perf_counter() - t0:.0f} s")
         cp.synchronize()
         t_col = time.perf_counter() - t0
         t1 = time.perf_counter()
@@ -152,8 +156,9 @@ def surrogate_leg(ocean, cfg, args, plain_value: float) -> dict:
         "salt_rel_residual": abs(s1 - s0) / abs(s0),
         "fallback_fraction_last_step": st["fallback_fraction"],
         "guard": args.guard,
-        "training": {"columns": n, "source": f"{args.train_steps} plain steps of an independent state draw "
-                     f"(seed {args.seed + 1000}) on the same grid", "epochs": args.epochs, "batch": 8192,
+        "training": {"columns": n, "source": f"{args.train_steps} plain steps of each of {args.train_states} "
+                     f"independent state draws (seeds {args.seed + 1000}..{args.seed + 999 + args.train_states}) "
+                     "on the same grid", "epochs": args.epochs, "batch": 8192,
                      "lr": 1e-3, "seconds": t_train, "val_loss_first": vl[0], "val_loss_last": vl[-1]},
         "mlp": {"forward_ms": mlp_ms, "flop_per_step": st["mlp_flop_per_step"], "achieved_tflops": tflops,
                 "peak_tflops": PEAK_BF16_TFLOPS, "mfma_frac": (tflops / PEAK_BF16_TFLOPS) if tflops else None,
@@ -324,14 +329,15 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "split"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-closure leg")
     ap.add_argument("--ensemble", action="store_true",
                     help="the 16-member batched ensemble (about 10.5 M columns) instead of the single tile")
     ap.add_argument("--guard", default="surface", choices=["strict", "surface", "range"])
-    ap.add_argument("--epochs", type=int, default=40)
-    ap.add_argument("--train-steps", type=int, default=2, help="plain steps recorded as training columns")
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--train-steps", type=int, default=2, help="plain steps recorded per training state")
+    ap.add_argument("--train-states", type=int, default=6, help="independent state draws the network trains on")
     args = ap.parse_args()
 
     if args.copy_probe:
@@ -431,7 +437,7 @@ def main() -> int:
         out["surrogate"] = surrogate_leg(ocean, cfg, args, value)
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(ocean, args.cpu_seconds, (384, 256))
+        out["cpu_baseline"] = cpu_baseline(ocean, args.cpu_seconds, (720, 540))
     print(json.dumps(out), flush=True)
     return 0
 

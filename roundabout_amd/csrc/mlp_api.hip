@@ -46,7 +46,8 @@ int upload_x(mlp::Net& net, int batch, int Mpad, const float* x) {
 extern "C" {
 
 int rdb_mlp_eval(const int* widths, int n_widths, const float* params, int batch, const float* x, float* y) {
-  if (batch < 1 || !x || !y) return rdb_colphys_push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "mlp_eval: bad batch or NULL");
+  if (batch < 1 || batch > (1 << 22) || !x || !y)
+    return rdb_colphys_push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "mlp_eval: batch outside [1, 2^22] or NULL");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, "mlp_eval: no HIP device available");
@@ -78,8 +79,8 @@ int rdb_mlp_eval(const int* widths, int n_widths, const float* params, int batch
 
 int rdb_mlp_loss_grad(const int* widths, int n_widths, const float* params, int batch, const float* x,
                       const float* t, const float* w, float* loss, float* grad) {
-  if (batch < 1 || !x || !t || !w || !loss || !grad)
-    return rdb_colphys_push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "mlp_loss_grad: bad batch or NULL");
+  if (batch < 1 || batch > (1 << 22) || !x || !t || !w || !loss || !grad)
+    return rdb_colphys_push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "mlp_loss_grad: batch outside [1, 2^22] or NULL");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, "mlp_loss_grad: no HIP device available");

@@ -265,24 +265,29 @@ __global__ void k_col_stats(const float* X, const float* W, long long n, int d, 
   }
 }
 
+// The normalisation kernels walk n x ld elements with a grid-stride loop:
+// a dispatch counts its work-items in 32 bits, and a training set of 13 M
+// columns x 384 inputs is 5e9 elements, so the grid is capped (gs_capped).
 __global__ void k_norm_features(const float* X, long long n, int din, const float* mean, const float* stdv, bf16* Xn,
                                 int ld) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * ld) return;
-  long long r = i / ld;
-  int c = (int)(i % ld);
-  Xn[i] = f2bf(c < din ? (X[(size_t)r * din + c] - mean[c]) / stdv[c] : 0.f);
+  const long long total = n * ld, stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    long long r = i / ld;
+    int c = (int)(i % ld);
+    Xn[i] = f2bf(c < din ? (X[(size_t)r * din + c] - mean[c]) / stdv[c] : 0.f);
+  }
 }
 
 __global__ void k_norm_targets(const float* T, const float* W, long long n, int dout, const float* mean,
                                const float* stdv, float* Tn, float* Wn, int ld) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * ld) return;
-  long long r = i / ld;
-  int c = (int)(i % ld);
-  bool in = c < dout;
-  Tn[i] = in ? (T[(size_t)r * dout + c] - mean[c]) / stdv[c] : 0.f;
-  Wn[i] = in ? W[(size_t)r * dout + c] : 0.f;
+  const long long total = n * ld, stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    long long r = i / ld;
+    int c = (int)(i % ld);
+    bool in = c < dout;
+    Tn[i] = in ? (T[(size_t)r * dout + c] - mean[c]) / stdv[c] : 0.f;
+    Wn[i] = in ? W[(size_t)r * dout + c] : 0.f;
+  }
 }
 
 // rows idx[b] of the normalised set into a training batch (rows >= nb zero)
@@ -388,6 +393,8 @@ __global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double
 }
 
 inline dim3 gs(long long n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
+// for grid-stride kernels: at most 2^20 workgroups (2^28 work-items at 256)
+inline dim3 gs_capped(long long n, int b) { return dim3((unsigned)std::min<long long>((n + b - 1) / b, 1LL << 20)); }
 
 }  // namespace
 
@@ -494,10 +501,11 @@ int surr_train(rdb_colphys* h, const rdb_colphys_mlp_config* cfg, float* train_l
                int* n_done) {
   SurrState* S = h->surr;
   if (!S || S->n < 64) return perr(RDB_COLPHYS_ERR_NOT_INITIALISED, "mlp_train: collect training columns first");
-  if (!cfg || cfg->batch < 64 || cfg->hidden < 1 || cfg->n_hidden < 1 || cfg->epochs < 0)
+  if (!cfg || cfg->batch < 64 || cfg->batch > (1 << 20) || cfg->hidden < 1 || cfg->n_hidden < 1 || cfg->epochs < 0)
     return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "mlp_train: bad configuration");
   const long long N = S->n;
   const int din = S->din, dout = S->dout;
+  if (N > 0xffffffffLL) return perr(RDB_COLPHYS_ERR_BAD_SHAPE, "mlp_train: more than 2^32 training columns");
   // statistics over the whole collected set
   std::vector<float> stats(4 * (size_t)din + 2 * (size_t)dout);
   float* dst = nullptr;
@@ -549,9 +557,9 @@ int surr_train(rdb_colphys* h, const rdb_colphys_mlp_config* cfg, float* train_l
   SG_TRY(hipMalloc((void**)&S->Tn, 4 * (size_t)N * ldt));
   SG_TRY(hipMalloc((void**)&S->Wn, 4 * (size_t)N * ldt));
   SG_TRY(hipMalloc((void**)&S->idx, 4 * (size_t)N));
-  hipLaunchKernelGGL(k_norm_features, gs(N * ldx, 256), dim3(256), 0, h->stream, S->Xraw, N, din, dst, dst + din,
-                     S->Xn, ldx);
-  hipLaunchKernelGGL(k_norm_targets, gs(N * ldt, 256), dim3(256), 0, h->stream, S->Traw, S->Wraw, N, dout,
+  hipLaunchKernelGGL(k_norm_features, gs_capped(N * ldx, 256), dim3(256), 0, h->stream, S->Xraw, N, din, dst,
+                     dst + din, S->Xn, ldx);
+  hipLaunchKernelGGL(k_norm_targets, gs_capped(N * ldt, 256), dim3(256), 0, h->stream, S->Traw, S->Wraw, N, dout,
                      dst + 4 * din, dst + 4 * din + dout, S->Tn, S->Wn, ldt);
   SG_TRY(hipGetLastError());
   std::vector<float> rowsum((size_t)N);
