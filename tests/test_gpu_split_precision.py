@@ -25,18 +25,24 @@ def split_errors(orc, ocean, nsteps):
         h0, s0 = cp.heat_salt_totals()
         cp.step(DT, nsteps)
         h1, s1 = cp.heat_salt_totals()
-        got = {n: cp.get_field(n) for n in ("temperature", "salinity", "kv", "kt", "kbl")}
+        got = {n: cp.get_field(n) for n in ("temperature", "salinity", "tracer_1", "tracer_2", "kv", "kt", "ks",
+                                            "bl_depth", "b0", "kbl")}
     wet = ocean.wet > 0.5
     live = (ocean.h > 1.5e-4) & wet[:, :, None]
     li = np.zeros(ocean.kv_shape if hasattr(ocean, "kv_shape") else ost.kv.shape, dtype=bool)
     li[:, :, 1:-1] = live[:, :, 1:] & live[:, :, :-1]
     out = {}
-    for n, ref in (("temperature", ost.ocean.T), ("salinity", ost.ocean.S)):
+    for n, ref in (("temperature", ost.ocean.T), ("salinity", ost.ocean.S), ("tracer_1", ost.ocean.passive[0]),
+                   ("tracer_2", ost.ocean.passive[1])):
         d = (got[n] - ref)[live]
         out[n] = {"max_abs": float(np.abs(d).max()), "rms": float(np.sqrt(np.mean(d * d)))}
-    for n, ref in (("kv", ost.kv), ("kt", ost.kt)):
+    for n, ref in (("kv", ost.kv), ("kt", ost.kt), ("ks", ost.ks)):
         r = np.abs(got[n] - ref)[li] / np.abs(ref)[li]
         out[n] = {"max_rel": float(r.max()), "rms_rel": float(np.sqrt(np.mean(r * r)))}
+    d = (got["bl_depth"] - ost.ocean.bl_depth)[wet]
+    out["bl_depth"] = {"max_abs_m": float(np.abs(d).max()), "rms_m": float(np.sqrt(np.mean(d * d)))}
+    r = (np.abs(got["b0"] - ost.b0) / np.maximum(np.abs(ost.b0), 1e-30))[wet]
+    out["b0"] = {"max_rel": float(r.max()), "rms_rel": float(np.sqrt(np.mean(r * r)))}
     out["kbl_mismatch_fraction"] = float((got["kbl"] != ost.kbl)[wet].mean())
     dep = nsteps * DT / (cfg.rho0 * cfg.cp) * float((ocean.Q_heat * ocean.wet).sum())
     out["heat_conservation_rel"] = abs((h1 - h0) - dep) / abs(h0)
@@ -61,5 +67,8 @@ def test_split_precision_error_budget(orc, gpu_lib, workload):
         assert e["temperature"]["max_abs"] < 0.1 and e["temperature"]["rms"] < 5e-4
         assert e["salinity"]["max_abs"] < 0.02 and e["salinity"]["rms"] < 1e-4
         assert e["kt"]["rms_rel"] < 1e-3
+    # passives follow salinity (same factorisation); ks = kt clipped at the same floor
+    assert e["tracer_1"]["rms"] < 10 * e["salinity"]["rms"] + 1e-9 and e["ks"]["rms_rel"] <= 2 * e["kt"]["rms_rel"]
+    assert e["b0"]["max_rel"] < 1e-6  # B0 is a forcing combination: FP32 rounding only
     assert e["kbl_mismatch_fraction"] < 0.02
     assert e["heat_conservation_rel"] < 1e-13 and e["salt_conservation_rel"] < 1e-13
