@@ -416,6 +416,7 @@ def main() -> int:
             "kernel": "k_column_step (fused, one lane per column, hipGraph replay)",
             "block_threads": stats["block_threads"],
             "lds_levels": stats["lds_levels"],
+            "column_order": "deepest first" if stats["order_mode"] == 1 else "grid order",
             "live_levels_mean": stats["live_levels"] / max(stats["wet_columns"], 1),
         },
         "roofline": {

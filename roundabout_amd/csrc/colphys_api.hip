@@ -771,11 +771,13 @@ int rdb_colphys_set_field(rdb_colphys* h, const char* name, const double* data, 
     // live levels per wet column (from the lowest live layer up), for the
     // algorithmic-bytes figure of the step
     long long live = 0;
+    h->col_live.assign((size_t)nx * ny, 0);
     for (int p = 0; p < h->n_wet; ++p) {
       int c = h->col_host[(size_t)p], kb = nz;
       for (int k = 0; k < nz; ++k)
         if (data[(size_t)k * nx * ny + c] > colphys::H_VANISHED) { kb = k; break; }
       live += nz - kb;
+      h->col_live[(size_t)c] = nz - kb;
     }
     h->live_levels = live;
   }
@@ -1020,6 +1022,68 @@ int rdb_colphys_synchronize(rdb_colphys* h) {
   return RDB_COLPHYS_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Re-pack the wet columns of a single-grid plan into `order` (a permutation
+// of its wet grid columns): every packed array is permuted on the device, the
+// column map follows.  Exact (a permutation), so the state is unchanged.
+int repack(rdb_colphys* h, const std::vector<int>& order) {
+  const int n = h->n_wet, P = h->P, nz = h->nz;
+  if ((int)order.size() != n) return push_error(RDB_COLPHYS_ERR_SETUP, "repack: order has the wrong length");
+  std::vector<int> where((size_t)h->n_all, -1), newpos((size_t)std::max(n, 1));
+  for (int i = 0; i < n; ++i) where[(size_t)order[(size_t)i]] = i;
+  for (int p = 0; p < n; ++p) {
+    const int np = where[(size_t)h->col_host[(size_t)p]];
+    if (np < 0) return push_error(RDB_COLPHYS_ERR_SETUP, "repack: order is not a permutation of the wet columns");
+    newpos[(size_t)p] = np;
+  }
+  const int maxlev = std::max(nz + 1, nz * std::max(h->n_passive, 1));
+  int* d_newpos = nullptr;
+  void* tmp = nullptr;
+  HIP_TRY(hipMalloc((void**)&d_newpos, sizeof(int) * newpos.size()));
+  hipError_t e = hipMalloc(&tmp, (size_t)8 * (size_t)maxlev * (size_t)P);
+  if (e != hipSuccess) {
+    (void)hipFree(d_newpos);
+    return push_error(RDB_COLPHYS_ERR_SETUP, "repack: hipMalloc failed: %s", hipGetErrorString(e));
+  }
+  e = hipMemcpyAsync(d_newpos, newpos.data(), sizeof(int) * newpos.size(), hipMemcpyHostToDevice, h->stream);
+  StateView& sv = h->s;
+  auto perm_d = [&](double* a, int nlev) {
+    if (e != hipSuccess || !a || n == 0) return;
+    hipLaunchKernelGGL(colphys::k_permute_d, dim3(grid1(n, 256).x, (unsigned)nlev), dim3(256), 0, h->stream, a,
+                       (double*)tmp, d_newpos, n, P, nlev);
+    e = hipMemcpy2DAsync(a, 8 * (size_t)P, tmp, 8 * (size_t)P, 8 * (size_t)n, (size_t)nlev, hipMemcpyDeviceToDevice,
+                         h->stream);
+  };
+  auto perm_i = [&](int* a, int nlev) {
+    if (e != hipSuccess || !a || n == 0) return;
+    hipLaunchKernelGGL(colphys::k_permute_i, dim3(grid1(n, 256).x, (unsigned)nlev), dim3(256), 0, h->stream, a,
+                       (int*)tmp, d_newpos, n, P, nlev);
+    e = hipMemcpy2DAsync(a, 4 * (size_t)P, tmp, 4 * (size_t)P, 4 * (size_t)n, (size_t)nlev, hipMemcpyDeviceToDevice,
+                         h->stream);
+  };
+  for (double* a : {sv.h, sv.ub, sv.vb, sv.T, sv.S, sv.rho}) perm_d(a, nz);
+  perm_d(sv.tr, nz * std::max(h->n_passive, 1));
+  for (double* a : {sv.kv, sv.kt, sv.ks, sv.gamma_t, sv.gamma_s}) perm_d(a, nz + 1);
+  for (double* a : {sv.tau_tot, sv.Qh, sv.Qs, sv.wet, sv.bl, sv.b0, h->d_txc, h->d_tyc, h->d_smag, h->d_shelf,
+                    h->d_coriolis})
+    perm_d(a, 1);
+  for (int* a : {sv.kbl, sv.ktop, sv.kb, sv.fdirty, h->d_col}) perm_i(a, 1);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(tmp);
+  (void)hipFree(d_newpos);
+  if (e != hipSuccess) return push_error(RDB_COLPHYS_ERR_SETUP, "repack failed: %s", hipGetErrorString(e));
+  std::copy(order.begin(), order.end(), h->col_host.begin());
+  return RDB_COLPHYS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int rdb_colphys_autotune(rdb_colphys* h, double dt) {
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "autotune: bad handle");
   if (h->block_pinned) return RDB_COLPHYS_OK;
@@ -1045,9 +1109,10 @@ int rdb_colphys_autotune(rdb_colphys* h, double dt) {
       }
     }
   };
-  save(false);
   // candidates: launch block x live levels of Thomas state held in LDS
-  // (dynamic LDS per block <= 64 KiB)
+  // (dynamic LDS per block <= 64 KiB) x column packing (single grids: the
+  // current order, grid order, and deepest-first by live levels, which
+  // keeps a wavefront's columns of one depth together)
   const size_t nq = (size_t)(3 + std::min(h->n_passive, 4) + (h->cfg.kt_bg != h->cfg.ks_bg ? 1 : 0));
   std::vector<std::pair<int, int>> cands;
   for (int b : {64, 128, 256}) {
@@ -1056,29 +1121,49 @@ int rdb_colphys_autotune(rdb_colphys* h, double dt) {
       if (m <= mmax) cands.push_back({b, m});
     }
   }
+  std::vector<std::vector<int>> orders{std::vector<int>(h->col_host.begin(), h->col_host.begin() + h->n_wet)};
+  std::vector<int> order_modes{h->order_mode};
+  if (h->members.empty() && h->col_live.size() == (size_t)h->n_all) {
+    std::vector<int> natural = orders[0];
+    std::sort(natural.begin(), natural.end());
+    std::vector<int> deep = natural;
+    std::stable_sort(deep.begin(), deep.end(), [&](int a, int b) { return h->col_live[(size_t)a] > h->col_live[(size_t)b]; });
+    for (auto& [o, mode] : {std::pair<std::vector<int>*, int>{&natural, 0}, {&deep, 1}})
+      if (*o != orders[0]) {
+        orders.push_back(*o);
+        order_modes.push_back(mode);
+      }
+  }
   double best = 1e300;
   int best_b = h->block, best_m = h->lds_levels;
+  size_t best_o = 0;
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  for (auto [b, mm] : cands) {
-    h->block = b;
-    h->lds_levels = mm;
-    st = colphys::ensure_work(h);
-    if (!st) st = colphys::launch_step_all(h, dt, false);  // warm-up
+  for (size_t oi = 0; oi < orders.size() && !st; ++oi) {
+    if (oi > 0) st = repack(h, orders[oi]);
     if (st) break;
-    (void)hipEventRecord(e0, h->stream);
-    for (int r = 0; r < 3 && !st; ++r) st = colphys::launch_step_all(h, dt, false);
-    (void)hipEventRecord(e1, h->stream);
-    (void)hipEventSynchronize(e1);
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    if (ms < best) {
-      best = ms;
-      best_b = b;
-      best_m = mm;
+    save(false);  // the state in this packing
+    for (auto [b, mm] : cands) {
+      h->block = b;
+      h->lds_levels = mm;
+      st = colphys::ensure_work(h);
+      if (!st) st = colphys::launch_step_all(h, dt, false);  // warm-up
+      if (st) break;
+      (void)hipEventRecord(e0, h->stream);
+      for (int r = 0; r < 3 && !st; ++r) st = colphys::launch_step_all(h, dt, false);
+      (void)hipEventRecord(e1, h->stream);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) {
+        best = ms;
+        best_b = b;
+        best_m = mm;
+        best_o = oi;
+      }
+      save(true);
     }
-    save(true);
   }
   (void)hipStreamSynchronize(h->stream);
   (void)hipEventDestroy(e0);
@@ -1086,6 +1171,10 @@ int rdb_colphys_autotune(rdb_colphys* h, double dt) {
   (void)hipFree(bak);
   h->block = best_b;
   h->lds_levels = best_m;
+  if (!st && orders.size() > 1) {
+    st = repack(h, orders[best_o]);  // the state itself is unchanged: a permutation
+    if (!st) h->order_mode = order_modes[best_o];
+  }
   if (st) return st;
   return colphys::ensure_work(h);
 }

@@ -60,6 +60,7 @@ struct rdb_colphys {
   int block = 256;
   int lds_levels = 0;  // live levels of Thomas state kept in LDS per lane
   long long live_levels = 0;  // sum over wet columns of live levels
+  std::vector<int> col_live;  // live levels per grid column (from the last h_layer upload)
   bool block_pinned = false;
   int order_mode = 0;
   int precision = RDB_COLPHYS_PREC_FP64;

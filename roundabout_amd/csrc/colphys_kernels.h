@@ -37,6 +37,8 @@ struct StateView {
 
 __global__ void k_pack_layer(const double* src, double* dst, const int* col, int ncol, int P, int nlev,
                              long long plane);
+__global__ void k_permute_d(const double* src, double* dst, const int* newpos, int n, int P, int nlev);
+__global__ void k_permute_i(const int* src, int* dst, const int* newpos, int n, int P, int nlev);
 __global__ void k_unpack_layer(const double* src, double* dst, const int* col, int ncol, int P, int nlev,
                                long long plane);
 __global__ void k_unpack_int(const int* src, double* dst, const int* col, int ncol);

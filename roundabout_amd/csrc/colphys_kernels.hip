@@ -25,6 +25,21 @@ __global__ void k_pack_layer(const double* __restrict__ src, double* __restrict_
   LV(dst, k) = src[(size_t)k * (size_t)plane + (size_t)col[p]];
 }
 
+// Column repacking for the autotuner: dst[k][newpos[p]] = src[k][p] for the
+// packed columns p < n (the wet ones), nlev levels of pitch P.
+__global__ void k_permute_d(const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ newpos,
+                            int n, int P, int nlev) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+  if (p >= n || k >= nlev) return;
+  dst[(size_t)k * (size_t)P + (size_t)newpos[p]] = src[(size_t)k * (size_t)P + (size_t)p];
+}
+__global__ void k_permute_i(const int* __restrict__ src, int* __restrict__ dst, const int* __restrict__ newpos, int n,
+                            int P, int nlev) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+  if (p >= n || k >= nlev) return;
+  dst[(size_t)k * (size_t)P + (size_t)newpos[p]] = src[(size_t)k * (size_t)P + (size_t)p];
+}
+
 __global__ void k_unpack_layer(const double* __restrict__ src, double* __restrict__ dst,
                                const int* __restrict__ col, int ncol, int P, int nlev, long long plane) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;

@@ -123,6 +123,21 @@ def test_fused_two_factor_and_natural_order(orc, gpu_lib):
         compare_state(cp, ost, ocean.nz, ocean.n_passive)
 
 
+@pytest.mark.parametrize("sort", [False, True])
+def test_autotune_repacking_keeps_bitwise_parity(orc, gpu_lib, sort):
+    say(f"[case] autotune (launch shape x column packing) mid-run, tile crop 120x96x75 seed 7, sort={sort}")
+    ocean = synth.tile(7, nx=120, ny=96)
+    cfg = default_config()
+    ost = run_oracle(orc, ocean, cfg, 24)
+    with ColumnPhysics.from_ocean(ocean, config=cfg, sort=sort) as cp:
+        cp.step(DT, 3)
+        cp.autotune(DT)  # may re-pack the columns; restores the state it times on
+        st = cp.stats()
+        say(f"autotuned: block {st['block_threads']} lds_levels {st['lds_levels']} order_mode {st['order_mode']}")
+        cp.step(DT, 21)
+        compare_state(cp, ost, ocean.nz, ocean.n_passive)
+
+
 def test_fused_kpp_off(orc, gpu_lib):
     say("[case] fused step, use_kpp=0, small seed 7")
     ocean = synth.small(7)
