@@ -300,7 +300,7 @@ def ensemble_bench(args) -> dict:
     }
 
 
-def copy_probe(nbytes: int = 1 << 30, reps: int = 3) -> None:
+def copy_probe(nbytes: int = 4 << 30, reps: int = 10) -> None:
     """Device-to-device copies of a known size (the runtime's copy kernel),
     the calibration run for the FETCH_SIZE / WRITE_SIZE counters."""
     import ctypes
@@ -310,13 +310,19 @@ def copy_probe(nbytes: int = 1 << 30, reps: int = 3) -> None:
         if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
             raise RuntimeError("hipMalloc failed")
     hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # warm-up
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
     for _ in range(reps):
         if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
             raise RuntimeError("hipMemcpy failed")
     hip.hipDeviceSynchronize()
+    dt = (time.perf_counter() - t0) / reps
     hip.hipFree(a)
     hip.hipFree(b)
-    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
+    # a copy reads and writes nbytes each
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps, "ms_per_copy": dt * 1e3,
+                      "read_plus_write_GBs": 2 * nbytes / dt / 1e9}), flush=True)
 
 
 def main() -> int:
