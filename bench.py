@@ -20,6 +20,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -28,8 +29,7 @@ sys.path.insert(0, str(ROOT))
 
 import numpy as np  # noqa: E402
 
-METRIC = json.loads((ROOT / "BASELINE.json").read_text())["metric"] if (ROOT / "BASELINE.json").exists() else \
-    "column-steps/s"
+METRIC = "column-steps/s surrogate vs plain FP64 vmix+vdiff at \u22640.01 K T error; HBM GB/s"
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0  # dense BF16 MFMA
 DT = 3600.0
@@ -95,14 +95,13 @@ def surrogate_leg(ocean, cfg, args, plain_value: float) -> dict:
     """The learned closure on the same tile: train on the GPU on columns the
     plain path generates from an independent state draw on the same grid,
     then run K guarded surrogate steps from the benchmark's initial state and
-    judge them against K plain steps from that state."""
+    compare them with K plain steps from that state."""
     from roundabout_amd import synth
     from roundabout_amd.rdb import CLOSURE_SURROGATE, GUARD_RANGE, GUARD_STRICT, GUARD_SURFACE, ColumnPhysics
 
     guard = {"strict": GUARD_STRICT, "surface": GUARD_SURFACE, "range": GUARD_RANGE}[args.guard]
     K = args.steps
-    net_path = ROOT / "gpurun_out" / "bench_closure.bin"
-    net_path.parent.mkdir(exist_ok=True)
+    net_path = Path(tempfile.gettempdir()) / f"rdb_bench_closure_{os.getpid()}.bin"
     t0 = time.perf_counter()
     with ColumnPhysics.from_ocean(synth.redraw(ocean, args.seed + 1000), config=cfg) as cp:
         for s in range(args.train_states):
@@ -184,9 +183,9 @@ def _member(args):
 
 
 def ensemble_bench(args) -> dict:
-    """16 members of uneven grids, each seeded separately, stepped
+    """The batched ensemble: 16 members of uneven grids, each seeded separately, stepped
     by one persistent graph-captured launch per step; then the guarded
-    surrogate over the same ensemble, judged against the all-plain run."""
+    surrogate over the same ensemble, compared with the all-plain run."""
     from concurrent.futures import ProcessPoolExecutor
 
     from roundabout_amd.rdb import (CLOSURE_SURROGATE, GUARD_RANGE, GUARD_STRICT, GUARD_SURFACE, ColumnEnsemble,
@@ -224,8 +223,7 @@ def ensemble_bench(args) -> dict:
     log(f"[bench] ensemble plain: {K} steps in {wall_plain * 1e3:.1f} ms "
         f"(block {st_plain['block_threads']}, lds {st_plain['lds_levels']})")
     guard = {"strict": GUARD_STRICT, "surface": GUARD_SURFACE, "range": GUARD_RANGE}[args.guard]
-    net_path = ROOT / "gpurun_out" / "bench_ensemble_closure.bin"
-    net_path.parent.mkdir(exist_ok=True)
+    net_path = Path(tempfile.gettempdir()) / f"rdb_bench_ensemble_closure_{os.getpid()}.bin"
     train_oceans = generate([(sz, args.seed + 100 * i, args.seed + 5000 + i) for i, sz in enumerate(sizes)],
                             "training state of")
     with ColumnEnsemble.from_oceans(train_oceans, config=cfg) as ens:
@@ -351,7 +349,7 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the column path shards by columns "
-                          "(see DESIGN.md) but no multi-GPU run is part of it", "n_gpus": args.gpus}))
+                          "but no multi-GPU run is part of it", "n_gpus": args.gpus}))
         return 2
 
     if args.ensemble:

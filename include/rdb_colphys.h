@@ -55,7 +55,7 @@ extern "C" {
 /* vmix_assemble's guard status (rdb_ocean_vmix.F90:2248-2251, 2265-2277). */
 #define RDB_COLPHYS_GUARD_TRIPPED 1
 
-/* Precision of the diffusivity assembly (BASELINE split-precision config). */
+/* Precision of the diffusivity assembly (split precision: FP32 assembly, FP64 solve). */
 #define RDB_COLPHYS_PREC_FP64 0  /* every stage in FP64                       */
 #define RDB_COLPHYS_PREC_SPLIT 1 /* EOS..clip in FP32, Thomas + totals in FP64 */
 

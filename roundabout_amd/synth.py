@@ -21,7 +21,7 @@ Land follows the reference land contract (``h = H_VANISHED``, c = 0,
 ``src/core/ocean/state/rdb_ocean_state_seed.F90:1842-1880``); velocity faces
 touching land or a filler layer are zero.
 
-Physical draws (per BASELINE): T = 2 + A exp(-z/500 m) + N(0, 0.1) with
+Physical draws: T = 2 + A exp(-z/500 m) + N(0, 0.1) with
 A ~ U(10, 28) per column; S = 35 + N(0, 0.3); u, v ~ N(0, 0.1) m/s per layer
 and face; |tau| lognormal with median 0.08 N/m^2 split by a uniform angle;
 Q_heat ~ N(-50, 100) W/m^2; Q_salt = 0; f on a 20-60 N beta-plane; passive

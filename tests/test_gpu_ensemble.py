@@ -3,8 +3,8 @@ stepped by one persistent launch per step.
 
 The plain ensemble must reproduce, bit for bit, each member stepped alone
 (which the single-grid parity tests tie to the oracle); the guarded learned
-closure over an uneven-nz ensemble must conserve heat and salt and is judged
-against the all-plain ensemble.
+closure over an uneven-nz ensemble must conserve heat and salt and is compared
+with the all-plain ensemble.
 """
 
 from __future__ import annotations
