@@ -143,6 +143,11 @@ def surrogate_leg(ocean, cfg, args, plain_value: float) -> dict:
     t_rms, s_rms = live_rms(ocean, T, T_ref), live_rms(ocean, S, S_ref)
     mlp_ms = st["mlp_forward_ms"]
     tflops = st["mlp_flop_per_step"] / (mlp_ms * 1e-3) / 1e12 if mlp_ms > 0 else None
+    # training work: per epoch, forward + backward (about twice the forward)
+    # over the training columns and a forward over the held-out ones
+    fwd_per_col = st["mlp_flop_per_step"] / max(st["wet_columns"], 1)
+    n_val = round(0.05 * n)
+    train_tflops = args.epochs * fwd_per_col * (3 * (n - n_val) + n_val) / t_train / 1e12
     return {
         "value": value,
         "unit": "column-steps/s",
@@ -155,7 +160,8 @@ def surrogate_leg(ocean, cfg, args, plain_value: float) -> dict:
         "salt_rel_residual": abs(s1 - s0) / abs(s0),
         "fallback_fraction_last_step": st["fallback_fraction"],
         "guard": args.guard,
-        "training": {"columns": n, "source": f"{args.train_steps} plain steps of each of {args.train_states} "
+        "training": {"columns": n, "tflops": train_tflops, "mfma_frac": train_tflops / PEAK_BF16_TFLOPS,
+                     "source": f"{args.train_steps} plain steps of each of {args.train_states} "
                      f"independent state draws (seeds {args.seed + 1000}..{args.seed + 999 + args.train_states}) "
                      "on the same grid", "epochs": args.epochs, "batch": 8192,
                      "lr": 1e-3, "seconds": t_train, "val_loss_first": vl[0], "val_loss_last": vl[-1]},

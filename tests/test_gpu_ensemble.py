@@ -123,3 +123,54 @@ def test_ensemble_guarded_surrogate(gpu_lib, tmp_path):
           flush=True)
     assert all(np.isfinite(r) for r in rms) and max(rms) < 0.2
     assert 0.0 <= fb < 1.0 and used > 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_ensemble_edge_members_bitwise(gpu_lib, precision):
+    """No passive tracers, a single-layer member and a z-star member next to
+    z-level ones; FP64 and split precision, each against the members alone."""
+    from roundabout_amd.rdb import PREC_FP64, PREC_SPLIT
+    print(f"[case] ensemble edge members ({precision}): nz 1/20/75, no passives, 6 steps", flush=True)
+    mode = PREC_SPLIT if precision == "split" else PREC_FP64
+    oceans = [synth.make_ocean(30, 20, 1, 80, land_frac=0.2, topo="zstar", n_passive=0),
+              synth.make_ocean(40, 30, 20, 81, land_frac=0.1, topo="zstar", n_passive=0),
+              synth.make_ocean(36, 28, 75, 82, land_frac=0.3, topo="zlevel", dmin=50.0, dmax=5500.0, n_passive=0)]
+    cfg = default_config()
+    fields = ("temperature", "salinity", "kv", "kt", "ks", "bl_depth", "kbl")
+    alone = []
+    for o in oceans:
+        with ColumnPhysics.from_ocean(o, config=cfg) as cp:
+            cp.set_precision(mode)
+            cp.step(DT, 6)
+            alone.append({f: cp.get_field(f) for f in fields})
+    with ColumnEnsemble.from_oceans(oceans, config=cfg) as ens:
+        ens.set_precision(mode)
+        ens.step(DT, 6)
+        for m in range(len(oceans)):
+            for f in fields:
+                assert np.array_equal(ens.member_get_field(m, f), alone[m][f]), (m, f)
+
+
+def test_learned_closure_error_paths(gpu_lib, tmp_path):
+    from roundabout_amd.rdb import CLOSURE_SURROGATE
+    print("[case] learned-closure error paths", flush=True)
+    o = synth.small(90)
+    with ColumnPhysics.from_ocean(o) as cp:
+        with pytest.raises(RdbError):
+            cp.mlp_train(epochs=1)  # nothing collected
+        with pytest.raises(RdbError):
+            cp.set_closure(CLOSURE_SURROGATE)  # no network
+        with pytest.raises(RdbError):
+            cp.mlp_load(tmp_path / "missing.bin")
+        cp.mlp_collect(DT, 1)
+        with pytest.raises(RdbError):
+            cp.mlp_train(epochs=1, batch=1 << 21)  # batch above the limit
+        with pytest.raises(RdbError):
+            cp.mlp_train(epochs=1, batch=1 << 16)  # more than the collected columns: no full batch
+        tl, vl = cp.mlp_train(epochs=1, batch=512)
+        assert len(tl) == 1 and np.isfinite(tl[0]) and np.isfinite(vl[0])
+        cp.mlp_save(tmp_path / "small.bin")
+    # a network trained for nz = 20 does not load into a 75-level grid
+    with ColumnPhysics.from_ocean(synth.tile(91, nx=16, ny=12)) as cp:
+        with pytest.raises(RdbError):
+            cp.mlp_load(tmp_path / "small.bin")
