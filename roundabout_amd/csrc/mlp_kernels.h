@@ -34,8 +34,11 @@ __global__ void k_gemm_bf16(const bf16* A, int lda, const bf16* B, int ldb, int 
                             float* Cf, int ldcf, bf16* Cb, int ldcb, int kc, long long split_stride);
 __global__ void k_transpose_bf16(const bf16* in, int ldi, bf16* out, int ldo, int R, int C);
 __global__ void k_gelu_backward(const float* g, const float* pre, bf16* d, long long n);
+constexpr int MSE_BLOCKS = 512;  // workgroups of k_mse (grid-stride), one loss partial each
+constexpr int BP_PHASES = 16;    // row phases of k_bias_partial
 __global__ void k_mse(const float* Y, const float* T, const float* W, int M, int Mvalid, int N, int ld,
-                      float inv_count, bf16* dY, float* loss);
+                      float inv_count, bf16* dY, float* part);
+__global__ void k_loss_final(const float* part, int n, float inv_count, float* loss);
 __global__ void k_bias_partial(const bf16* d, int M, int N, int ld, int rows, float* part);
 __global__ void k_reduce_splits(const float* part, int S, long long n, long long stride, float* out);
 __global__ void k_adam(float* p, const float* g, float* m1, float* m2, long long n, float lr, float b1, float b2,

@@ -353,18 +353,23 @@ __global__ void k_gelu_backward(const float* __restrict__ g, const float* __rest
 
 // Masked mean-squared error of the output layer and its gradient (bf16).
 // Y, T, W (weights 0/1) are M x N (pitch ld); rows >= Mvalid contribute 0.
+// Masked mean-squared error of the output layer and its gradient (bf16):
+// grid-stride over the M x N elements (pitch ld; rows >= Mvalid and
+// columns >= N give 0); each workgroup writes its partial loss sum to
+// part[blockIdx.x], which k_loss_final adds up in a fixed order (no
+// atomics: the loss is reproducible).
 __global__ void k_mse(const float* __restrict__ Y, const float* __restrict__ T, const float* __restrict__ W, int M,
-                      int Mvalid, int N, int ld, float inv_count, bf16* __restrict__ dY, float* __restrict__ loss) {
+                      int Mvalid, int N, int ld, float inv_count, bf16* __restrict__ dY, float* __restrict__ part) {
   __shared__ float red[256];
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long total = (long long)M * ld, stride = (long long)gridDim.x * blockDim.x;
   float l = 0.f;
-  if (i < (long long)M * ld) {
-    int m = (int)(i / ld), n = (int)(i % ld);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    const int m = (int)(i / ld), n = (int)(i % ld);
     float g = 0.f;
     if (m < Mvalid && n < N) {
-      float w = W[(size_t)m * ld + n];
-      float e = Y[(size_t)m * ld + n] - T[(size_t)m * ld + n];
-      l = w * e * e;
+      const float w = W[i];
+      const float e = Y[i] - T[i];
+      l += w * e * e;
       g = 2.f * w * e * inv_count;
     }
     dY[i] = f2bf(g);
@@ -375,24 +380,41 @@ __global__ void k_mse(const float* __restrict__ Y, const float* __restrict__ T, 
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicAdd(loss, red[0] * inv_count);
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// *loss += inv_count * sum of part[0..n) (one workgroup of 256, fixed order)
+__global__ void k_loss_final(const float* __restrict__ part, int n, float inv_count, float* __restrict__ loss) {
+  __shared__ float red[256];
+  float a = 0.f;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) a += part[i];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *loss += red[0] * inv_count;
 }
 
 // Partial column sums of d (bf16, M x N, pitch ld) over the row chunk
 // [blockIdx.y * rows, (blockIdx.y + 1) * rows): part[blockIdx.y * N + n].
-// Block (64, 4): 64 columns, 4 row phases combined in a fixed order.
+// Block (64, BP_PHASES): 64 columns, BP_PHASES row phases combined in a
+// fixed order.
 __global__ void k_bias_partial(const bf16* __restrict__ d, int M, int N, int ld, int rows, float* __restrict__ part) {
-  __shared__ float red[4][64];
+  __shared__ float red[BP_PHASES][64];
   const int n = blockIdx.x * 64 + threadIdx.x;
   const int r0 = blockIdx.y * rows, r1 = min(M, r0 + rows);
   float s = 0.f;
   if (n < N)
-    for (int m = r0 + threadIdx.y; m < r1; m += 4) s += bf2f(d[(size_t)m * ld + n]);
+    for (int m = r0 + threadIdx.y; m < r1; m += BP_PHASES) s += bf2f(d[(size_t)m * ld + n]);
   red[threadIdx.y][threadIdx.x] = s;
   __syncthreads();
-  if (threadIdx.y == 0 && n < N)
-    part[(size_t)blockIdx.y * N + n] = (red[0][threadIdx.x] + red[1][threadIdx.x]) +
-                                       (red[2][threadIdx.x] + red[3][threadIdx.x]);
+  for (int w = BP_PHASES / 2; w > 0; w >>= 1) {
+    if (threadIdx.y < w) red[threadIdx.y][threadIdx.x] += red[threadIdx.y + w][threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.y == 0 && n < N) part[(size_t)blockIdx.y * N + n] = red[0][threadIdx.x];
 }
 
 // out[i] = sum over s < S of part[s * stride + i], in order of s

@@ -46,6 +46,7 @@ struct Net {
   bf16* dpre = nullptr;
   bf16* dpreT = nullptr;
   float* dloss = nullptr;
+  float* lpart = nullptr;  // per-workgroup partial losses of k_mse
   float* ws = nullptr;  // split-K partials of the weight and bias gradients
   hipStream_t stream = nullptr;
   std::string err;

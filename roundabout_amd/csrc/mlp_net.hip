@@ -89,6 +89,7 @@ int Net::init(const std::vector<int>& dims, hipStream_t s) {
   NT_TRY(hipMalloc((void**)&Wb, 2 * nwb));
   NT_TRY(hipMalloc((void**)&WTb, 2 * nwb));
   NT_TRY(hipMalloc((void**)&dloss, 64));
+  NT_TRY(hipMalloc((void**)&lpart, sizeof(float) * MSE_BLOCKS));
   NT_TRY(hipMemsetAsync(p, 0, 4 * np, stream));
   NT_TRY(hipMemsetAsync(g, 0, 4 * np, stream));
   NT_TRY(hipMemsetAsync(m1, 0, 4 * np, stream));
@@ -184,7 +185,8 @@ void Net::release() {
   F(Wb);
   F(WTb);
   F(dloss);
-  p = g = m1 = m2 = dloss = nullptr;
+  F(lpart);
+  p = g = m1 = m2 = dloss = lpart = nullptr;
   Wb = WTb = nullptr;
   nl = 0;
 }
@@ -289,8 +291,10 @@ int Net::backward(int Mpad, int Mvalid, double count, float* loss_out, float* lo
   float* lacc = loss_dev ? loss_dev : dloss;
   if (!loss_dev) NT_TRY(hipMemsetAsync(dloss, 0, 4, stream));
   const float inv = count > 0 ? (float)(1.0 / count) : 0.f;
-  hipLaunchKernelGGL(k_mse, g1((long long)Mpad * po, 256), dim3(256), 0, stream, Y, T, Wm, Mpad, Mvalid, dout[L], po,
-                     inv, dY, lacc);
+  const int nb = (int)std::min<long long>(MSE_BLOCKS, ((long long)Mpad * po + 255) / 256);
+  hipLaunchKernelGGL(k_mse, dim3((unsigned)nb), dim3(256), 0, stream, Y, T, Wm, Mpad, Mvalid, dout[L], po, inv, dY,
+                     lpart);
+  hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(256), 0, stream, lpart, nb, inv, lacc);
   int S, kc;
   batch_splits(Mpad, &S, &kc);
   // d (bf16, Mpad x pout[l]) is the gradient w.r.t. layer l's output (pre-activation)
@@ -308,7 +312,7 @@ int Net::backward(int Mpad, int Mvalid, double count, float* loss_out, float* lo
     gemm<EPI_F32>(stream, dYT, Mpad, xinT, Mpad, o, i, Mpad, nullptr, ws, i, nullptr, 0, S, kc, oi);
     hipLaunchKernelGGL(k_reduce_splits, g1(oi, 256), dim3(256), 0, stream, ws, S, oi, oi, g + offW[l]);
     float* bpart = ws + (size_t)S * (size_t)oi;
-    hipLaunchKernelGGL(k_bias_partial, dim3((unsigned)(o / 64), (unsigned)S), dim3(64, 4), 0, stream, d, Mpad, o, o,
+    hipLaunchKernelGGL(k_bias_partial, dim3((unsigned)(o / 64), (unsigned)S), dim3(64, BP_PHASES), 0, stream, d, Mpad, o, o,
                        kc, bpart);
     hipLaunchKernelGGL(k_reduce_splits, g1(o, 256), dim3(256), 0, stream, bpart, S, (long long)o, (long long)o,
                        g + offB[l]);
