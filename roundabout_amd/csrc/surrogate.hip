@@ -105,18 +105,29 @@ struct LevelVals {
 // (the loads and the EOS do not depend on liveness, so that a walk over
 // levels has no branches and keeps several levels' loads in flight; a
 // vanished level's values are never used)
-__device__ __forceinline__ LevelVals load_level(const StateView& s, int p, int k, double rho0, double p_ref) {
+// (split in the memory part, load_raw, and the arithmetic, finish_level,
+// so that a walk can have the next level's loads in flight while it
+// computes the current one)
+__device__ __forceinline__ LevelVals load_raw(const StateView& s, int p, int k) {
   LevelVals v{1.0e-4, 0.0, 0.0, 0.0, 0.0, 0.0, false};
   if (k < 0 || k >= s.nz) return v;
   const int P = s.P;
   v.h = LV(s.h, k);
-  v.live = v.h > H_VANISHED;
+  v.live = true;  // "exists"; finish_level turns it into "live"
   v.T = LV(s.T, k);
   v.S = LV(s.S, k);
   v.U = LV(s.ub, k);
   v.V = LV(s.vb, k);
+  return v;
+}
+__device__ __forceinline__ LevelVals finish_level(LevelVals v, double rho0, double p_ref) {
+  if (!v.live) return v;  // a level this grid does not have
+  v.live = v.h > H_VANISHED;
   v.rho = eos_wright<double>(v.T, v.S, v.h, rho0, p_ref);
   return v;
+}
+__device__ __forceinline__ LevelVals load_level(const StateView& s, int p, int k, double rho0, double p_ref) {
+  return finish_level(load_raw(s, p, k), rho0, p_ref);
 }
 
 // sign(x) log10(1 + |x| / x0), in single precision: the argument is formed
@@ -156,15 +167,13 @@ __device__ __forceinline__ float forcing_feature(const StateView& s, const doubl
   }
 }
 
-__device__ __forceinline__ double live_depth(const StateView& s, int p) {
-  const int P = s.P;
-  double d = 0.0;
-  for (int k = s.kb[p]; k < s.nz; ++k) {
-    double hk = LV(s.h, k);
-    if (hk > H_VANISHED) d += hk;
-  }
-  return d;
-}
+
+// The network's levels are walked in FW groups of per = ceil(nzn / FW)
+// (one wavefront each in k_features_infer); the live column depth is the
+// sum, over the groups in order, of each group's live thicknesses summed
+// from its top level down -- the same order in both feature kernels, so
+// that training and inference features agree bit for bit.
+constexpr int FC = 64, FW = 8, FB = 64 * FW;
 
 // Raw features of packed columns c = i * stride (i < ncols) into rows row0 + i.
 // The network has nzn >= nz levels with the surfaces aligned: network level
@@ -176,14 +185,19 @@ __global__ void k_features_raw(StateView s, const double* cor, int ncols, int st
   int p = i * stride;
   float* row = X + (size_t)(row0 + i) * (size_t)din;
   LevelVals above = load_level(s, p, nzn - lev_off, rho0, p_ref);  // above the top: none
+  const int per = (nzn + FW - 1) / FW;
+  double dpart[FW];
+  for (int w = 0; w < FW; ++w) dpart[w] = 0.0;
   for (int j = nzn - 1; j >= 0; --j) {
     const LevelVals cur = load_level(s, p, j - lev_off, rho0, p_ref);
     float o[5];
     level_slots(cur, above, rho0, o);
     for (int b = 0; b < 5; ++b) row[b * nzn + j] = o[b];
+    if (cur.live) dpart[j / per] += cur.h;
     above = cur;
   }
-  const double depth = live_depth(s, p);
+  double depth = 0.0;
+  for (int w = 0; w < FW; ++w) depth += dpart[w];
   for (int f = 5 * nzn; f < din; ++f) row[f] = forcing_feature(s, cor, p, f - 5 * nzn, depth);
 }
 
@@ -314,13 +328,13 @@ __global__ void k_gather(const bf16* Xn, int ldx, const float* Tn, const float* 
 // in LDS.  Every value is the
 // same arithmetic as feature(), so training and inference features agree
 // bit for bit.
-constexpr int FC = 64, FW = 8, FB = 64 * FW;
 __global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double* cor, int n, int din, int ld,
                                                      const float* st, bf16* X0, unsigned char* guard,
                                                      unsigned int* nfall, int guard_mode, double rho0, double p_ref,
                                                      int nzn, int lev_off) {
   __shared__ bf16 tile[384][FC + 1];
   __shared__ unsigned char flags[FW][FC];
+  __shared__ double sdepth[FW][FC];
   const float* mean = st;
   const float* stdv = st + din;
   const float* mn = st + 2 * din;
@@ -338,8 +352,16 @@ __global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double
     const int j0 = part * per, j1 = min(nzn, j0 + per);
     LevelVals above = load_level(s, p, j1 - lev_off, rho0, p_ref);  // the level above this part's highest
     bool unstable = false;
+    double dpart = 0.0;  // this group's live thickness, summed from its top down
+    // the next two levels' loads stay in flight while a level is computed
+    LevelVals nxt = load_raw(s, p, j1 - 1 - lev_off);
+    LevelVals nxt2 = load_raw(s, p, j1 - 2 - lev_off);
     for (int j = j1 - 1; j >= j0; --j) {
-      const LevelVals cur = load_level(s, p, j - lev_off, rho0, p_ref);
+      const LevelVals raw = nxt;
+      nxt = nxt2;
+      if (j - 2 >= j0) nxt2 = load_raw(s, p, j - 2 - lev_off);
+      __builtin_amdgcn_sched_barrier(0);
+      const LevelVals cur = finish_level(raw, rho0, p_ref);
       float o[5];
       level_slots(cur, above, rho0, o);
 #pragma unroll
@@ -347,13 +369,10 @@ __global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double
       // static stability of the pair (this level, the one above): N^2 < 0
       // where the upper of two live layers is denser (A3's sign)
       if (guard_mode == RDB_COLPHYS_GUARD_STRICT && cur.live && above.live && above.rho > cur.rho) unstable = true;
+      if (cur.live) dpart += cur.h;
       above = cur;
     }
-    if (part == 0) {
-      const double depth = live_depth(s, p);
-      for (int f = 5 * nzn; f < din; ++f) put(f, forcing_feature(s, cor, p, f - 5 * nzn, depth));
-      for (int f = din; f < ld; ++f) tile[f][c] = 0;
-    }
+    sdepth[part][c] = dpart;
     if (part == FW - 1 && guard_mode == RDB_COLPHYS_GUARD_SURFACE) {
       const int kb = s.kb[p];
       int k_sfc = nz - 1;
@@ -376,11 +395,17 @@ __global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double
   flags[part][c] = fl;
   __syncthreads();
   if (part == 0 && p < n) {
-    unsigned char g = 0;
-    for (int w = 0; w < FW; ++w) g |= flags[w][c];
+    // the forcing features, with the column depth from the groups' sums
+    double depth = 0.0;
+    for (int w = 0; w < FW; ++w) depth += sdepth[w][c];
+    for (int f = 5 * nzn; f < din; ++f) put(f, forcing_feature(s, cor, p, f - 5 * nzn, depth));
+    for (int f = din; f < ld; ++f) tile[f][c] = 0;
+    unsigned char g = fl;  // this wave's flags, now with the forcing features' range checks
+    for (int w = 1; w < FW; ++w) g |= flags[w][c];
     guard[p] = g;
     if (g) atomicAdd(nfall, 1u);
   }
+  __syncthreads();  // the forcing features are in the tile
   // cooperative row stores: the FC rows of this block, 2 bf16 per lane,
   // consecutive lanes along a row
   const int rows = min(FC, n - (int)blockIdx.x * FC);
