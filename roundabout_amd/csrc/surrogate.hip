@@ -472,10 +472,16 @@ int surr_collect(rdb_colphys* h, double dt, int n_steps, int stride, long long* 
     // grow (copying what is already collected)
     const long long cap = std::max(need, S->cap * 2);
     float *X = nullptr, *T = nullptr, *W = nullptr, *R = nullptr;
-    SG_TRY(hipMalloc((void**)&X, 4 * (size_t)cap * S->din));
-    SG_TRY(hipMalloc((void**)&T, 4 * (size_t)cap * S->dout));
-    SG_TRY(hipMalloc((void**)&W, 4 * (size_t)cap * S->dout));
-    SG_TRY(hipMalloc((void**)&R, 4 * (size_t)cap));
+    hipError_t ae = hipMalloc((void**)&X, 4 * (size_t)cap * S->din);
+    if (ae == hipSuccess) ae = hipMalloc((void**)&T, 4 * (size_t)cap * S->dout);
+    if (ae == hipSuccess) ae = hipMalloc((void**)&W, 4 * (size_t)cap * S->dout);
+    if (ae == hipSuccess) ae = hipMalloc((void**)&R, 4 * (size_t)cap);
+    if (ae != hipSuccess) {
+      for (float* q : {X, T, W, R})
+        if (q) (void)hipFree(q);
+      return perr(RDB_COLPHYS_ERR_SETUP, "mlp_collect: no device memory for %lld training columns: %s", cap,
+                  hipGetErrorString(ae));
+    }
     if (S->n) {
       SG_TRY(hipMemcpyAsync(X, S->Xraw, 4 * (size_t)S->n * S->din, hipMemcpyDeviceToDevice, h->stream));
       SG_TRY(hipMemcpyAsync(T, S->Traw, 4 * (size_t)S->n * S->dout, hipMemcpyDeviceToDevice, h->stream));

@@ -3,7 +3,9 @@
 !>                                 ring, refused knobs, NULL-safe destroy
 !>   colphys_fortran_check gpu  -- a 16 x 12 x 8 ocean stepped 3 times; prints
 !>                                 the outputs' checksums and a parity line
-!>                                 against the staged entry points
+!>                                 against the staged entry points, and a
+!>                                 two-member ensemble whose members equal
+!>                                 the single-grid step bitwise
 program colphys_fortran_check
    use, intrinsic :: iso_c_binding
    use rdb_colphys
@@ -13,11 +15,14 @@ program colphys_fortran_check
    integer(c_int) :: st
    character(len=8) :: mode
    integer, parameter :: nx = 16, ny = 12, nz = 8
-   real(c_double) :: wet(nx, ny), hl(nx, ny, nz), T(nx, ny, nz), S(nx, ny, nz)
+   real(c_double), target :: wet(nx, ny)
+   real(c_double) :: hl(nx, ny, nz), T(nx, ny, nz), S(nx, ny, nz)
    real(c_double) :: u(nx + 1, ny, nz), v(nx, ny + 1, nz), tx(nx + 1, ny, 1), ty(nx, ny + 1, 1)
    real(c_double) :: qh(nx, ny, 1), qs(nx, ny, 1), T1(nx, ny, nz), T2(nx, ny, nz), kt1(nx, ny, nz + 1)
    real(c_double) :: kt2(nx, ny, nz + 1), heat, salt
-   integer :: i, j, k
+   integer :: i, j, k, m
+   integer(c_int) :: n_mem, enx(2), eny(2), enz(2)
+   type(c_ptr) :: ewet(2)
 
    call get_command_argument(1, mode)
    st = rdb_colphys_config_default(cfg)
@@ -101,6 +106,36 @@ program colphys_fortran_check
       print '(a)', "fortran-abi-gpu: fused and staged paths differ"
       stop 13
    end if
+
+   ! the same ocean twice as a two-member ensemble, one launch per step
+   enx = nx
+   eny = ny
+   enz = nz
+   ewet = [c_loc(wet), c_loc(wet)]
+   st = rdb_colphys_ensemble_create(cfg, 2, enx, eny, enz, 0, ewet, c_null_ptr, h)
+   if (st /= 0) then
+      print '(a)', trim(colphys_last_error())
+      stop 14
+   end if
+   st = rdb_colphys_member_count(h, n_mem)
+   if (st /= 0 .or. n_mem /= 2) stop 15
+   if (colphys_set(h, "temperature", T) /= RDB_COLPHYS_ERR_BAD_HANDLE) stop 16
+   do m = 0, 1
+      call upload_member(m)
+   end do
+   st = rdb_colphys_step(h, 3600.0_c_double, 3)
+   if (st /= 0) stop 17
+   do m = 0, 1
+      T2 = 0.0_c_double
+      kt2 = 0.0_c_double
+      if (colphys_member_get(h, m, "temperature", T2) /= 0) stop 18
+      if (colphys_member_get(h, m, "kt", kt2) /= 0) stop 18
+      if (any(T1 /= T2) .or. any(kt1 /= kt2)) then
+         print '(a, i0, a)', "fortran-abi-gpu: ensemble member ", m, " differs from the single grid"
+         stop 19
+      end if
+   end do
+   st = rdb_colphys_destroy(h)
    print '(a, es24.16, a, es24.16)', "fortran-abi-gpu: ok heat=", heat, " sum(T)=", sum(T1)
 
 contains
@@ -118,5 +153,18 @@ contains
       if (colphys_set(h, "Q_heat", qh) /= 0) stop 27
       if (colphys_set(h, "Q_salt", qs) /= 0) stop 28
    end subroutine upload
+
+   subroutine upload_member(mm)
+      integer, intent(in) :: mm
+      if (colphys_member_set(h, mm, "h_layer", hl) /= 0) stop 30
+      if (colphys_member_set(h, mm, "temperature", T) /= 0) stop 31
+      if (colphys_member_set(h, mm, "salinity", S) /= 0) stop 32
+      if (colphys_member_set(h, mm, "u_face_x_layer", u) /= 0) stop 33
+      if (colphys_member_set(h, mm, "v_face_y_layer", v) /= 0) stop 34
+      if (colphys_member_set(h, mm, "tau_x", tx) /= 0) stop 35
+      if (colphys_member_set(h, mm, "tau_y", ty) /= 0) stop 36
+      if (colphys_member_set(h, mm, "Q_heat", qh) /= 0) stop 37
+      if (colphys_member_set(h, mm, "Q_salt", qs) /= 0) stop 38
+   end subroutine upload_member
 
 end program colphys_fortran_check

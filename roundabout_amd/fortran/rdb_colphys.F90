@@ -55,6 +55,9 @@ module rdb_colphys
       real(c_double) :: val_fraction
    end type rdb_colphys_mlp_config_t
 
+   public :: rdb_colphys_ensemble_create, rdb_colphys_member_count
+   public :: rdb_colphys_member_set_field, rdb_colphys_member_get_field
+   public :: colphys_member_set, colphys_member_get
    public :: rdb_colphys_mlp_config_default, rdb_colphys_mlp_collect, rdb_colphys_mlp_train
    public :: rdb_colphys_mlp_save, rdb_colphys_mlp_load, rdb_colphys_set_guard
    public :: rdb_colphys_config_default, rdb_colphys_working_precision
@@ -159,6 +162,40 @@ module rdb_colphys
          type(c_ptr), value :: handle
          real(c_double), intent(out) :: heat, salt
       end function
+      !> wet(m) and sort_key(m): C pointers to member m's masks (c_loc of the member arrays)
+      integer(c_int) function rdb_colphys_ensemble_create(cfg, n_members, nx, ny, nz, n_passive, wet, sort_key, &
+                                                          handle) bind(c, name="rdb_colphys_ensemble_create")
+         import :: c_int, c_ptr, rdb_colphys_config_t
+         type(rdb_colphys_config_t), intent(in) :: cfg
+         integer(c_int), value :: n_members, n_passive
+         integer(c_int), intent(in) :: nx(*), ny(*), nz(*)
+         type(c_ptr), intent(in) :: wet(*)
+         type(c_ptr), value :: sort_key
+         type(c_ptr), intent(out) :: handle
+      end function
+      integer(c_int) function rdb_colphys_member_count(handle, n) bind(c, name="rdb_colphys_member_count")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int), intent(out) :: n
+      end function
+      integer(c_int) function rdb_colphys_member_set_field(handle, member, name, data, n1, n2, n3) &
+         bind(c, name="rdb_colphys_member_set_field")
+         import :: c_int, c_double, c_char, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int), value :: member
+         character(kind=c_char), intent(in) :: name(*)
+         real(c_double), intent(in) :: data(*)
+         integer(c_int), value :: n1, n2, n3
+      end function
+      integer(c_int) function rdb_colphys_member_get_field(handle, member, name, out, n1, n2, n3) &
+         bind(c, name="rdb_colphys_member_get_field")
+         import :: c_int, c_double, c_char, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int), value :: member
+         character(kind=c_char), intent(in) :: name(*)
+         real(c_double), intent(out) :: out(*)
+         integer(c_int), value :: n1, n2, n3
+      end function
       integer(c_int) function rdb_colphys_mlp_config_default(cfg) bind(c, name="rdb_colphys_mlp_config_default")
          import :: c_int, rdb_colphys_mlp_config_t
          type(rdb_colphys_mlp_config_t), intent(out) :: cfg
@@ -222,6 +259,28 @@ contains
       st = rdb_colphys_get_field(handle, trim(name)//c_null_char, out, &
                                  int(size(out, 1), c_int), int(size(out, 2), c_int), int(size(out, 3), c_int))
    end function colphys_get
+
+   !> Upload a 3-D field (n1, n2, n3) of one ensemble member (0-based) by name.
+   integer(c_int) function colphys_member_set(handle, member, name, data) result(st)
+      type(c_ptr), intent(in) :: handle
+      integer, intent(in) :: member
+      character(len=*), intent(in) :: name
+      real(c_double), intent(in) :: data(:, :, :)
+      st = rdb_colphys_member_set_field(handle, int(member, c_int), trim(name)//c_null_char, data, &
+                                        int(size(data, 1), c_int), int(size(data, 2), c_int), &
+                                        int(size(data, 3), c_int))
+   end function colphys_member_set
+
+   !> Download a 3-D field (n1, n2, n3) of one ensemble member (0-based) by name.
+   integer(c_int) function colphys_member_get(handle, member, name, out) result(st)
+      type(c_ptr), intent(in) :: handle
+      integer, intent(in) :: member
+      character(len=*), intent(in) :: name
+      real(c_double), intent(out) :: out(:, :, :)
+      st = rdb_colphys_member_get_field(handle, int(member, c_int), trim(name)//c_null_char, out, &
+                                        int(size(out, 1), c_int), int(size(out, 2), c_int), &
+                                        int(size(out, 3), c_int))
+   end function colphys_member_get
 
    !> Most recent error-ring message as a Fortran string.
    function colphys_last_error() result(msg)
