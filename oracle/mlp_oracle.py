@@ -86,12 +86,14 @@ def loss_grad(widths, params, x, t, w, bf16=False):
     count = float(np.sum(w))
     e = y - t
     loss = float(np.sum(w * e * e) / count)
-    d = r(2.0 * w * e / count)
+    # the output gradient 2 w e travels back unnormalised (as stored in
+    # BF16); the 1 / count of the mean scales the summed parameter gradient
+    d = r(2.0 * w * e)
     grads = [None] * len(layers)
     for li in range(len(layers) - 1, -1, -1):
         W, _ = layers[li]
-        gW = d.T @ acts[li]
-        gb = d.sum(axis=0)
+        gW = (d.T @ acts[li]) / count
+        gb = d.sum(axis=0) / count
         grads[li] = (gW, gb)
         if li > 0:
             dx = d @ r(W)

@@ -61,7 +61,7 @@ int rdb_mlp_eval(const int* widths, int n_widths, const float* params, int batch
     const int Mpad = mlp::round_up(batch, 128);  // the fused inference kernel's row block
     if (!st && net.ensure_workspace(Mpad, false)) st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, net.err.c_str());
     if (!st) st = upload_x(net, batch, Mpad, x);
-    if (!st && net.forward(Mpad, false)) st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, net.err.c_str());
+    if (!st && net.forward(Mpad)) st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, net.err.c_str());
     if (!st) {
       const int out = net.dout[net.nl - 1], po = net.pout[net.nl - 1];
       std::vector<float> yp((size_t)Mpad * po);
@@ -97,19 +97,22 @@ int rdb_mlp_loss_grad(const int* widths, int n_widths, const float* params, int 
     if (!st) {
       const int out = net.dout[net.nl - 1], po = net.pout[net.nl - 1];
       std::vector<float> tp((size_t)Mpad * po, 0.f), wp((size_t)Mpad * po, 0.f);
-      double count = 0.0;
       for (int m = 0; m < batch; ++m)
         for (int c = 0; c < out; ++c) {
           tp[(size_t)m * po + c] = t[(size_t)m * out + c];
           wp[(size_t)m * po + c] = w[(size_t)m * out + c];
-          count += w[(size_t)m * out + c];
         }
       if (hipMemcpyAsync(net.T, tp.data(), 4 * tp.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
           hipMemcpyAsync(net.Wm, wp.data(), 4 * wp.size(), hipMemcpyHostToDevice, s) != hipSuccess)
         st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, "mlp_loss_grad: target upload failed");
-      if (!st && net.forward(Mpad, true)) st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, net.err.c_str());
-      if (!st && net.backward(Mpad, batch, count, loss))
+      if (!st && hipMemsetAsync(net.dloss, 0, 4, s) != hipSuccess)
+        st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, "mlp_loss_grad: loss clear failed");
+      if (!st && (net.transpose_input(Mpad) || net.train_forward(Mpad, batch) || net.backward(Mpad) ||
+                  net.update(false, 0.f, 0.f, 0.f, 0.f, net.dloss, nullptr)))
         st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, net.err.c_str());
+      if (!st && (hipMemcpyAsync(loss, net.dloss, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                  hipStreamSynchronize(s) != hipSuccess))
+        st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, "mlp_loss_grad: loss download failed");
       if (!st && net.get_grads(grad)) st = rdb_colphys_push_error(RDB_COLPHYS_ERR_SETUP, net.err.c_str());
     }
     (void)hipStreamSynchronize(s);

@@ -1,6 +1,7 @@
 // mlp_kernels.hip -- BF16 MFMA kernels of the learned vertical-mixing closure:
-// GEMM with fused epilogues, GELU backward, bias gradients, masked MSE loss,
-// Adam, transposes and weight refresh.
+// GEMM with fused epilogues (bias, GELU, GELU backward with bias-gradient
+// partials, transposed copies), masked MSE loss, the fused gradient
+// reduction + Adam + weight refresh, and the fused inference kernel.
 //
 // The GEMM computes C[m][n] = sum_k A[m][k] * B[n][k] with both operands
 // row-major and k contiguous (A: M x K, B: N x K), FP32 accumulation on
@@ -10,6 +11,8 @@
 // for j = 0..3.  One wavefront owns a 64 x 64 tile of C (4 x 4 MFMA tiles);
 // the forward pass Y = X W^T, the input gradient dX = dY W and the weight
 // gradient dW = dY^T X are all this product with the right transposed copy.
+// The transposed copies the weight gradient needs (activations and output
+// gradients, feature-major) are written by the epilogues that produce them.
 #include "mlp_kernels.h"
 
 namespace mlp {
@@ -47,112 +50,308 @@ __device__ __forceinline__ float gelu_grad(float x) {
   return sg + 2.0f * x * sg * (1.0f - sg) * up;
 }
 
-template <int EPI>
-__global__ void __launch_bounds__(64) k_gemm_bf16(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B,
-                                                  int ldb, int M, int N, int K, const float* __restrict__ bias,
-                                                  float* __restrict__ Cf, int ldcf, bf16* __restrict__ Cb, int ldcb,
-                                                  int kc, long long split_stride) {
-  const int lane = threadIdx.x;
-  // n-tiles fastest, so that the workgroups sharing a row block of A run
-  // together and find it in L2
-  const int nt = N / 64;
-  const int m0 = (int)(blockIdx.x / nt) * 64, n0 = (int)(blockIdx.x % nt) * 64;
-  if (m0 >= M || n0 >= N) return;
-  // split-K: this workgroup sums k in [kbeg, kend) into its own partial
-  const int kbeg = blockIdx.y * kc;
-  const int kend = min(K, kbeg + kc);
-  A += kbeg;
-  B += kbeg;
-  K = max(kend - kbeg, 0);
-  if (EPI == EPI_F32) Cf += (size_t)blockIdx.y * (size_t)split_stride;
-  const int r = lane & 15, q = lane >> 4;
-  f32x4 acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const bf16* Ap[4];
-  const bf16* Bp[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    Ap[i] = A + (size_t)(m0 + 16 * i + r) * (size_t)lda + q * 8;
-    Bp[i] = B + (size_t)(n0 + 16 * i + r) * (size_t)ldb + q * 8;
-  }
-  bf16x8 a_cur[4], b_cur[4];
-  if (K > 0) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      a_cur[i] = *(const bf16x8*)(Ap[i]);
-      b_cur[i] = *(const bf16x8*)(Bp[i]);
-    }
-  }
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    bf16x8 a_nxt[4], b_nxt[4];
-    const bool more = (k0 + 32 < K);
-    if (more) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        a_nxt[i] = *(const bf16x8*)(Ap[i] + k0 + 32);
-        b_nxt[i] = *(const bf16x8*)(Bp[i] + k0 + 32);
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_cur[i], b_cur[j], acc[i][j], 0, 0, 0);
-    if (more) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        a_cur[i] = a_nxt[i];
-        b_cur[i] = b_nxt[i];
-      }
-    }
-  }
-  // epilogue: tile (i, j) element (row 16 i + 4 q + e, col 16 j + r)
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int n = n0 + 16 * j + r;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int m = m0 + 16 * i + 4 * q + e;
-        float v = acc[i][j][e];
-        if (EPI == EPI_F32) {
-          Cf[(size_t)m * ldcf + n] = v;
-        } else if (EPI == EPI_BIAS_F32) {
-          Cf[(size_t)m * ldcf + n] = v + bias[n];
-        } else if (EPI == EPI_BIAS_F32_T) {  // output-major: Cf[n][m]
-          Cf[(size_t)n * ldcf + m] = v + bias[n];
-        } else {  // EPI_BIAS_GELU: bf16 activation, optional fp32 pre-activation
-          float pre = v + bias[n];
-          if (Cf) Cf[(size_t)m * ldcf + n] = pre;
-          Cb[(size_t)m * ldcb + n] = f2bf(gelu(pre));
-        }
-      }
-    }
-}
-
-template __global__ void k_gemm_bf16<EPI_F32>(const bf16*, int, const bf16*, int, int, int, int, const float*,
-                                             float*, int, bf16*, int, int, long long);
-template __global__ void k_gemm_bf16<EPI_BIAS_F32>(const bf16*, int, const bf16*, int, int, int, int, const float*,
-                                                  float*, int, bf16*, int, int, long long);
-template __global__ void k_gemm_bf16<EPI_BIAS_F32_T>(const bf16*, int, const bf16*, int, int, int, int, const float*,
-                                                    float*, int, bf16*, int, int, long long);
-template __global__ void k_gemm_bf16<EPI_BIAS_GELU>(const bf16*, int, const bf16*, int, int, int, int, const float*,
-                                                   float*, int, bf16*, int, int, long long);
-
-// s_waitcnt vmcnt(n) (n = 0, 1, 2, 3) without waiting on the LDS or export
-// counters; the gfx9 immediate holds vmcnt in bits 3:0, expcnt in 6:4 and
-// lgkmcnt in 11:8.
+// s_waitcnt vmcnt(n) without waiting on the LDS or export counters; the gfx9
+// immediate holds vmcnt[3:0] in bits 3:0, expcnt in 6:4, lgkmcnt in 11:8
+// and vmcnt[5:4] in bits 15:14.
 __device__ __forceinline__ void wait_vm_except(int n) {
   switch (n) {
     case 0: __builtin_amdgcn_s_waitcnt(0xF70); break;
     case 1: __builtin_amdgcn_s_waitcnt(0xF71); break;
     case 2: __builtin_amdgcn_s_waitcnt(0xF72); break;
-    default: __builtin_amdgcn_s_waitcnt(0xF73); break;
+    case 3: __builtin_amdgcn_s_waitcnt(0xF73); break;
+    default: __builtin_amdgcn_s_waitcnt(0xF74); break;  // 4
   }
 }
+
+// Four 16-byte LDS reads (MFMA fragments) and the wait for them, as one asm
+// statement.  Written as plain loads, the compiler's wait analysis could not
+// tell these reads from the direct loads in flight into the other stages
+// and waited for every outstanding load (a full memory round trip) before
+// most k-steps; here the reads are ordered by the explicit vmcnt waits and
+// barriers of the k-loop, and their own completion by lgkmcnt(0).
+__device__ __forceinline__ void lds_read4(const bf16* p0, const bf16* p1, const bf16* p2, const bf16* p3,
+                                          bf16x8& x0, bf16x8& x1, bf16x8& x2, bf16x8& x3) {
+  typedef __attribute__((ext_vector_type(4))) unsigned u32x4;
+  auto lds = [](const bf16* p) {
+    return (unsigned)(uintptr_t)(const __attribute__((address_space(3))) bf16*)p;
+  };
+  u32x4 v0, v1, v2, v3;
+  asm volatile(
+      "ds_read_b128 %0, %4\n\t"
+      "ds_read_b128 %1, %5\n\t"
+      "ds_read_b128 %2, %6\n\t"
+      "ds_read_b128 %3, %7\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3)
+      : "v"(lds(p0)), "v"(lds(p1)), "v"(lds(p2)), "v"(lds(p3))
+      : "memory");
+  x0 = __builtin_bit_cast(bf16x8, v0);
+  x1 = __builtin_bit_cast(bf16x8, v1);
+  x2 = __builtin_bit_cast(bf16x8, v2);
+  x3 = __builtin_bit_cast(bf16x8, v3);
+}
+
+// Eight 2-byte LDS reads and the wait for them, as one asm statement (for
+// the same reason as lds_read4); returns the values packed low half first.
+__device__ __forceinline__ uint4 lds_read8_u16(const unsigned (&ad)[8]) {
+  unsigned v[8];
+  asm volatile(
+      "ds_read_u16 %0, %8\n\t"
+      "ds_read_u16 %1, %9\n\t"
+      "ds_read_u16 %2, %10\n\t"
+      "ds_read_u16 %3, %11\n\t"
+      "ds_read_u16 %4, %12\n\t"
+      "ds_read_u16 %5, %13\n\t"
+      "ds_read_u16 %6, %14\n\t"
+      "ds_read_u16 %7, %15\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]), "=&v"(v[6]), "=&v"(v[7])
+      : "v"(ad[0]), "v"(ad[1]), "v"(ad[2]), "v"(ad[3]), "v"(ad[4]), "v"(ad[5]), "v"(ad[6]), "v"(ad[7])
+      : "memory");
+  return make_uint4(v[0] | (v[1] << 16), v[2] | (v[3] << 16), v[4] | (v[5] << 16), v[6] | (v[7] << 16));
+}
+
+// The k-loop of a workgroup's 64 x 64 tile over nsteps steps of 32, four
+// waves each owning a 32 x 32 quarter (2 x 2 MFMA tiles).  Operands stream
+// through LDS with direct global-to-LDS loads (global_load_lds_dwordx4: lane
+// l writes its 16 bytes at the instruction's LDS base + 16 l) in four
+// stages, three steps ahead of the MFMAs; a stage holds A rows 0..63 then B
+// rows 0..63, 64 bytes each, with the 16-byte chunk q of row n at slot
+// (q + n / 4) mod 4 so that the 16 rows of an MFMA operand start in 16 bank
+// groups.  Wave w loads row group w of A and of B.  The four stages are
+// separate LDS objects and the loop is unrolled by four, so that every
+// access names its stage.  (One wave per 64 x 64 tile left half the SIMDs
+// idle and serialised each wave's loads, MFMAs and epilogue; register
+// prefetching was undone by the compiler, which moved the loads next to
+// their MFMAs.)
+__device__ __forceinline__ void gemm_steps(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B, int ldb,
+                                           const unsigned* __restrict__ aidx, int anv, bf16* __restrict__ AT,
+                                           int ldat, int m0, int n0, int nsteps, int wave, int lane, bf16* st0,
+                                           bf16* st1, bf16* st2, bf16* st3, f32x4 (&acc)[2][2]) {
+  if (nsteps <= 0) return;
+  const int r = lane & 15, q = lane >> 4;
+  const int lrow = lane >> 2;
+  const int lchunk = ((lane & 3) - (lane >> 4) + 4) & 3;
+  // this lane's rows of group `wave` (A gathered through aidx if given)
+  const int ma = m0 + 16 * wave + lrow;
+  const size_t arow = aidx ? (ma < anv ? (size_t)aidx[ma] : 0) : (size_t)ma;
+  const bf16* Ag = A + arow * (size_t)lda + lchunk * 8;
+  const bf16* Bg = B + (size_t)(n0 + 16 * wave + lrow) * (size_t)ldb + lchunk * 8;
+  auto load_stage = [&](int t, bf16* buf) {
+    const int k0 = 32 * t;
+    __builtin_amdgcn_global_load_lds((const void*)(Ag + k0),
+                                     (__attribute__((address_space(3))) void*)(buf + wave * 512), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((const void*)(Bg + k0),
+                                     (__attribute__((address_space(3))) void*)(buf + 2048 + wave * 512), 16, 0, 0);
+  };
+  const int ra = 32 * (wave >> 1), rb = 32 * (wave & 1);  // this wave's quarter
+  // step t: stage t from `cur`; stage t + 3 starts into `ahead` (the stage
+  // every wave finished reading before the barrier of step t - 1).  Past
+  // the last stage the loads repeat it into the free buffer: every step
+  // issues the same loads and waits the same count.
+  const int last = nsteps - 1;
+  auto step = [&](int t, const bf16* cur, bf16* ahead) {
+    load_stage(min(t + 3, last), ahead);
+    bf16x8 a[2], b[2];
+    const bf16* pf[4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int na = ra + 16 * i + r, nb = rb + 16 * i + r;
+      pf[i] = cur + na * 32 + ((q + (na >> 2)) & 3) * 8;
+      pf[2 + i] = cur + 2048 + nb * 32 + ((q + (nb >> 2)) & 3) * 8;
+    }
+    lds_read4(pf[0], pf[1], pf[2], pf[3], a[0], a[1], b[0], b[1]);
+    if (AT) {
+      // the stage's A slice, feature-major: thread -> feature k = tid / 8 of
+      // the slice, rows 8 (tid % 8) .. + 8 of the tile (16 bytes; eight
+      // threads write a 128-byte row piece)
+      const int tid = threadIdx.x, k = tid >> 3, mr = (tid & 7) * 8;
+      unsigned ad[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int m = mr + e;
+        ad[e] = (unsigned)(uintptr_t)(const __attribute__((address_space(3))) bf16*)(
+            cur + m * 32 + (((k >> 3) + (m >> 2)) & 3) * 8 + (k & 7));
+      }
+      *(uint4*)(AT + (size_t)(32 * t + k) * (size_t)ldat + mr) = lds_read8_u16(ad);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    // this wave's share of stage t + 1 has landed (its loads of steps t + 2
+    // and t + 3, two each, may be in flight); the barrier makes every
+    // wave's share visible.  A bare s_barrier: the loop writes no LDS
+    // itself and the reads of `cur` are complete (they fed the MFMAs).
+    wait_vm_except(4);
+    __builtin_amdgcn_s_barrier();
+  };
+  load_stage(0, st0);
+  load_stage(min(1, last), st1);
+  load_stage(min(2, last), st2);
+  wait_vm_except(4);  // stage 0 has landed
+  __builtin_amdgcn_s_barrier();
+  for (int t = 0; t < nsteps; t += 4) {
+    step(t, st0, st3);
+    if (t + 1 < nsteps) step(t + 1, st1, st0);
+    if (t + 2 < nsteps) step(t + 2, st2, st1);
+    if (t + 3 < nsteps) step(t + 3, st3, st2);
+  }
+  wait_vm_except(0);  // no load into LDS outlives the loop
+}
+
+// LDS of one GEMM workgroup: the four operand stages and the cross-wave
+// column and loss sums of the epilogue
+struct GemmSmem {
+  bf16 st[4][4096];
+  float csp[2][64];
+  float lsp[4], wsp[4];
+};
+
+// Workgroup `id` of product g: tile (id % tiles), k-split (id / tiles).
+template <int EPI>
+__device__ __forceinline__ void gemm_tile(const GemmArgs& g, int id, GemmSmem& sm) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const GemmEpi& ep = g.ep;
+  const int M = g.M, N = g.N;
+  // n-tiles fastest, so that the workgroups sharing a row block of A run
+  // together and find it in L2
+  const int nt = N / 64, tiles = (M / 64) * nt;
+  const int tile = id % tiles, split = id / tiles;
+  const int m0 = (tile / nt) * 64, n0 = (tile % nt) * 64;
+  // split-K: this workgroup sums k in [kbeg, kend) into its own partial
+  const int kc = g.splits > 1 ? ep.kc : g.K;
+  const int kbeg = split * kc;
+  const int kend = min(g.K, kbeg + kc);
+  const int K = max(kend - kbeg, 0);
+  float* Cf = ep.Cf;
+  if (EPI == EPI_F32) Cf += (size_t)split * (size_t)ep.split_stride;
+  const int r = lane & 15, q = lane >> 4;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // the feature-major copy of A, from the workgroups of the first n-tile
+  bf16* AT = (ep.AT && n0 == 0) ? ep.AT + (size_t)kbeg * (size_t)ep.ldat + m0 : nullptr;
+  gemm_steps(g.A + kbeg, g.lda, g.B + kbeg, g.ldb, ep.aidx, ep.anv, AT, ep.ldat, m0, n0, K / 32, wave, lane,
+             sm.st[0], sm.st[1], sm.st[2], sm.st[3], acc);
+  // epilogue: this wave's tile (i, j) element (row ra + 16 i + 4 q + e,
+  // column rb + 16 j + r of the workgroup's tile)
+  const int ra = 32 * (wave >> 1), rb = 32 * (wave & 1);
+  constexpr bool SUMS = (EPI == EPI_GELU_BWD || EPI == EPI_MSE);
+  float lsum = 0.f, wsum = 0.f;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int n = n0 + rb + 16 * j + r;
+    const float bn = (EPI == EPI_F32 || EPI == EPI_GELU_BWD) ? 0.f : ep.bias[n];
+    float csum = 0.f;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int mb = m0 + ra + 16 * i + 4 * q;
+      bf16 tv[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int m = mb + e;
+        const float v = acc[i][j][e];
+        if (EPI == EPI_F32) {
+          Cf[(size_t)m * ep.ldcf + n] = v;
+        } else if (EPI == EPI_BIAS_F32) {
+          Cf[(size_t)m * ep.ldcf + n] = v + bn;
+        } else if (EPI == EPI_BIAS_F32_T) {  // output-major: Cf[n][m]
+          Cf[(size_t)n * ep.ldcf + m] = v + bn;
+        } else if (EPI == EPI_BIAS_GELU) {  // bf16 activation, optional fp32 pre-activation
+          const float pre = v + bn;
+          if (Cf) Cf[(size_t)m * ep.ldcf + n] = pre;
+          tv[e] = f2bf(gelu(pre));
+          ep.Cb[(size_t)m * ep.ldcb + n] = tv[e];
+        } else if (EPI == EPI_GELU_BWD) {  // gradient at the pre-activation of the layer below
+          tv[e] = f2bf(v * gelu_grad(ep.pre[(size_t)m * ep.ldp + n]));
+          ep.Cb[(size_t)m * ep.ldcb + n] = tv[e];
+          csum += bf2f(tv[e]);
+        } else {  // EPI_MSE: output y = v + b, loss w (y - t)^2 and its gradient
+          // 2 w (y - t); the 1 / sum(w) of the mean is applied to the summed
+          // gradient in k_grad_step, where the batch's sum(w) is known
+          float gr = 0.f;
+          if (m < ep.mvalid && n < ep.nvalid) {
+            const size_t tt = (ep.tidx ? (size_t)ep.tidx[m] : (size_t)m) * (size_t)ep.ldt + n;
+            const float w = ep.W[tt];
+            const float d = (v + bn) - ep.T[tt];
+            lsum += w * d * d;
+            wsum += w;
+            gr = 2.f * w * d;
+          }
+          tv[e] = f2bf(gr);
+          ep.Cb[(size_t)m * ep.ldcb + n] = tv[e];
+          csum += bf2f(tv[e]);
+        }
+      }
+      if ((EPI == EPI_BIAS_GELU || SUMS) && ep.CbT) {
+        // rows mb .. mb + 3 of column n are adjacent in the transposed copy
+        uint2 w;
+        w.x = (uint32_t)tv[0] | ((uint32_t)tv[1] << 16);
+        w.y = (uint32_t)tv[2] | ((uint32_t)tv[3] << 16);
+        *(uint2*)(ep.CbT + (size_t)n * ep.ldct + mb) = w;
+      }
+    }
+    if (SUMS) {
+      // the column's 32 rows of this wave: lanes r, r + 16, r + 32, r + 48
+      // hold 8 rows each; pairwise sums are the same on every lane
+      csum += __shfl_xor(csum, 16);
+      csum += __shfl_xor(csum, 32);
+      if (q == 0) sm.csp[wave >> 1][rb + 16 * j + r] = csum;
+    }
+  }
+  if (EPI == EPI_MSE) {
+    // the wave's loss and weight sums: butterfly over the 64 lanes (the
+    // same value on each)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lsum += __shfl_xor(lsum, o);
+      wsum += __shfl_xor(wsum, o);
+    }
+    if (lane == 0) {
+      sm.lsp[wave] = lsum;
+      sm.wsp[wave] = wsum;
+    }
+  }
+  if (SUMS) {
+    // the tile's 64 rows: upper half + lower half, in that order
+    __syncthreads();
+    if (threadIdx.x < 64)
+      ep.colsum[(size_t)(m0 / 64) * (size_t)N + n0 + threadIdx.x] = sm.csp[0][threadIdx.x] + sm.csp[1][threadIdx.x];
+    if (EPI == EPI_MSE && threadIdx.x == 0) {
+      ep.lpart[tile] = (sm.lsp[0] + sm.lsp[1]) + (sm.lsp[2] + sm.lsp[3]);
+      ep.cpart[tile] = (sm.wsp[0] + sm.wsp[1]) + (sm.wsp[2] + sm.wsp[3]);
+    }
+  }
+}
+
+template <int EPI>
+__global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
+  __shared__ __align__(16) GemmSmem sm;
+  gemm_tile<EPI>(g, (int)blockIdx.x, sm);
+}
+
+template <int E1, int E2>
+__global__ void __launch_bounds__(256) k_gemm_pair(GemmArgs g1, GemmArgs g2) {
+  __shared__ __align__(16) GemmSmem sm;
+  const int n1 = (g1.M / 64) * (g1.N / 64) * g1.splits;
+  if ((int)blockIdx.x < n1)
+    gemm_tile<E1>(g1, (int)blockIdx.x, sm);
+  else
+    gemm_tile<E2>(g2, (int)blockIdx.x - n1, sm);
+}
+
+template __global__ void k_gemm_bf16<EPI_BIAS_F32>(GemmArgs);
+template __global__ void k_gemm_bf16<EPI_BIAS_F32_T>(GemmArgs);
+template __global__ void k_gemm_bf16<EPI_BIAS_GELU>(GemmArgs);
+template __global__ void k_gemm_bf16<EPI_MSE>(GemmArgs);
+template __global__ void k_gemm_bf16<EPI_F32>(GemmArgs);
+template __global__ void k_gemm_pair<EPI_F32, EPI_GELU_BWD>(GemmArgs, GemmArgs);
 
 // One layer of the fused kernel for a wave owning rows [row0, row0 + 64) of
 // the block and NTW 16-wide output tiles from column nw0:
@@ -343,49 +542,9 @@ __global__ void k_transpose_bf16(const bf16* __restrict__ in, int ldi, bf16* __r
   }
 }
 
-// d = g * gelu'(pre) (FP32 in), written as bf16 (row-major, pitch ld).
-__global__ void k_gelu_backward(const float* __restrict__ g, const float* __restrict__ pre, bf16* __restrict__ d,
-                                long long n) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  d[i] = f2bf(g[i] * gelu_grad(pre[i]));
-}
-
-// Masked mean-squared error of the output layer and its gradient (bf16).
-// Y, T, W (weights 0/1) are M x N (pitch ld); rows >= Mvalid contribute 0.
-// Masked mean-squared error of the output layer and its gradient (bf16):
-// grid-stride over the M x N elements (pitch ld; rows >= Mvalid and
-// columns >= N give 0); each workgroup writes its partial loss sum to
-// part[blockIdx.x], which k_loss_final adds up in a fixed order (no
-// atomics: the loss is reproducible).
-__global__ void k_mse(const float* __restrict__ Y, const float* __restrict__ T, const float* __restrict__ W, int M,
-                      int Mvalid, int N, int ld, float inv_count, bf16* __restrict__ dY, float* __restrict__ part) {
-  __shared__ float red[256];
-  const long long total = (long long)M * ld, stride = (long long)gridDim.x * blockDim.x;
-  float l = 0.f;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-    const int m = (int)(i / ld), n = (int)(i % ld);
-    float g = 0.f;
-    if (m < Mvalid && n < N) {
-      const float w = W[i];
-      const float e = Y[i] - T[i];
-      l += w * e * e;
-      g = 2.f * w * e * inv_count;
-    }
-    dY[i] = f2bf(g);
-  }
-  red[threadIdx.x] = l;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-}
-
-// *loss += inv_count * sum of part[0..n) (one workgroup of 256, fixed order)
-__global__ void k_loss_final(const float* __restrict__ part, int n, float inv_count, float* __restrict__ loss) {
-  __shared__ float red[256];
+// Fixed-order sum of part[0..n) by one workgroup of 256 (the same value in
+// every thread; red is overwritten).
+__device__ __forceinline__ float block_sum(const float* __restrict__ part, int n, float* red) {
   float a = 0.f;
   for (int i = threadIdx.x; i < n; i += blockDim.x) a += part[i];
   red[threadIdx.x] = a;
@@ -394,52 +553,82 @@ __global__ void k_loss_final(const float* __restrict__ part, int n, float inv_co
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *loss += red[0] * inv_count;
-}
-
-// Partial column sums of d (bf16, M x N, pitch ld) over the row chunk
-// [blockIdx.y * rows, (blockIdx.y + 1) * rows): part[blockIdx.y * N + n].
-// Block (64, BP_PHASES): 64 columns, BP_PHASES row phases combined in a
-// fixed order.
-__global__ void k_bias_partial(const bf16* __restrict__ d, int M, int N, int ld, int rows, float* __restrict__ part) {
-  __shared__ float red[BP_PHASES][64];
-  const int n = blockIdx.x * 64 + threadIdx.x;
-  const int r0 = blockIdx.y * rows, r1 = min(M, r0 + rows);
-  float s = 0.f;
-  if (n < N)
-    for (int m = r0 + threadIdx.y; m < r1; m += BP_PHASES) s += bf2f(d[(size_t)m * ld + n]);
-  red[threadIdx.y][threadIdx.x] = s;
+  const float v = red[0];
   __syncthreads();
-  for (int w = BP_PHASES / 2; w > 0; w >>= 1) {
-    if (threadIdx.y < w) red[threadIdx.y][threadIdx.x] += red[threadIdx.y + w][threadIdx.x];
-    __syncthreads();
+  return v;
+}
+
+// The batch loss from the output layer's per-tile partials: *loss +=
+// sum(lpart) / sum(cpart), *count += sum(cpart).
+__device__ __forceinline__ void batch_loss(const float* __restrict__ lpart, const float* __restrict__ cpart, int n,
+                                           float* __restrict__ loss, float* __restrict__ count, float* red) {
+  const float c = block_sum(cpart, n, red);
+  const float l = block_sum(lpart, n, red);
+  if (threadIdx.x == 0) {
+    *loss += c > 0.f ? l * (1.f / c) : 0.f;
+    if (count) *count += c;
   }
-  if (threadIdx.y == 0 && n < N) part[(size_t)blockIdx.y * N + n] = red[0][threadIdx.x];
 }
 
-// out[i] = sum over s < S of part[s * stride + i], in order of s
-// (deterministic split reductions).
-__global__ void k_reduce_splits(const float* __restrict__ part, int S, long long n, long long stride,
-                                float* __restrict__ out) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float a = 0.f;
-  for (int s = 0; s < S; ++s) a += part[(size_t)s * (size_t)stride + (size_t)i];
-  out[i] = a;
+__global__ void __launch_bounds__(256) k_loss_final(const float* __restrict__ lpart, const float* __restrict__ cpart,
+                                                    int n, float* __restrict__ loss, float* __restrict__ count) {
+  __shared__ float red[256];
+  batch_loss(lpart, cpart, n, loss, count, red);
 }
 
-// Adam on FP32 master parameters (Kingma & Ba 2015, bias-corrected).
-__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
-                       float* __restrict__ m2, long long n, float lr, float b1, float b2, float eps, float c1,
-                       float c2) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float gi = g[i];
-  float a = b1 * m1[i] + (1.f - b1) * gi;
-  float b = b2 * m2[i] + (1.f - b2) * gi * gi;
-  m1[i] = a;
-  m2[i] = b;
-  p[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+// One launch after the backward pass, thread per parameter of the padded
+// FP32 buffer: the gradient is the fixed-order sum of its split partials
+// (S k-chunks of the batch for weights, SB 64-row blocks for biases) times
+// 1 / sum(w) of the batch (every workgroup forms that sum from the output
+// layer's per-tile partials, in the same order), kept in g; with update,
+// Adam (Kingma & Ba 2015, bias-corrected by c1, c2) moves the FP32 master
+// and the BF16 copies W and W^T follow.  The last workgroup also adds the
+// batch loss to *loss and sum(w) to *count.
+__global__ void __launch_bounds__(256) k_grad_step(GradPlan gp, int S, int SB, long long np, float* __restrict__ p,
+                                                   float* __restrict__ g, float* __restrict__ m1,
+                                                   float* __restrict__ m2, bf16* __restrict__ Wb,
+                                                   bf16* __restrict__ WTb, int update, float lr, float b1, float b2,
+                                                   float eps, float c1, float c2, const float* __restrict__ lpart,
+                                                   const float* __restrict__ cpart, int nlp, float* __restrict__ loss,
+                                                   float* __restrict__ count) {
+  __shared__ float red[256];
+  const float cnt = block_sum(cpart, nlp, red);
+  const float inv = cnt > 0.f ? 1.f / cnt : 0.f;
+  if (loss && blockIdx.x == gridDim.x - 1) batch_loss(lpart, cpart, nlp, loss, count, red);
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= np) return;
+  int l = 0;
+  while (l + 1 < gp.nl && j >= gp.offW[l + 1]) ++l;
+  const long long e = j - gp.offW[l];
+  const long long oi = (long long)gp.pout[l] * gp.pin[l];
+  // the partials of a group of 16 are loaded together, then added in order
+  const bool wt = e < oi;
+  const float* q = wt ? gp.pw[l] + e : gp.pb[l] + (e - oi);
+  const size_t st = wt ? (size_t)oi : (size_t)gp.pout[l];
+  const int ns = wt ? S : SB;
+  float gv = 0.f;
+  for (int s0 = 0; s0 < ns; s0 += 16) {
+    float v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = s0 + u < ns ? q[(size_t)(s0 + u) * st] : 0.f;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) gv += v[u];
+  }
+  gv *= inv;
+  g[j] = gv;
+  if (!update) return;
+  const float a = b1 * m1[j] + (1.f - b1) * gv;
+  const float b = b2 * m2[j] + (1.f - b2) * gv * gv;
+  m1[j] = a;
+  m2[j] = b;
+  const float pn = p[j] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+  p[j] = pn;
+  if (e < oi) {
+    const int rr = (int)(e / gp.pin[l]), cc = (int)(e % gp.pin[l]);
+    const bf16 v = f2bf(pn);
+    Wb[gp.offWb[l] + e] = v;
+    WTb[gp.offWb[l] + (long long)cc * gp.pout[l] + rr] = v;
+  }
 }
 
 // BF16 copies of one layer: W (out x in, padded rows/cols zero) and its

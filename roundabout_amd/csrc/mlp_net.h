@@ -40,16 +40,30 @@ struct Net {
   float* Y = nullptr;
   float* T = nullptr;
   float* Wm = nullptr;
-  bf16* dY = nullptr;
-  bf16* dYT = nullptr;
-  float* dXf = nullptr;
-  bf16* dpre = nullptr;
-  bf16* dpreT = nullptr;
+  bf16* dY = nullptr;      // gradient at the current layer's output, row-major
+  bf16* dYT = nullptr;     // the same, feature-major (dW operand)
+  bf16* dpre = nullptr;    // next (lower) layer's gradient, row-major
+  bf16* dpreT = nullptr;   // and feature-major
   float* dloss = nullptr;
-  float* lpart = nullptr;  // per-workgroup partial losses of k_mse
-  float* ws = nullptr;  // split-K partials of the weight and bias gradients
+  float* lpart = nullptr;  // per-tile sums of w (y - t)^2 of the output layer
+  float* cpart = nullptr;  // per-tile sums of w
+  float* wsw = nullptr;    // weight-gradient partials: S x (pout * pin) per layer
+  float* wsb = nullptr;    // bias-gradient partials: SB x pout per layer
+  std::vector<size_t> offPW, offPB;  // layer offsets into wsw / wsb
+  int bw_S = 0, bw_SB = 0, bw_nlp = 0;  // partials of the last training forward / backward pass
   hipStream_t stream = nullptr;
   std::string err;
+  // Batch source of train_forward(): by default X0, T and Wm.
+  // A caller that holds the whole training set on the device points the
+  // first layer and the loss at it instead, with batch row m taken from set
+  // row idx[m] (m < nvalid); the first layer then writes X0T itself.
+  struct Source {
+    const bf16* X = nullptr;  // rows of pin[0] bf16
+    const float* T = nullptr;  // rows of pout[last] floats
+    const float* W = nullptr;
+    const unsigned* idx = nullptr;
+    int nvalid = 0;
+  } src;
 
   ~Net() { release(); }
   int init(const std::vector<int>& dims, hipStream_t s);
@@ -58,17 +72,28 @@ struct Net {
   int get_params(float* host) const;
   int get_grads(float* host) const;
   size_t n_public() const;
-  // X0 (Mpad x pin[0] bf16) must hold the inputs; Y receives the outputs,
-  // row-major (Mpad x pout) or, with out_major, output-major (pout x Mpad)
-  // Inference (keep_pre false) with Mpad a multiple of 128 and widths within
-  // the fused kernel's limits runs as one fused launch (k_mlp_fused).
-  int forward(int Mpad, bool keep_pre, bool out_major = false);
+  // Inference: X0 (Mpad x pin[0] bf16) must hold the inputs; Y receives the
+  // outputs, row-major (Mpad x pout) or, with out_major, output-major (pout
+  // x Mpad).  Mpad a multiple of 128 and widths within the fused kernel's
+  // limits run as one fused launch (k_mlp_fused), else per-layer GEMMs.
+  int forward(int Mpad, bool out_major = false);
   bool fused_ok(int Mpad) const;
-  // T, Wm (Mpad x pout[last]) hold targets and weights; rows >= Mvalid
-  // ignored.  The loss is added to *loss_dev (device, no synchronisation)
-  // when given, else returned in *loss_out (synchronises) when that is given.
-  int backward(int Mpad, int Mvalid, double count, float* loss_out, float* loss_dev = nullptr);
-  int adam(float lr, float b1, float b2, float eps);
+  // X0 -> X0T (feature-major copy of the inputs) for callers that fill X0
+  // themselves.
+  int transpose_input(int Mpad);
+  // Training.  train_forward() runs the layers keeping what the backward
+  // pass needs; the output layer's epilogue forms the masked MSE (targets
+  // and 0/1 weights from T, Wm or the source set; rows >= Mvalid ignored;
+  // the mean over sum(w) of the batch) as per-tile partials together with
+  // its gradient.  backward() leaves the parameter gradient as split
+  // partials; update() reduces and normalises them (into g), takes an Adam
+  // step when adam_step is set, and adds the batch loss to *loss_dev and
+  // sum(w) to *count_dev (device; count_dev may be null); add_loss() adds
+  // the loss alone (validation).
+  int train_forward(int Mpad, int Mvalid);
+  int backward(int Mpad);
+  int update(bool adam_step, float lr, float b1, float b2, float eps, float* loss_dev, float* count_dev);
+  int add_loss(float* loss_dev, float* count_dev);
   void release();
 
  private:

@@ -15,13 +15,26 @@ inline dim3 g1(long long n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
 
 // M, N multiples of 64, K of 32; splits > 1 (EPI_F32 only) writes the
 // partial products of k-chunks of length kc to Cf + s * split_stride.
-template <int EPI>
-void gemm(hipStream_t s, const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, const float* bias,
-          float* Cf, int ldcf, bf16* Cb, int ldcb, int splits = 1, int kc = 0, long long split_stride = 0) {
-  dim3 grid((unsigned)((long long)(M / 64) * (N / 64)), (unsigned)splits);
-  hipLaunchKernelGGL(k_gemm_bf16<EPI>, grid, dim3(64), 0, s, A, lda, B, ldb, M, N, K, bias, Cf, ldcf, Cb, ldcb,
-                     splits > 1 ? kc : K, split_stride);
+inline int gemm_blocks(const GemmArgs& g) { return (g.M / 64) * (g.N / 64) * g.splits; }
+inline GemmArgs gargs(const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, const GemmEpi& ep,
+                      int splits = 1) {
+  GemmArgs g{};
+  g.A = A;
+  g.lda = lda;
+  g.B = B;
+  g.ldb = ldb;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.splits = splits;
+  g.ep = ep;
+  return g;
 }
+template <int EPI>
+void gemm(hipStream_t s, const GemmArgs& g) {
+  hipLaunchKernelGGL(k_gemm_bf16<EPI>, dim3((unsigned)gemm_blocks(g)), dim3(256), 0, s, g);
+}
+inline GemmEpi epi() { return GemmEpi{}; }
 
 // k-splits of a weight-gradient product over Mpad batch rows: at most 32
 // chunks of at least 64 rows, chunk length a multiple of 32
@@ -89,7 +102,6 @@ int Net::init(const std::vector<int>& dims, hipStream_t s) {
   NT_TRY(hipMalloc((void**)&Wb, 2 * nwb));
   NT_TRY(hipMalloc((void**)&WTb, 2 * nwb));
   NT_TRY(hipMalloc((void**)&dloss, 64));
-  NT_TRY(hipMalloc((void**)&lpart, sizeof(float) * MSE_BLOCKS));
   NT_TRY(hipMemsetAsync(p, 0, 4 * np, stream));
   NT_TRY(hipMemsetAsync(g, 0, 4 * np, stream));
   NT_TRY(hipMemsetAsync(m1, 0, 4 * np, stream));
@@ -159,16 +171,18 @@ void Net::free_workspace() {
   F(Wm);
   F(dY);
   F(dYT);
-  F(dXf);
   F(dpre);
   F(dpreT);
-  F(ws);
-  ws = nullptr;
+  F(wsw);
+  F(wsb);
+  F(lpart);
+  F(cpart);
+  wsw = wsb = lpart = cpart = nullptr;
   X0 = X0T = nullptr;
   pre.clear();
   act.clear();
   actT.clear();
-  Y = T = Wm = dXf = nullptr;
+  Y = T = Wm = nullptr;
   dY = dYT = dpre = dpreT = nullptr;
   Mcap = 0;
 }
@@ -185,8 +199,7 @@ void Net::release() {
   F(Wb);
   F(WTb);
   F(dloss);
-  F(lpart);
-  p = g = m1 = m2 = dloss = lpart = nullptr;
+  p = g = m1 = m2 = dloss = nullptr;
   Wb = WTb = nullptr;
   nl = 0;
 }
@@ -216,14 +229,24 @@ int Net::ensure_workspace(int Mpad, bool training) {
     NT_TRY(hipMalloc((void**)&Wm, 4 * M * po));
     NT_TRY(hipMalloc((void**)&dY, 2 * M * wmax));
     NT_TRY(hipMalloc((void**)&dYT, 2 * M * wmax));
-    NT_TRY(hipMalloc((void**)&dXf, 4 * M * wmax));
     NT_TRY(hipMalloc((void**)&dpre, 2 * M * wmax));
     NT_TRY(hipMalloc((void**)&dpreT, 2 * M * wmax));
     int S, kc;
     batch_splits(Mpad, &S, &kc);
-    size_t per = 0;
-    for (int l = 0; l < nl; ++l) per = std::max(per, (size_t)pout[l] * pin[l] + pout[l]);
-    NT_TRY(hipMalloc((void**)&ws, 4 * (size_t)S * per));
+    const size_t SB = M / 64;
+    offPW.assign(nl, 0);
+    offPB.assign(nl, 0);
+    size_t nw = 0, nbias = 0;
+    for (int l = 0; l < nl; ++l) {
+      offPW[l] = nw;
+      nw += (size_t)S * pout[l] * pin[l];
+      offPB[l] = nbias;
+      nbias += SB * pout[l];
+    }
+    NT_TRY(hipMalloc((void**)&wsw, 4 * nw));
+    NT_TRY(hipMalloc((void**)&wsb, 4 * nbias));
+    NT_TRY(hipMalloc((void**)&lpart, 4 * SB * (size_t)(po / 64)));
+    NT_TRY(hipMalloc((void**)&cpart, 4 * SB * (size_t)(po / 64)));
   }
   Mcap = Mpad;
   return 0;
@@ -241,8 +264,8 @@ bool Net::fused_ok(int Mpad) const {
   return true;
 }
 
-int Net::forward(int Mpad, bool keep_pre, bool out_major) {
-  if (!keep_pre && fused_ok(Mpad)) {
+int Net::forward(int Mpad, bool out_major) {
+  if (fused_ok(Mpad)) {
     // inference: one launch over all layers, activations in LDS
     static bool lds_set = false;
     const size_t shmem = FP_LDS_BYTES;
@@ -265,85 +288,164 @@ int Net::forward(int Mpad, bool keep_pre, bool out_major) {
     NT_TRY(hipGetLastError());
     return 0;
   }
+  // per-layer GEMMs (batches the fused kernel does not take)
   const bf16* x = X0;
   int ldx = pin[0];
   for (int l = 0; l < nl; ++l) {
+    GemmEpi ep = epi();
+    ep.bias = p + offB[l];
     const bf16* W = Wb + offWb[l];
-    const float* b = p + offB[l];
     if (l < nl - 1) {
-      gemm<EPI_BIAS_GELU>(stream, x, ldx, W, pin[l], Mpad, pout[l], pin[l], b, keep_pre ? pre[l] : nullptr, pout[l],
-                          act[l], pout[l]);
+      ep.Cb = act[l];
+      ep.ldcb = pout[l];
+      gemm<EPI_BIAS_GELU>(stream, gargs(x, ldx, W, pin[l], Mpad, pout[l], pin[l], ep));
       x = act[l];
       ldx = pout[l];
     } else if (out_major) {
-      gemm<EPI_BIAS_F32_T>(stream, x, ldx, W, pin[l], Mpad, pout[l], pin[l], b, Y, Mpad, nullptr, 0);
+      ep.Cf = Y;
+      ep.ldcf = Mpad;
+      gemm<EPI_BIAS_F32_T>(stream, gargs(x, ldx, W, pin[l], Mpad, pout[l], pin[l], ep));
     } else {
-      gemm<EPI_BIAS_F32>(stream, x, ldx, W, pin[l], Mpad, pout[l], pin[l], b, Y, pout[l], nullptr, 0);
+      ep.Cf = Y;
+      ep.ldcf = pout[l];
+      gemm<EPI_BIAS_F32>(stream, gargs(x, ldx, W, pin[l], Mpad, pout[l], pin[l], ep));
     }
   }
   NT_TRY(hipGetLastError());
   return 0;
 }
 
-int Net::backward(int Mpad, int Mvalid, double count, float* loss_out, float* loss_dev) {
+int Net::transpose_input(int Mpad) {
+  transpose(stream, X0, pin[0], X0T, Mpad, Mpad, pin[0]);
+  NT_TRY(hipGetLastError());
+  return 0;
+}
+
+int Net::train_forward(int Mpad, int Mvalid) {
   const int L = nl - 1;
   const int po = pout[L];
-  float* lacc = loss_dev ? loss_dev : dloss;
-  if (!loss_dev) NT_TRY(hipMemsetAsync(dloss, 0, 4, stream));
-  const float inv = count > 0 ? (float)(1.0 / count) : 0.f;
-  const int nb = (int)std::min<long long>(MSE_BLOCKS, ((long long)Mpad * po + 255) / 256);
-  hipLaunchKernelGGL(k_mse, dim3((unsigned)nb), dim3(256), 0, stream, Y, T, Wm, Mpad, Mvalid, dout[L], po, inv, dY,
-                     lpart);
-  hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(256), 0, stream, lpart, nb, inv, lacc);
-  int S, kc;
-  batch_splits(Mpad, &S, &kc);
-  // d (bf16, Mpad x pout[l]) is the gradient w.r.t. layer l's output (pre-activation)
-  bf16* d = dY;
-  for (int l = L; l >= 0; --l) {
-    const int o = pout[l], i = pin[l];
-    const bf16* xin = (l == 0) ? X0 : act[l - 1];
-    bf16* xinT = (l == 0) ? X0T : actT[l - 1];
-    // dW_l = d^T x_in : (o x Mpad) (i x Mpad)^T
-    transpose(stream, d, o, dYT, Mpad, Mpad, o);
-    transpose(stream, xin, i, xinT, Mpad, Mpad, i);
-    // split over the batch (K = Mpad is long, the output small), then a
-    // fixed-order sum of the partials
-    const long long oi = (long long)o * i;
-    gemm<EPI_F32>(stream, dYT, Mpad, xinT, Mpad, o, i, Mpad, nullptr, ws, i, nullptr, 0, S, kc, oi);
-    hipLaunchKernelGGL(k_reduce_splits, g1(oi, 256), dim3(256), 0, stream, ws, S, oi, oi, g + offW[l]);
-    float* bpart = ws + (size_t)S * (size_t)oi;
-    hipLaunchKernelGGL(k_bias_partial, dim3((unsigned)(o / 64), (unsigned)S), dim3(64, BP_PHASES), 0, stream, d, Mpad, o, o,
-                       kc, bpart);
-    hipLaunchKernelGGL(k_reduce_splits, g1(o, 256), dim3(256), 0, stream, bpart, S, (long long)o, (long long)o,
-                       g + offB[l]);
-    if (l > 0) {
-      // dX = d W_l : (Mpad x o) (i x o)^T, then through GELU of layer l-1
-      gemm<EPI_F32>(stream, d, o, WTb + offWb[l], o, Mpad, i, o, nullptr, dXf, i, nullptr, 0);
-      hipLaunchKernelGGL(k_gelu_backward, g1((long long)Mpad * i, 256), dim3(256), 0, stream, dXf, pre[l - 1], dpre,
-                         (long long)Mpad * i);
-      // the next layer's d lives in dpre; swap the role of the two buffers
-      std::swap(dY, dpre);
-      d = dY;
+  int kc;
+  batch_splits(Mpad, &bw_S, &kc);
+  bw_SB = Mpad / 64;
+  bw_nlp = (Mpad / 64) * (po / 64);
+  const bool gathered = src.X != nullptr;
+  const bf16* x = gathered ? src.X : X0;
+  int ldx = pin[0];
+  for (int l = 0; l < nl; ++l) {
+    GemmEpi ep = epi();
+    ep.bias = p + offB[l];
+    const bf16* W = Wb + offWb[l];
+    if (l == 0 && gathered) {
+      // the batch rows straight from the set, and their feature-major copy
+      // (the first layer's weight-gradient operand) as they stream in
+      ep.aidx = src.idx;
+      ep.anv = src.nvalid;
+      ep.AT = X0T;
+      ep.ldat = Mpad;
+    }
+    if (l < L) {
+      // hidden layer: the FP32 pre-activation (for GELU') and the activation
+      // row-major (next layer's input) and feature-major (next layer's
+      // weight-gradient operand)
+      ep.Cf = pre[l];
+      ep.ldcf = pout[l];
+      ep.Cb = act[l];
+      ep.ldcb = pout[l];
+      ep.CbT = actT[l];
+      ep.ldct = Mpad;
+      gemm<EPI_BIAS_GELU>(stream, gargs(x, ldx, W, pin[l], Mpad, pout[l], pin[l], ep));
+      x = act[l];
+      ldx = pout[l];
+    } else {
+      // output layer: the loss partials, the output gradient in both
+      // layouts and its bias-gradient partials straight from the tile
+      ep.T = src.T ? src.T : T;
+      ep.W = src.T ? src.W : Wm;
+      ep.tidx = src.T ? src.idx : nullptr;
+      ep.ldt = po;
+      ep.mvalid = Mvalid;
+      ep.nvalid = dout[L];
+      ep.lpart = lpart;
+      ep.cpart = cpart;
+      ep.Cb = dY;
+      ep.ldcb = po;
+      ep.CbT = dYT;
+      ep.ldct = Mpad;
+      ep.colsum = wsb + offPB[L];
+      gemm<EPI_MSE>(stream, gargs(x, ldx, W, pin[l], Mpad, po, pin[l], ep));
     }
   }
   NT_TRY(hipGetLastError());
-  if (loss_out && !loss_dev) {
-    NT_TRY(hipMemcpyAsync(loss_out, dloss, 4, hipMemcpyDeviceToHost, stream));
-    NT_TRY(hipStreamSynchronize(stream));
-  }
   return 0;
 }
 
-int Net::adam(float lr, float b1, float b2, float eps) {
-  ++adam_t;
-  float c1 = 1.f - std::pow(b1, (float)adam_t), c2 = 1.f - std::pow(b2, (float)adam_t);
-  hipLaunchKernelGGL(k_adam, g1((long long)np, 256), dim3(256), 0, stream, p, g, m1, m2, (long long)np, lr, b1, b2,
-                     eps, c1, c2);
-  for (int l = 0; l < nl; ++l) {
-    dim3 grid((unsigned)((pin[l] + 255) / 256), (unsigned)pout[l]);
-    hipLaunchKernelGGL(k_refresh_weights, grid, dim3(256), 0, stream, p + offW[l], pout[l], pin[l], Wb + offWb[l],
-                       pin[l], WTb + offWb[l], pout[l], pout[l], pin[l]);
+int Net::backward(int Mpad) {
+  const int L = nl - 1;
+  int S, kc;
+  batch_splits(Mpad, &S, &kc);
+  for (int l = L; l >= 0; --l) {
+    const int o = pout[l], i = pin[l];
+    const bf16* xinT = (l == 0) ? X0T : actT[l - 1];
+    // dW_l = d^T x_in : (o x Mpad) (i x Mpad)^T, split over the batch (K =
+    // Mpad is long, the output small) into partials summed in k_grad_step
+    GemmEpi ew = epi();
+    ew.Cf = wsw + offPW[l];
+    ew.ldcf = i;
+    ew.kc = kc;
+    ew.split_stride = (long long)o * i;
+    const GemmArgs gw = gargs(dYT, Mpad, xinT, Mpad, o, i, Mpad, ew, S);
+    if (l == 0) {
+      gemm<EPI_F32>(stream, gw);
+      break;
+    }
+    // with it, in the same launch: d_{l-1} = (d W_l) * GELU'(pre_{l-1}) :
+    // (Mpad x o) (i x o)^T, its feature-major copy and the bias-gradient
+    // partials of layer l - 1
+    GemmEpi ex = epi();
+    ex.pre = pre[l - 1];
+    ex.ldp = i;
+    ex.Cb = dpre;
+    ex.ldcb = i;
+    ex.CbT = dpreT;
+    ex.ldct = Mpad;
+    ex.colsum = wsb + offPB[l - 1];
+    const GemmArgs gx = gargs(dY, o, WTb + offWb[l], o, Mpad, i, o, ex);
+    hipLaunchKernelGGL((k_gemm_pair<EPI_F32, EPI_GELU_BWD>), dim3((unsigned)(gemm_blocks(gw) + gemm_blocks(gx))),
+                       dim3(256), 0, stream, gw, gx);
+    std::swap(dY, dpre);
+    std::swap(dYT, dpreT);
   }
+  NT_TRY(hipGetLastError());
+  return 0;
+}
+
+int Net::update(bool adam_step, float lr, float b1, float b2, float eps, float* loss_dev, float* count_dev) {
+  GradPlan gp{};
+  gp.nl = nl;
+  for (int l = 0; l < nl; ++l) {
+    gp.pin[l] = pin[l];
+    gp.pout[l] = pout[l];
+    gp.offW[l] = (long long)offW[l];
+    gp.offWb[l] = (long long)offWb[l];
+    gp.pw[l] = wsw + offPW[l];
+    gp.pb[l] = wsb + offPB[l];
+  }
+  float c1 = 1.f, c2 = 1.f;
+  if (adam_step) {
+    ++adam_t;
+    c1 = 1.f - std::pow(b1, (float)adam_t);
+    c2 = 1.f - std::pow(b2, (float)adam_t);
+  }
+  hipLaunchKernelGGL(k_grad_step, g1((long long)np, 256), dim3(256), 0, stream, gp, bw_S, bw_SB, (long long)np, p, g,
+                     m1, m2, Wb, WTb, adam_step ? 1 : 0, lr, b1, b2, eps, c1, c2, (const float*)lpart,
+                     (const float*)cpart, bw_nlp, loss_dev, count_dev);
+  NT_TRY(hipGetLastError());
+  return 0;
+}
+
+int Net::add_loss(float* loss_dev, float* count_dev) {
+  hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(256), 0, stream, (const float*)lpart, (const float*)cpart, bw_nlp,
+                     loss_dev, count_dev);
   NT_TRY(hipGetLastError());
   return 0;
 }

@@ -49,7 +49,6 @@ struct SurrState {
   float* Xraw = nullptr;
   float* Traw = nullptr;
   float* Wraw = nullptr;
-  float* rowsum = nullptr;
   long long n = 0, cap = 0;
   // normalised training set
   bf16* Xn = nullptr;
@@ -207,8 +206,8 @@ __global__ void k_features_raw(StateView s, const double* cor, int ncols, int st
 // have.  (Weighting the kt targets by the diffusive flux they carry was
 // tried and left the 24-step temperature error no better: the network's
 // limit there is resolving Ri, not the weighting.)
-__global__ void k_targets_raw(StateView s, int ncols, int stride, int dout, float* T, float* W, float* rowsum,
-                              long long row0, int nzn, int lev_off) {
+__global__ void k_targets_raw(StateView s, int ncols, int stride, int dout, float* T, float* W, long long row0,
+                              int nzn, int lev_off) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ncols) return;
   const int P = s.P;
@@ -216,7 +215,6 @@ __global__ void k_targets_raw(StateView s, int ncols, int stride, int dout, floa
   const int kb = s.kb[p];
   float* t = T + (size_t)(row0 + i) * (size_t)dout;
   float* w = W + (size_t)(row0 + i) * (size_t)dout;
-  float sum = 0.f;
   for (int j = 1; j < nzn; ++j) {
     const int k = j - lev_off;  // this grid's interface (between layers k - 1 and k)
     const bool on = k >= 1 && k >= kb;
@@ -225,11 +223,9 @@ __global__ void k_targets_raw(StateView s, int ncols, int stride, int dout, floa
     t[nzn - 1 + j - 1] = k >= 1 ? (float)log10(fmax(LV(s.kv, k), 1e-30)) : 0.f;
     w[j - 1] = wt;
     w[nzn - 1 + j - 1] = wv;
-    sum += wt + wv;
   }
   t[2 * (nzn - 1)] = (float)s.bl[p];
   w[2 * (nzn - 1)] = 1.f;
-  rowsum[row0 + i] = sum + 1.f;
 }
 
 // Per-column mean/std (weighted when W is given) and min/max of an n x d
@@ -301,19 +297,6 @@ __global__ void k_norm_targets(const float* T, const float* W, long long n, int 
     bool in = c < dout;
     Tn[i] = in ? (T[(size_t)r * dout + c] - mean[c]) / stdv[c] : 0.f;
     Wn[i] = in ? W[(size_t)r * dout + c] : 0.f;
-  }
-}
-
-// rows idx[b] of the normalised set into a training batch (rows >= nb zero)
-__global__ void k_gather(const bf16* Xn, int ldx, const float* Tn, const float* Wn, int ldt, const unsigned* idx,
-                         int nb, bf16* X0, float* T, float* W) {
-  const int b = blockIdx.x;
-  const bool valid = b < nb;
-  const size_t src = valid ? (size_t)idx[b] : 0;
-  for (int c = threadIdx.x; c < ldx; c += blockDim.x) X0[(size_t)b * ldx + c] = valid ? Xn[src * ldx + c] : (bf16)0;
-  for (int c = threadIdx.x; c < ldt; c += blockDim.x) {
-    T[(size_t)b * ldt + c] = valid ? Tn[src * ldt + c] : 0.f;
-    W[(size_t)b * ldt + c] = valid ? Wn[src * ldt + c] : 0.f;
   }
 }
 
@@ -433,9 +416,9 @@ SurrState::~SurrState() {
 }
 
 void SurrState::free_dataset() {
-  for (void* q : {(void*)Xraw, (void*)Traw, (void*)Wraw, (void*)rowsum, (void*)Xn, (void*)Tn, (void*)Wn, (void*)idx})
+  for (void* q : {(void*)Xraw, (void*)Traw, (void*)Wraw, (void*)Xn, (void*)Tn, (void*)Wn, (void*)idx})
     if (q) (void)hipFree(q);
-  Xraw = Traw = Wraw = rowsum = Tn = Wn = nullptr;
+  Xraw = Traw = Wraw = Tn = Wn = nullptr;
   Xn = nullptr;
   idx = nullptr;
   n = cap = 0;
@@ -471,13 +454,12 @@ int surr_collect(rdb_colphys* h, double dt, int n_steps, int stride, long long* 
   if (need > S->cap) {
     // grow (copying what is already collected)
     const long long cap = std::max(need, S->cap * 2);
-    float *X = nullptr, *T = nullptr, *W = nullptr, *R = nullptr;
+    float *X = nullptr, *T = nullptr, *W = nullptr;
     hipError_t ae = hipMalloc((void**)&X, 4 * (size_t)cap * S->din);
     if (ae == hipSuccess) ae = hipMalloc((void**)&T, 4 * (size_t)cap * S->dout);
     if (ae == hipSuccess) ae = hipMalloc((void**)&W, 4 * (size_t)cap * S->dout);
-    if (ae == hipSuccess) ae = hipMalloc((void**)&R, 4 * (size_t)cap);
     if (ae != hipSuccess) {
-      for (float* q : {X, T, W, R})
+      for (float* q : {X, T, W})
         if (q) (void)hipFree(q);
       return perr(RDB_COLPHYS_ERR_SETUP, "mlp_collect: no device memory for %lld training columns: %s", cap,
                   hipGetErrorString(ae));
@@ -486,7 +468,6 @@ int surr_collect(rdb_colphys* h, double dt, int n_steps, int stride, long long* 
       SG_TRY(hipMemcpyAsync(X, S->Xraw, 4 * (size_t)S->n * S->din, hipMemcpyDeviceToDevice, h->stream));
       SG_TRY(hipMemcpyAsync(T, S->Traw, 4 * (size_t)S->n * S->dout, hipMemcpyDeviceToDevice, h->stream));
       SG_TRY(hipMemcpyAsync(W, S->Wraw, 4 * (size_t)S->n * S->dout, hipMemcpyDeviceToDevice, h->stream));
-      SG_TRY(hipMemcpyAsync(R, S->rowsum, 4 * (size_t)S->n, hipMemcpyDeviceToDevice, h->stream));
       SG_TRY(hipStreamSynchronize(h->stream));
     }
     long long n_keep = S->n;
@@ -494,7 +475,6 @@ int surr_collect(rdb_colphys* h, double dt, int n_steps, int stride, long long* 
     S->Xraw = X;
     S->Traw = T;
     S->Wraw = W;
-    S->rowsum = R;
     S->cap = cap;
     S->n = n_keep;
   }
@@ -516,7 +496,7 @@ int surr_collect(rdb_colphys* h, double dt, int n_steps, int stride, long long* 
       const int nc = (m->n_wet + stride - 1) / stride;
       if (nc > 0)
         hipLaunchKernelGGL(k_targets_raw, gs(nc, 128), dim3(128), 0, h->stream, m->s, nc, stride, S->dout, S->Traw,
-                           S->Wraw, S->rowsum, row, nzn, nzn - m->nz);
+                           S->Wraw, row, nzn, nzn - m->nz);
       row += nc;
     }
     SG_TRY(hipGetLastError());
@@ -593,8 +573,6 @@ int surr_train(rdb_colphys* h, const rdb_colphys_mlp_config* cfg, float* train_l
   hipLaunchKernelGGL(k_norm_targets, gs_capped(N * ldt, 256), dim3(256), 0, h->stream, S->Traw, S->Wraw, N, dout,
                      dst + 4 * din, dst + 4 * din + dout, S->Tn, S->Wn, ldt);
   SG_TRY(hipGetLastError());
-  std::vector<float> rowsum((size_t)N);
-  SG_TRY(hipMemcpyAsync(rowsum.data(), S->rowsum, 4 * (size_t)N, hipMemcpyDeviceToHost, h->stream));
   SG_TRY(hipStreamSynchronize(h->stream));
   // split: a held-out share for validation
   std::mt19937_64 rng((uint64_t)cfg->seed * 7919u + 17u);
@@ -611,54 +589,88 @@ int surr_train(rdb_colphys* h, const rdb_colphys_mlp_config* cfg, float* train_l
   if (cfg->epochs > 0 && nb_train == 0)
     return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "mlp_train: %zu training rows do not fill one batch of %d",
                 train.size(), B);
-  // per-batch loss slots on the device: one read-back per epoch
-  float* dl = nullptr;
-  SG_TRY(hipMalloc((void**)&dl, 4 * (nb_train + nb_val + 1)));
-  std::vector<float> lh(nb_train + nb_val);
-  std::vector<double> cnt(nb_train + nb_val);
-  // the validation rows sit after the training rows in S->idx
-  SG_TRY(hipMemcpyAsync(S->idx + train.size(), val.data(), 4 * val.size(), hipMemcpyHostToDevice, h->stream));
-  auto run_batch = [&](size_t off, const unsigned* rows, int nb, bool update, float* slot, double* count) -> int {
-    hipLaunchKernelGGL(k_gather, dim3((unsigned)Mpad), dim3(128), 0, h->stream, S->Xn, ldx, S->Tn, S->Wn, ldt,
-                       S->idx + off, nb, net.X0, net.T, net.Wm);
-    double c = 0;
-    for (int b = 0; b < nb; ++b) c += rowsum[rows[b]];
-    *count = c;
-    int r = net.forward(Mpad, true);
-    if (r) return perr(r, "mlp_train: %s", net.err.c_str());
-    r = net.backward(Mpad, nb, c, nullptr, slot);  // also the loss (its gradient is unused for validation)
-    if (r) return perr(r, "mlp_train: %s", net.err.c_str());
-    if (update) {
-      r = net.adam((float)cfg->lr, (float)cfg->beta1, (float)cfg->beta2, (float)cfg->eps);
-      if (r) return perr(r, "mlp_train: %s", net.err.c_str());
+  // per-batch loss and weight-sum slots on the device, read back once per
+  // epoch into pinned memory; the training order lives in pinned memory
+  // too, so that the next epoch's shuffle and upload overlap this epoch's
+  // batches
+  const size_t nslot = nb_train + nb_val, nt = train.size();
+  struct Bufs {
+    float* dl = nullptr;
+    float* lh = nullptr;
+    unsigned* order = nullptr;
+    hipStream_t s;
+    ~Bufs() {
+      (void)hipStreamSynchronize(s);
+      if (dl) (void)hipFree(dl);
+      if (lh) (void)hipHostFree(lh);
+      if (order) (void)hipHostFree(order);
     }
+  } bufs{nullptr, nullptr, nullptr, h->stream};
+  SG_TRY(hipMalloc((void**)&bufs.dl, 4 * (2 * nslot + 1)));
+  SG_TRY(hipHostMalloc((void**)&bufs.lh, 4 * (2 * nslot + 1)));
+  SG_TRY(hipHostMalloc((void**)&bufs.order, 4 * (nt + 1)));
+  float* dl = bufs.dl;
+  const float* lh = bufs.lh;
+  unsigned* order = bufs.order;
+  std::memcpy(order, train.data(), 4 * nt);
+  // the validation rows sit after the training rows in S->idx
+  SG_TRY(hipMemcpyAsync(S->idx + nt, val.data(), 4 * val.size(), hipMemcpyHostToDevice, h->stream));
+  // this epoch's order: shuffled in place (the same sequence of shuffles
+  // every run), uploaded behind the batches already queued
+  auto next_order = [&]() -> hipError_t {
+    std::shuffle(order, order + nt, rng);
+    return hipMemcpyAsync(S->idx, order, 4 * nt, hipMemcpyHostToDevice, h->stream);
+  };
+  // one batch: forward (reading the set through the index), then either the
+  // backward pass and an Adam step (training) or the loss alone (validation)
+  auto run_batch = [&](size_t off, int nb, bool update, size_t q) -> int {
+    net.src.X = S->Xn;
+    net.src.T = S->Tn;
+    net.src.W = S->Wn;
+    net.src.idx = S->idx + off;
+    net.src.nvalid = nb;
+    int r = net.train_forward(Mpad, nb);
+    if (!r && update) r = net.backward(Mpad);
+    if (!r && update)
+      r = net.update(true, (float)cfg->lr, (float)cfg->beta1, (float)cfg->beta2, (float)cfg->eps, dl + q,
+                     dl + nslot + q);
+    if (!r && !update) r = net.add_loss(dl + q, dl + nslot + q);
+    if (r) return perr(r, "mlp_train: %s", net.err.c_str());
     return 0;
   };
+  // the batches read the set through the index; inference after training
+  // reads the network's own input buffer again
+  struct SrcReset {
+    mlp::Net& n;
+    ~SrcReset() { n.src = mlp::Net::Source{}; }
+  } src_reset{net};
   int done = 0;
+  if (cfg->epochs > 0 && next_order() != hipSuccess)
+    return perr(RDB_COLPHYS_ERR_SETUP, "mlp_train: batch index upload failed");
   for (int ep = 0; ep < cfg->epochs; ++ep) {
-    std::shuffle(train.begin(), train.end(), rng);
     e = 0;
-    if (hipMemcpyAsync(S->idx, train.data(), 4 * train.size(), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-        hipMemsetAsync(dl, 0, 4 * (nb_train + nb_val), h->stream) != hipSuccess)
-      e = perr(RDB_COLPHYS_ERR_SETUP, "mlp_train: batch index upload failed");
-    for (size_t q = 0; !e && q < nb_train; ++q)
-      e = run_batch(q * B, train.data() + q * B, B, true, dl + q, &cnt[q]);
+    if (hipMemsetAsync(dl, 0, 4 * 2 * nslot, h->stream) != hipSuccess)
+      e = perr(RDB_COLPHYS_ERR_SETUP, "mlp_train: loss slot clear failed");
+    for (size_t q = 0; !e && q < nb_train; ++q) e = run_batch(q * B, B, true, q);
     for (size_t q = 0; !e && q < nb_val; ++q) {
       const size_t b0 = q * B;
       const int nb = (int)std::min<size_t>((size_t)B, val.size() - b0);
-      e = run_batch(train.size() + b0, val.data() + b0, nb, false, dl + nb_train + q, &cnt[nb_train + q]);
+      e = run_batch(nt + b0, nb, false, nb_train + q);
     }
-    if (!e && (hipMemcpyAsync(lh.data(), dl, 4 * lh.size(), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
-               hipStreamSynchronize(h->stream) != hipSuccess))
+    if (!e && hipMemcpyAsync(bufs.lh, dl, 4 * 2 * nslot, hipMemcpyDeviceToHost, h->stream) != hipSuccess)
       e = perr(RDB_COLPHYS_ERR_SETUP, "mlp_train: loss read-back failed");
-    if (e) {
-      (void)hipStreamSynchronize(h->stream);
-      (void)hipFree(dl);
-      return e;
-    }
+    // the upload is queued behind this epoch's batches, and the previous
+    // upload from `order` has completed (the stream was synchronised)
+    if (!e && ep + 1 < cfg->epochs && next_order() != hipSuccess)
+      e = perr(RDB_COLPHYS_ERR_SETUP, "mlp_train: batch index upload failed");
+    if (!e && hipStreamSynchronize(h->stream) != hipSuccess)
+      e = perr(RDB_COLPHYS_ERR_SETUP, "mlp_train: training batches failed");
+    if (e) return e;
+    // epoch losses: the batch means weighted by their sum(w)
     double ls = 0, cs = 0, vs = 0, vc = 0;
+    const float* cnt = lh + nslot;
     for (size_t q = 0; q < nb_train; ++q) ls += (double)lh[q] * cnt[q], cs += cnt[q];
-    for (size_t q = nb_train; q < lh.size(); ++q) vs += (double)lh[q] * cnt[q], vc += cnt[q];
+    for (size_t q = nb_train; q < nslot; ++q) vs += (double)lh[q] * cnt[q], vc += cnt[q];
     if (done < n_hist) {
       if (train_loss) train_loss[done] = cs > 0 ? (float)(ls / cs) : 0.f;
       if (val_loss) val_loss[done] = vc > 0 ? (float)(vs / vc) : 0.f;
@@ -666,7 +678,6 @@ int surr_train(rdb_colphys* h, const rdb_colphys_mlp_config* cfg, float* train_l
     ++done;
   }
   SG_TRY(hipStreamSynchronize(h->stream));
-  SG_TRY(hipFree(dl));
   if (n_done) *n_done = done;
   S->guard_mode = cfg->guard_mode;
   return RDB_COLPHYS_OK;
@@ -707,7 +718,7 @@ int surr_step(rdb_colphys* h, double dt) {
     if (!S->ev[1]) SG_TRY(hipEventCreate(&S->ev[1]));
     SG_TRY(hipEventRecord(S->ev[0], h->stream));
   }
-  int e = net.forward(Mpad, false, true);
+  int e = net.forward(Mpad, true);
   if (e) return perr(e, "surrogate: %s", net.err.c_str());
   if (h->timing) SG_TRY(hipEventRecord(S->ev[1], h->stream));
   double flop = 0.0;
