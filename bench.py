@@ -107,7 +107,7 @@ def surrogate_leg(ocean, cfg, args, plain_value: float) -> dict:
         for s in range(args.train_states):
             if s:
                 cp.upload(synth.redraw(ocean, args.seed + 1000 + s))
-            n = cp.mlp_collect(DT, args.train_steps)
+            n = cp.mlp_collect(DT, args.train_steps, args.train_stride)
             log(f"[bench] surrogate: training state {s}: {n} columns at {time.perf_counter() - t0:.0f} s")
         cp.synchronize()
         t_col = time.perf_counter() - t0
@@ -164,7 +164,8 @@ def surrogate_leg(ocean, cfg, args, plain_value: float) -> dict:
         "training": {"columns": n, "tflops": train_tflops, "mfma_frac": train_tflops / PEAK_BF16_TFLOPS,
                      "source": f"{args.train_steps} plain steps of each of {args.train_states} "
                      f"independent state draws (seeds {args.seed + 1000}..{args.seed + 999 + args.train_states}) "
-                     "on the same grid", "epochs": args.epochs, "batch": 8192,
+                     f"on the same grid, every {args.train_stride}. column of each step",
+                     "epochs": args.epochs, "batch": 8192,
                      "lr": 1e-3, "seconds": t_train, "val_loss_first": vl[0], "val_loss_last": vl[-1]},
         "mlp": {"forward_ms": mlp_ms, "flop_per_step": st["mlp_flop_per_step"], "achieved_tflops": tflops,
                 "peak_tflops": PEAK_BF16_TFLOPS, "mfma_frac": (tflops / PEAK_BF16_TFLOPS) if tflops else None,
@@ -347,9 +348,10 @@ def main() -> int:
     ap.add_argument("--ensemble", action="store_true",
                     help="the 16-member batched ensemble (about 10.5 M columns) instead of the single tile")
     ap.add_argument("--guard", default="surface", choices=["strict", "surface", "range"])
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=120)
     ap.add_argument("--train-steps", type=int, default=2, help="plain steps recorded per training state")
     ap.add_argument("--train-states", type=int, default=6, help="independent state draws the network trains on")
+    ap.add_argument("--train-stride", type=int, default=1, help="record every n-th column of each training step")
     args = ap.parse_args()
 
     if args.copy_probe:

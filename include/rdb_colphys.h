@@ -262,7 +262,7 @@ int rdb_colphys_stats(rdb_colphys *h, rdb_colphys_counters *out);
  * the interior interfaces and the KPP boundary-layer depth from the column
  * state and forcing (T, S, centred u, v as differences to the live level
  * above and log10 h at every level; |tau|, Q_heat, Q_salt, Coriolis f,
- * lagged boundary-layer depth, column depth).
+ * the convective velocity scale of the lagged boundary layer, column depth).
  * BF16 weights and activations on MFMA with FP32 accumulation.  Its kv/kt go
  * through the same assembly clip and FP64 tridiagonal solve as the plain
  * path; the non-local transport uses the predicted depth.  A column falls

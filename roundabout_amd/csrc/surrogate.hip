@@ -3,8 +3,15 @@
 // training loop, guarded inference feeding the FP64 solve, save/load.
 //
 // Features per wet column (5 nz + 6): five per level, then |tau| (+ shelf
-// stress), Q_heat, Q_salt, Coriolis f, the lagged KPP boundary-layer depth
-// and the live column depth.  The five level slots hold the column state
+// stress), Q_heat, Q_salt, Coriolis f, the convective velocity scale of the
+// lagged KPP boundary layer, w* = (max(0, -B0) h_lagged)^(1/3) (B0 the
+// surface buoyancy flux), and the live column depth.  The lagged depth
+// enters KPP only through w*, and only under cooling; given to the network
+// as a raw input it became a persistence shortcut (h_b ~ h_lagged): in a
+// surrogate run the network's own deep guesses fed back step after step
+// while the plain boundary layer shoaled, and the worst 0.1 % of the
+// columns -- all with h_b two to five times the plain one -- held a third
+// of the 24-step temperature error.  The five level slots hold the column state
 // (T, S, centred u, centred v, h) recoded against the live level above:
 //   buoyancy jump  slog(g (rho_k - rho_above) / rho0, 1e-7 m/s^2)  [T slot]
 //   S_k - S_above                                                  [S slot]
@@ -154,14 +161,16 @@ __device__ __forceinline__ void level_slots(const LevelVals& cur, const LevelVal
   }
 }
 
-// forcing feature i (0..5) of packed column p
-__device__ __forceinline__ float forcing_feature(const StateView& s, const double* cor, int p, int i, double depth) {
+// forcing feature i (0..5) of packed column p; the surface buoyancy flux is
+// B0 = b0T Q_heat - b0S Q_salt (kpp_b0 of the fluxes)
+__device__ __forceinline__ float forcing_feature(const StateView& s, const double* cor, int p, int i, double depth,
+                                                 double b0T, double b0S) {
   switch (i) {
     case 0: return (float)s.tau_tot[p];
     case 1: return (float)s.Qh[p];
     case 2: return (float)s.Qs[p];
     case 3: return (float)cor[p];
-    case 4: return (float)s.bl[p];
+    case 4: return (float)cbrt(fmax(0.0, b0S * s.Qs[p] - b0T * s.Qh[p]) * s.bl[p]);
     default: return (float)depth;
   }
 }
@@ -174,11 +183,16 @@ __device__ __forceinline__ float forcing_feature(const StateView& s, const doubl
 // that training and inference features agree bit for bit.
 constexpr int FC = 64, FW = 8, FB = 64 * FW;
 
+// B0 = b0_T Q_heat - b0_S Q_salt: kpp_b0 with qT = Q_heat / (rho0 cp) and
+// qS = Q_salt / rho0
+inline double b0_T(const Params& pr) { return GRAVITY / pr.rho0 * pr.alpha_T / (pr.rho0 * pr.cp); }
+inline double b0_S(const Params& pr) { return GRAVITY / pr.rho0 * pr.beta_S / pr.rho0; }
+
 // Raw features of packed columns c = i * stride (i < ncols) into rows row0 + i.
 // The network has nzn >= nz levels with the surfaces aligned: network level
 // j is this grid's level j - lev_off.
 __global__ void k_features_raw(StateView s, const double* cor, int ncols, int stride, int din, float* X, long long row0,
-                               int nzn, int lev_off, double rho0, double p_ref) {
+                               int nzn, int lev_off, double rho0, double p_ref, double b0T, double b0S) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ncols) return;
   int p = i * stride;
@@ -197,7 +211,7 @@ __global__ void k_features_raw(StateView s, const double* cor, int ncols, int st
   }
   double depth = 0.0;
   for (int w = 0; w < FW; ++w) depth += dpart[w];
-  for (int f = 5 * nzn; f < din; ++f) row[f] = forcing_feature(s, cor, p, f - 5 * nzn, depth);
+  for (int f = 5 * nzn; f < din; ++f) row[f] = forcing_feature(s, cor, p, f - 5 * nzn, depth, b0T, b0S);
 }
 
 // Targets of the same columns after the plain step: log10 kt, log10 kv at
@@ -314,7 +328,7 @@ __global__ void k_norm_targets(const float* T, const float* W, long long n, int 
 __global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double* cor, int n, int din, int ld,
                                                      const float* st, bf16* X0, unsigned char* guard,
                                                      unsigned int* nfall, int guard_mode, double rho0, double p_ref,
-                                                     int nzn, int lev_off) {
+                                                     int nzn, int lev_off, double b0T, double b0S) {
   __shared__ bf16 tile[384][FC + 1];
   __shared__ unsigned char flags[FW][FC];
   __shared__ double sdepth[FW][FC];
@@ -381,7 +395,7 @@ __global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double
     // the forcing features, with the column depth from the groups' sums
     double depth = 0.0;
     for (int w = 0; w < FW; ++w) depth += sdepth[w][c];
-    for (int f = 5 * nzn; f < din; ++f) put(f, forcing_feature(s, cor, p, f - 5 * nzn, depth));
+    for (int f = 5 * nzn; f < din; ++f) put(f, forcing_feature(s, cor, p, f - 5 * nzn, depth, b0T, b0S));
     for (int f = din; f < ld; ++f) tile[f][c] = 0;
     unsigned char g = fl;  // this wave's flags, now with the forcing features' range checks
     for (int w = 1; w < FW; ++w) g |= flags[w][c];
@@ -486,7 +500,8 @@ int surr_collect(rdb_colphys* h, double dt, int n_steps, int stride, long long* 
       const int nc = (m->n_wet + stride - 1) / stride;
       if (nc > 0)
         hipLaunchKernelGGL(k_features_raw, gs(nc, 128), dim3(128), 0, h->stream, m->s, m->d_coriolis, nc, stride,
-                           S->din, S->Xraw, row, nzn, nzn - m->nz, h->pr.rho0, h->pr.p_ref);
+                           S->din, S->Xraw, row, nzn, nzn - m->nz, h->pr.rho0, h->pr.p_ref, b0_T(h->pr),
+                           b0_S(h->pr));
       row += nc;
     }
     st = launch_step_all(h, dt, false);
@@ -710,7 +725,7 @@ int surr_step(rdb_colphys* h, double dt) {
     if (m->n_wet == 0) continue;
     hipLaunchKernelGGL(k_features_infer, gs(m->n_wet, FC), dim3(FB), 0, h->stream, m->s, m->d_coriolis, m->n_wet,
                        S->din, ld, S->d_stats, net.X0 + (size_t)m->row_off * (size_t)ld, S->guard + m->row_off,
-                       S->nfall, S->guard_mode, h->pr.rho0, h->pr.p_ref, nzn, nzn - m->nz);
+                       S->nfall, S->guard_mode, h->pr.rho0, h->pr.p_ref, nzn, nzn - m->nz, b0_T(h->pr), b0_S(h->pr));
   }
   SG_TRY(hipGetLastError());
   if (h->timing) {
@@ -785,7 +800,7 @@ int surr_save(rdb_colphys* h, const char* path) {
   if (!f) return perr(RDB_COLPHYS_ERR_IO, "mlp_save: cannot open %s", path);
   // version 3: buoyancy-jump / signed-log level encoding of the features,
   // logarithms in single precision
-  const char magic[8] = {'R', 'D', 'B', 'S', 'U', 'R', 'R', '3'};
+  const char magic[8] = {'R', 'D', 'B', 'S', 'U', 'R', 'R', '4'};
   int hdr[4] = {S->nz, S->din, S->dout, (int)S->widths.size()};
   bool ok = std::fwrite(magic, 1, 8, f) == 8 && std::fwrite(hdr, 4, 4, f) == 4 &&
             std::fwrite(S->widths.data(), 4, S->widths.size(), f) == S->widths.size() &&
@@ -801,7 +816,7 @@ int surr_load(rdb_colphys* h, const char* path) {
   if (!f) return perr(RDB_COLPHYS_ERR_IO, "mlp_load: cannot open %s", path);
   char magic[8];
   int hdr[4];
-  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "RDBSURR3", 8) == 0 && std::fread(hdr, 4, 4, f) == 4;
+  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "RDBSURR4", 8) == 0 && std::fread(hdr, 4, 4, f) == 4;
   if (!ok || hdr[0] != net_levels(h) || hdr[3] < 2 || hdr[3] > 16) {
     std::fclose(f);
     return perr(RDB_COLPHYS_ERR_BAD_SHAPE, "mlp_load: %s is not a network for nz = %d", path, net_levels(h));
