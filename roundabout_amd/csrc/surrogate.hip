@@ -107,27 +107,29 @@ struct LevelVals {
 };
 
 // level k of packed column p; levels outside [0, nz) do not exist in this
-// grid and read as a bed filler (h = 1e-4 m, not live)
-// (the loads and the EOS do not depend on liveness, so that a walk over
-// levels has no branches and keeps several levels' loads in flight; a
-// vanished level's values are never used)
+// grid and read as a bed filler (h = 1e-4 m, not live).
 // (split in the memory part, load_raw, and the arithmetic, finish_level,
-// so that a walk can have the next level's loads in flight while it
-// computes the current one)
+// so that a walk can have the next levels' loads in flight while it
+// computes the current one; load_raw always loads -- from the nearest level
+// that exists -- and finish_level substitutes the filler, so that the loads
+// never sit behind a branch, which made the compiler wait for every load in
+// flight; the loads and the EOS do not depend on liveness, and a vanished
+// level's values are never used)
 __device__ __forceinline__ LevelVals load_raw(const StateView& s, int p, int k) {
-  LevelVals v{1.0e-4, 0.0, 0.0, 0.0, 0.0, 0.0, false};
-  if (k < 0 || k >= s.nz) return v;
   const int P = s.P;
-  v.h = LV(s.h, k);
-  v.live = true;  // "exists"; finish_level turns it into "live"
-  v.T = LV(s.T, k);
-  v.S = LV(s.S, k);
-  v.U = LV(s.ub, k);
-  v.V = LV(s.vb, k);
+  const int kc = min(max(k, 0), s.nz - 1);
+  LevelVals v;
+  v.h = LV(s.h, kc);
+  v.T = LV(s.T, kc);
+  v.S = LV(s.S, kc);
+  v.U = LV(s.ub, kc);
+  v.V = LV(s.vb, kc);
+  v.rho = 0.0;
+  v.live = k >= 0 && k < s.nz;  // "exists"; finish_level turns it into "live"
   return v;
 }
 __device__ __forceinline__ LevelVals finish_level(LevelVals v, double rho0, double p_ref) {
-  if (!v.live) return v;  // a level this grid does not have
+  if (!v.live) return LevelVals{1.0e-4, 0.0, 0.0, 0.0, 0.0, 0.0, false};  // a level this grid does not have
   v.live = v.h > H_VANISHED;
   v.rho = eos_wright<double>(v.T, v.S, v.h, rho0, p_ref);
   return v;
@@ -332,11 +334,18 @@ __global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double
   __shared__ bf16 tile[384][FC + 1];
   __shared__ unsigned char flags[FW][FC];
   __shared__ double sdepth[FW][FC];
-  const float* mean = st;
-  const float* stdv = st + din;
-  const float* mn = st + 2 * din;
-  const float* mx = st + 3 * din;
+  // the normalisation statistics (mean, std, min, max per feature), staged
+  // in LDS once: read from global memory inside the level walk, each of
+  // their loads waited with vmcnt(0) -- which also drained the next levels'
+  // loads in flight, leaving the walk latency-bound
+  __shared__ float sst[4 * 384];
   const int tid = threadIdx.x, c = tid & (FC - 1), part = tid >> 6;
+  for (int i = tid; i < 4 * din; i += FB) sst[i] = st[i];
+  __syncthreads();
+  const float* mean = sst;
+  const float* stdv = sst + din;
+  const float* mn = sst + 2 * din;
+  const float* mx = sst + 3 * din;
   const int p = blockIdx.x * FC + c;
   const int P = s.P, nz = s.nz;
   unsigned char fl = 0;
@@ -350,14 +359,16 @@ __global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double
     LevelVals above = load_level(s, p, j1 - lev_off, rho0, p_ref);  // the level above this part's highest
     bool unstable = false;
     double dpart = 0.0;  // this group's live thickness, summed from its top down
-    // the next two levels' loads stay in flight while a level is computed
-    LevelVals nxt = load_raw(s, p, j1 - 1 - lev_off);
-    LevelVals nxt2 = load_raw(s, p, j1 - 2 - lev_off);
-    for (int j = j1 - 1; j >= j0; --j) {
-      const LevelVals raw = nxt;
-      nxt = nxt2;
-      if (j - 2 >= j0) nxt2 = load_raw(s, p, j - 2 - lev_off);
+    // the next two levels' loads stay in flight while a level is computed:
+    // three named slots, the walk unrolled by three, so that a level's
+    // values stay in their slot's registers (rotating them through copies
+    // made the compiler wait for every load in flight at each step)
+    // (the loads are unconditional -- past the group's lowest level they
+    // read the next group's levels, which its own wave loads as well)
+    auto step = [&](int j, const LevelVals& raw, LevelVals& ahead) {
+      ahead = load_raw(s, p, j - 2 - lev_off);
       __builtin_amdgcn_sched_barrier(0);
+      if (j < j0) return;
       const LevelVals cur = finish_level(raw, rho0, p_ref);
       float o[5];
       level_slots(cur, above, rho0, o);
@@ -368,6 +379,14 @@ __global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double
       if (guard_mode == RDB_COLPHYS_GUARD_STRICT && cur.live && above.live && above.rho > cur.rho) unstable = true;
       if (cur.live) dpart += cur.h;
       above = cur;
+    };
+    LevelVals sa = load_raw(s, p, j1 - 1 - lev_off);
+    LevelVals sb = load_raw(s, p, j1 - 2 - lev_off);
+    LevelVals sc;
+    for (int j = j1 - 1; j >= j0; j -= 3) {
+      step(j, sa, sc);
+      step(j - 1, sb, sa);
+      step(j - 2, sc, sb);
     }
     sdepth[part][c] = dpart;
     if (part == FW - 1 && guard_mode == RDB_COLPHYS_GUARD_SURFACE) {
