@@ -348,7 +348,7 @@ def main() -> int:
     ap.add_argument("--ensemble", action="store_true",
                     help="the 16-member batched ensemble (about 10.5 M columns) instead of the single tile")
     ap.add_argument("--guard", default="surface", choices=["strict", "surface", "range"])
-    ap.add_argument("--epochs", type=int, default=120)
+    ap.add_argument("--epochs", type=int, default=240)
     ap.add_argument("--train-steps", type=int, default=2, help="plain steps recorded per training state")
     ap.add_argument("--train-states", type=int, default=6, help="independent state draws the network trains on")
     ap.add_argument("--train-stride", type=int, default=1, help="record every n-th column of each training step")
