@@ -14,12 +14,18 @@ enum { EPI_F32 = 0, EPI_BIAS_F32 = 1, EPI_BIAS_GELU = 2, EPI_BIAS_F32_T = 3, EPI
 // pout x pin, k contiguous) and FP32 biases, padded widths.
 constexpr int FP_MAXL = 8;
 constexpr int FP_MAXK = 384;
-constexpr int FP_PITCH = FP_MAXK + 8;  // LDS row pitch (bf16): rows start 4 banks apart
-constexpr int FP_MAXN = 256;           // widest layer output
-constexpr int FP_WAVES = 8;            // wavefronts per workgroup
-// dynamic LDS of k_mlp_fused (the activation tile); three weight slices of
-// FP_MAXN x 32 bf16 (48 KB) are static LDS on top
-constexpr size_t FP_LDS_BYTES = (size_t)128 * FP_PITCH * 2;
+constexpr int FP_MAXN = 256;  // widest layer output
+constexpr int FP_WAVES = 8;   // wavefronts per workgroup
+// LDS row pitches (bf16) of the activation tile's two regions: A holds
+// columns 0..255, B the first layer's input columns 256..383; rows start 4
+// banks apart
+constexpr int FA_PITCH = 256 + 8;
+constexpr int FB_PITCH = 128 + 8;
+// dynamic LDS of k_mlp_fused (the activation tile; region B must also hold
+// two weight slices of FP_MAXN x 32 bf16); three slices (48 KB) are static
+// LDS on top
+constexpr size_t FP_LDS_BYTES = (size_t)128 * (FA_PITCH + FB_PITCH) * 2;
+static_assert(128 * FB_PITCH >= 2 * FP_MAXN * 32, "region B holds two weight slices");
 struct FusedNet {
   const bf16* W[FP_MAXL];
   const float* b[FP_MAXL];

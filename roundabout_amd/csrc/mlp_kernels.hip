@@ -24,23 +24,22 @@ __device__ __forceinline__ float bf2f(bf16 h) {
   uint32_t u = (uint32_t)h << 16;
   return __uint_as_float(u);
 }
-// round to nearest even (no NaN inputs reach it in this path)
-__device__ __forceinline__ bf16 f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (bf16)(u >> 16);
-}
+// round to nearest even: gfx950's v_cvt_pk_bf16_f32 (one instruction; the
+// integer form u += 0x7fff + ((u >> 16) & 1) gives the same bits for the
+// finite values of this path and took four)
+__device__ __forceinline__ bf16 f2bf(float f) { return __builtin_bit_cast(bf16, (__bf16)f); }
 
 // GELU, tanh form: 0.5 x (1 + tanh(u)), u = sqrt(2/pi) (x + 0.044715 x^3),
 // evaluated as x * sigma(2 u) with sigma(z) = 1 / (1 + e^-z) (the identity
-// 0.5 (1 + tanh(u)) = sigma(2 u)): one hardware exp and one divide.  Large
-// |u| saturates correctly (e^-2u -> inf gives -0, -> 0 gives x).
+// 0.5 (1 + tanh(u)) = sigma(2 u)): one hardware exp and one hardware
+// reciprocal (1 ulp; the result is rounded to BF16).  Large |u| saturates
+// correctly (e^-2u -> inf gives rcp(inf) = 0, -> 0 gives x).
 __device__ __forceinline__ float gelu_sigma(float x, float* u_prime) {
   const float c = 0.7978845608028654f;
   const float x2 = x * x;
   const float u = c * (x + 0.044715f * x2 * x);
   if (u_prime) *u_prime = c * (1.0f + 3.0f * 0.044715f * x2);
-  return 1.0f / (1.0f + __expf(-2.0f * u));
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
 }
 __device__ __forceinline__ float gelu(float x) { return x * gelu_sigma(x, nullptr); }
 // d/dx [x sigma(2u)] = sigma + 2 x sigma (1 - sigma) u'
@@ -59,7 +58,9 @@ __device__ __forceinline__ void wait_vm_except(int n) {
     case 1: __builtin_amdgcn_s_waitcnt(0xF71); break;
     case 2: __builtin_amdgcn_s_waitcnt(0xF72); break;
     case 3: __builtin_amdgcn_s_waitcnt(0xF73); break;
-    default: __builtin_amdgcn_s_waitcnt(0xF74); break;  // 4
+    case 4: __builtin_amdgcn_s_waitcnt(0xF74); break;
+    case 5: __builtin_amdgcn_s_waitcnt(0xF75); break;
+    default: __builtin_amdgcn_s_waitcnt(0xF76); break;  // 6
   }
 }
 
@@ -353,100 +354,149 @@ template __global__ void k_gemm_bf16<EPI_MSE>(GemmArgs);
 template __global__ void k_gemm_bf16<EPI_F32>(GemmArgs);
 template __global__ void k_gemm_pair<EPI_F32, EPI_GELU_BWD>(GemmArgs, GemmArgs);
 
+// Eight 16-byte LDS reads (MFMA fragments) and the wait for them, as one
+// asm statement (for the same reason as lds_read4).
+__device__ __forceinline__ void lds_read8(const bf16* const (&p)[8], bf16x8 (&x)[8]) {
+  typedef __attribute__((ext_vector_type(4))) unsigned u32x4;
+  auto lds = [](const bf16* a) {
+    return (unsigned)(uintptr_t)(const __attribute__((address_space(3))) bf16*)a;
+  };
+  u32x4 v[8];
+  asm volatile(
+      "ds_read_b128 %0, %8\n\t"
+      "ds_read_b128 %1, %9\n\t"
+      "ds_read_b128 %2, %10\n\t"
+      "ds_read_b128 %3, %11\n\t"
+      "ds_read_b128 %4, %12\n\t"
+      "ds_read_b128 %5, %13\n\t"
+      "ds_read_b128 %6, %14\n\t"
+      "ds_read_b128 %7, %15\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]), "=&v"(v[6]), "=&v"(v[7])
+      : "v"(lds(p[0])), "v"(lds(p[1])), "v"(lds(p[2])), "v"(lds(p[3])), "v"(lds(p[4])), "v"(lds(p[5])),
+        "v"(lds(p[6])), "v"(lds(p[7]))
+      : "memory");
+#pragma unroll
+  for (int i = 0; i < 8; ++i) x[i] = __builtin_bit_cast(bf16x8, v[i]);
+}
+
 // One layer of the fused kernel for a wave owning rows [row0, row0 + 64) of
 // the block and NTW 16-wide output tiles from column nw0:
 // acc[i][j] (rows row0 + 16 i.., columns nw0 + 16 j..) = act * W^T.
 //
-// W streams through LDS in k-slices of 32 (N rows x 64 B) in 3 buffers, with the direct global-to-LDS load (global_load_lds_dwordx4:
-// lane l of a wave writes its 16 bytes at the instruction's LDS base + 16 l),
-// two slices ahead of the MFMAs that use them.  Within a
-// slice, the 16-byte chunk q of row n sits at slot 4 n + ((q + n / 4) mod 4):
-// the 16 rows of one MFMA operand then start in 16 different bank groups.
-// NTW is a compile-time constant, so the loop body has no branches; the
-// scheduling barriers keep each k-step as: fragment reads, MFMAs, wait.
-template <int NTW>
+// The layer input is the block's LDS tile: columns 0..255 in region A (row
+// pitch FA_PITCH), columns 256.. in region B (pitch FB_PITCH).  W streams
+// through LDS in k-slices of 32 (N rows x 64 B) in NB buffers, NB - 1
+// slices ahead of the MFMAs that use them, with the direct global-to-LDS
+// load (global_load_lds_dwordx4: lane l of a wave writes its 16 bytes at the
+// instruction's LDS base + 16 l).  Within a slice, the 16-byte chunk q of
+// row n sits at slot 4 n + ((q + n / 4) mod 4): the 16 rows of one MFMA
+// operand then start in 16 different bank groups.  Each k-step issues the
+// slice NB - 1 ahead, reads all fragments (one asm statement), runs the MFMAs,
+// then waits for the next slice only and meets the other waves at a bare
+// s_barrier.  NTW and NB are compile-time constants; the loop is unrolled
+// by the buffer rotation.
+// This wave's share of the k-slice [k0, k0 + 32) of W (N x K) into an LDS
+// slice buffer: N / 16 row groups of 16 rows (1 KB of LDS, one instruction
+// each), group g by wave g mod FP_WAVES; lane l loads row g * 16 + l / 4
+// into LDS slot l % 4, which holds global chunk (l % 4 - l / 16) mod 4.
+__device__ __forceinline__ void load_w_slice(const bf16* __restrict__ W, int K, int N, int wave, int lane, int k0,
+                                             bf16* dst) {
+  const int lrow = lane >> 2;
+  const int lchunk = ((lane & 3) - (lane >> 4) + 4) & 3;
+  for (int g = wave; g < N / 16; g += FP_WAVES) {
+    const bf16* src = W + (size_t)(g * 16 + lrow) * (size_t)K + k0 + lchunk * 8;
+    __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)(dst + g * 512), 16,
+                                     0, 0);
+  }
+}
+
+// The first NB - 1 slices of a layer (issued before the previous layer's
+// epilogue, so that they arrive while it runs).
+template <int NB>
+__device__ __forceinline__ void fused_prefetch(const bf16* __restrict__ W, int K, int N, int wave, int lane,
+                                               bf16* const (&buf)[NB]) {
+#pragma unroll
+  for (int u = 0; u < NB - 1; ++u)
+    if (u < K / 32) load_w_slice(W, K, N, wave, lane, 32 * u, buf[u]);
+}
+
+template <int NTW, int NB>
 __device__ __forceinline__ void fused_layer(const bf16* __restrict__ W, int K, int N, int wave, int lane, int row0,
-                                            int nw0, const bf16* act, bf16* w0, bf16* w1, bf16* w2,
+                                            int nw0, const bf16* actA, const bf16* actB, bf16* const (&buf)[NB],
                                             f32x4 (&acc)[4][4]) {
   const int r = lane & 15, q = lane >> 4;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < NTW; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  // a slice is N / 16 row groups of 16 rows (1 KB of LDS, one instruction
-  // per wave each); lane l loads row g * 16 + l / 4 into LDS slot l % 4,
-  // which holds global chunk (l % 4 - l / 16) mod 4
-  const int lrow = lane >> 2;
-  const int lchunk = ((lane & 3) - (lane >> 4) + 4) & 3;
   const int groups = N / 16;
-  auto load_slice = [&](int k0, bf16* buf) {
-    for (int g = wave; g < groups; g += FP_WAVES) {
-      const bf16* src = W + (size_t)(g * 16 + lrow) * (size_t)K + k0 + lchunk * 8;
-      __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)(buf + g * 512), 16,
-                                       0, 0);
-    }
-  };
   // this wave's loads per slice (the vmcnt a slice still in flight holds)
   const int per_wave = wave < groups ? (groups - wave + FP_WAVES - 1) / FP_WAVES : 0;
   const int nsteps = K / 32;
-  const bf16* arow = act + (row0 + r) * FP_PITCH + q * 8;
-  // k-step t: slice t from `cur`; slice t + 2 starts into `ahead_buf`
+  auto load_slice = [&](int k0, bf16* dst) { load_w_slice(W, K, N, wave, lane, k0, dst); };
+  // k-step t: slice t from `cur`; slice t + NB - 1 starts into `ahead_buf`
+  // (the buffer every wave finished reading before the barrier of step t - 1)
   auto step = [&](int t, const bf16* cur, bf16* ahead_buf) {
     const int k0 = 32 * t;
-    const bool ahead = t + 2 < nsteps;
-    if (ahead) load_slice(k0 + 64, ahead_buf);
-    bf16x8 a[4], b[NTW];
+    if (t + NB - 1 < nsteps) load_slice(k0 + 32 * (NB - 1), ahead_buf);
+    const bool inA = k0 < 256;
+    const int ap = inA ? FA_PITCH : FB_PITCH;
+    const bf16* arow = (inA ? actA + k0 : actB + (k0 - 256)) + (row0 + r) * ap + q * 8;
+    const bf16* pf[8];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) a[i] = *(const bf16x8*)(arow + 16 * i * FP_PITCH + k0);
+    for (int i = 0; i < 4; ++i) pf[i] = arow + 16 * i * ap;
 #pragma unroll
-    for (int j = 0; j < NTW; ++j) {
-      const int n = nw0 + 16 * j + r;
-      b[j] = *(const bf16x8*)(cur + (4 * n + ((q + (n >> 2)) & 3)) * 8);
+    for (int j = 0; j < 4; ++j) {
+      const int n = nw0 + 16 * (j < NTW ? j : NTW - 1) + r;  // (unused reads repeat the last tile)
+      pf[4 + j] = cur + (4 * n + ((q + (n >> 2)) & 3)) * 8;
     }
+    bf16x8 x[8];
+    lds_read8(pf, x);
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < NTW; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < NTW; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x[i], x[4 + j], acc[i][j], 0, 0, 0);
     __builtin_amdgcn_sched_barrier(0);
-    // slice t + 1 has landed (slice t + 2 may still be in flight); the
-    // barrier also frees `cur` for slice t + 3.  A bare s_barrier: the loop
-    // writes no LDS itself, the reads of `cur` are complete (their values
-    // fed the MFMAs) and the direct loads are covered by the wait above, so
-    // the full fence of __syncthreads -- which would drain slice t + 2 as
-    // well -- is not needed.
-    wait_vm_except(ahead ? per_wave : 0);
+    // slice t + 1 has landed (slices t + 2 .. t + NB - 1 may still be in
+    // flight); the barrier also frees `cur` for slice t + NB.  A bare
+    // s_barrier: the loop writes no LDS itself, the reads of `cur` are
+    // complete (their values fed the MFMAs) and the direct loads are covered
+    // by the wait above.
+    const int newer = min(NB - 2, nsteps - 2 - t);
+    wait_vm_except(newer > 0 ? newer * per_wave : 0);
     __builtin_amdgcn_s_barrier();
   };
-  load_slice(0, w0);
-  if (nsteps > 1) load_slice(32, w1);
-  wait_vm_except(nsteps > 1 ? per_wave : 0);  // slice 0 has landed
-  __syncthreads();
-  // unrolled by the buffer rotation, so that every LDS access names its
-  // buffer and the fragment reads need not wait for the loads in flight
-  for (int t = 0; t < nsteps; t += 3) {
-    step(t, w0, w2);
-    if (t + 1 < nsteps) step(t + 1, w1, w0);
-    if (t + 2 < nsteps) step(t + 2, w2, w1);
+  // slices 0 .. NB - 2 were issued by fused_prefetch
+  {
+    const int newer = min(NB - 2, nsteps - 1);
+    wait_vm_except(newer > 0 ? newer * per_wave : 0);  // slice 0 has landed
+  }
+  __builtin_amdgcn_s_barrier();
+  for (int t = 0; t < nsteps; t += NB) {
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+      if (t + u < nsteps) step(t + u, buf[u], buf[(u + NB - 1) % NB]);
   }
 }
 
-// Epilogue of one layer: GELU(acc + b) to the block's LDS input (hidden
-// layers), or acc + b to Y (the last layer).
+// Epilogue of one layer: GELU(acc + b) to region A of the block's LDS tile
+// (hidden layers, N <= 256), or acc + b to Y (the last layer).
 template <int NTW>
-__device__ __forceinline__ void fused_epilogue(const f32x4 (&acc)[4][4], const float* __restrict__ bias, int row0,
-                                               int nw0, int r, int q, bf16* act, bool last, float* __restrict__ Y,
+__device__ __forceinline__ void fused_epilogue(const f32x4 (&acc)[4][4], const float (&bnv)[4], int row0, int nw0,
+                                               int r, int q, bf16* actA, bool last, float* __restrict__ Y,
                                                long long ldy, long long m0, int out_major) {
 #pragma unroll
   for (int j = 0; j < NTW; ++j) {
     const int n = nw0 + 16 * j + r;
-    const float bn = bias[n];
+    const float bn = bnv[j];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int mr = row0 + 16 * i + 4 * q;  // rows mr .. mr + 3 of the block
       if (!last) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) act[(mr + e) * FP_PITCH + n] = f2bf(gelu(acc[i][j][e] + bn));
+        for (int e = 0; e < 4; ++e) actA[(mr + e) * FA_PITCH + n] = f2bf(gelu(acc[i][j][e] + bn));
       } else if (out_major) {
         f32x4 v = acc[i][j];
         v[0] += bn;
@@ -462,66 +512,113 @@ __device__ __forceinline__ void fused_epilogue(const f32x4 (&acc)[4][4], const f
   }
 }
 
+template <int NB>
+__device__ __forceinline__ void fused_layer_n(int ntw, const bf16* W, int K, int N, int wave, int lane, int row0,
+                                              int nw0, const bf16* actA, const bf16* actB, bf16* const (&buf)[NB],
+                                              f32x4 (&acc)[4][4]) {
+  switch (ntw) {
+    case 4: fused_layer<4, NB>(W, K, N, wave, lane, row0, nw0, actA, actB, buf, acc); break;
+    case 3: fused_layer<3, NB>(W, K, N, wave, lane, row0, nw0, actA, actB, buf, acc); break;
+    case 2: fused_layer<2, NB>(W, K, N, wave, lane, row0, nw0, actA, actB, buf, acc); break;
+    default: fused_layer<1, NB>(W, K, N, wave, lane, row0, nw0, actA, actB, buf, acc); break;
+  }
+}
+
 // The whole network on a block of 128 rows, activations resident in LDS.
 //
-// Workgroup of 8 wavefronts: the block's layer input (128 x pin bf16, row
-// pitch FP_PITCH) sits in LDS; wave w owns rows 64 (w / 4) .. + 64 and
-// output columns [(w % 4) pout / 4, (w % 4 + 1) pout / 4), two waves per
-// SIMD.  The inputs are staged with direct-to-LDS loads, all in flight at
-// once; W streams through LDS (fused_layer).  After a layer's k-loop the
-// block overwrites its LDS input with GELU(acc + b) in bf16; the last layer
-// adds the bias and stores FP32 to Y, row-major (Y[m ldy + n]) or
-// output-major (Y[n ldy + m]).  HBM sees only the input features and the
-// outputs: no activation round trips between layers.
+// Workgroup of 8 wavefronts: the block's layer input sits in LDS -- columns
+// 0..255 in region A (128 x FA_PITCH bf16), input columns 256..383 of the
+// first layer in region B (128 x FB_PITCH); wave w owns rows
+// 64 (w / 4) .. + 64 and output columns [(w % 4) pout / 4, (w % 4 + 1) pout
+// / 4), two waves per SIMD.  The inputs are staged with direct-to-LDS loads,
+// all in flight at once.  W streams through LDS (fused_layer): three static
+// slice buffers while region B holds layer input, five -- the two extra in
+// region B -- once it does not, so that the layers of width 256 keep four
+// slices in flight (the weight-slice latency from L2 bounds the kernel).
+// After a layer's k-loop the block overwrites region A with GELU(acc + b)
+// in bf16; the last layer adds the bias and stores FP32 to Y, row-major
+// (Y[m ldy + n]) or output-major (Y[n ldy + m]).  HBM sees only the input
+// features and the outputs: no activation round trips between layers.
 // Limits (checked by the host): pin <= FP_MAXK, pout <= FP_MAXN, all
 // multiples of 64; rows a multiple of 128.
 __global__ void __launch_bounds__(64 * FP_WAVES) k_mlp_fused(FusedNet fn, const bf16* __restrict__ X0, int ldx,
                                                              float* __restrict__ Y, long long ldy, int out_major) {
-  extern __shared__ __align__(16) bf16 act[];  // 128 x FP_PITCH (dynamic LDS)
-  // weight slices: separate objects from `act`, so that reading one never
-  // has to wait for the direct loads into another
+  extern __shared__ __align__(16) bf16 act[];  // region A then region B (dynamic LDS)
   __shared__ __align__(16) bf16 w0[FP_MAXN * 32], w1[FP_MAXN * 32], w2[FP_MAXN * 32];
+  // every layer's bias, staged once: read from global memory at the
+  // epilogue, its loads sat behind the prefetched slices in the in-order
+  // vmcnt and the compiler drained them all
+  __shared__ float sbias[FP_MAXL * FP_MAXN];
+  bf16* actA = act;
+  bf16* actB = act + 128 * FA_PITCH;
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;  // wave index in a scalar register
   const int r = lane & 15, q = lane >> 4;
   const long long m0 = (long long)blockIdx.x * 128;
   {
-    // the block's LDS image (128 rows x FP_PITCH) in 1 KB pieces of 64
-    // 16-byte chunks; the pad chunks at the end of each row re-read the
-    // row's first chunk (never used)
-    constexpr int CPR = FP_PITCH / 8;  // chunks per LDS row
-    constexpr int NCH = 128 * CPR;     // a multiple of 64
-    for (int c0 = wave * 64; c0 < NCH; c0 += 64 * FP_WAVES) {
-      const int c = c0 + lane;
-      const int row = c / CPR, col = (c % CPR) * 8;
-      const bf16* src = X0 + (size_t)(m0 + row) * (size_t)ldx + (col < ldx ? col : 0);
-      __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)(act + c0 * 8), 16,
-                                       0, 0);
-    }
-    __builtin_amdgcn_s_waitcnt(0);
+    // the block's LDS image, region by region, in 1 KB pieces of 64 16-byte
+    // chunks; the pad chunk at the end of each row re-reads the row's first
+    // chunk of that region (never used)
+    auto stage = [&](bf16* dst, int pitch, int col0, int ncol) {
+      const int cpr = pitch / 8;  // chunks per LDS row
+      const int nch = 128 * cpr;  // a multiple of 64
+      for (int c0 = wave * 64; c0 < nch; c0 += 64 * FP_WAVES) {
+        const int c = c0 + lane;
+        const int row = c / cpr, col = (c % cpr) * 8;
+        const bf16* src = X0 + (size_t)(m0 + row) * (size_t)ldx + col0 + (col < ncol ? col : 0);
+        __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)(dst + c0 * 8),
+                                         16, 0, 0);
+      }
+    };
+    stage(actA, FA_PITCH, 0, min(ldx, 256));
+    if (ldx > 256) stage(actB, FB_PITCH, 256, ldx - 256);
   }
-  __syncthreads();
   const int row0 = 64 * (wave >> 2);
+  bf16* const b3[3] = {w0, w1, w2};
+  bf16* const b5[5] = {w0, w1, w2, actB, actB + FP_MAXN * 32};
+  // a layer whose input reaches into region B streams W through three
+  // buffers, any other through five
+  auto prefetch = [&](int l) {
+    if (fn.pin[l] > 256)
+      fused_prefetch<3>(fn.W[l], fn.pin[l], fn.pout[l], wave, lane, b3);
+    else
+      fused_prefetch<5>(fn.W[l], fn.pin[l], fn.pout[l], wave, lane, b5);
+  };
+  prefetch(0);
+  for (int l = 0; l < fn.nl; ++l)
+    for (int n = tid; n < fn.pout[l]; n += 64 * FP_WAVES) sbias[l * FP_MAXN + n] = fn.b[l][n];
+  __builtin_amdgcn_s_waitcnt(0);  // the inputs (and the first slices) have landed
+  __syncthreads();
   for (int l = 0; l < fn.nl; ++l) {
     const int K = fn.pin[l], N = fn.pout[l];
     const int ntw = N / 64;                // 16-wide n-tiles per wave (1..4)
     const int nw0 = (wave & 3) * (N / 4);  // this wave's first output column
     const bool last = (l == fn.nl - 1);
     f32x4 acc[4][4];
+    if (K > 256)
+      fused_layer_n<3>(ntw, fn.W[l], K, N, wave, lane, row0, nw0, actA, actB, b3, acc);
+    else
+      fused_layer_n<5>(ntw, fn.W[l], K, N, wave, lane, row0, nw0, actA, actB, b5, acc);
+    // this wave's bias values, while no load is in flight
+    float bn[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bn[j] = j < ntw ? sbias[l * FP_MAXN + nw0 + 16 * j + r] : 0.f;
+    // the k-loop ended with a barrier: every wave has read this layer's
+    // input and the slice buffers are free, so the next layer's first
+    // slices go out now and arrive during the epilogue
+    if (!last) prefetch(l + 1);
     switch (ntw) {
-      case 4: fused_layer<4>(fn.W[l], K, N, wave, lane, row0, nw0, act, w0, w1, w2, acc); break;
-      case 3: fused_layer<3>(fn.W[l], K, N, wave, lane, row0, nw0, act, w0, w1, w2, acc); break;
-      case 2: fused_layer<2>(fn.W[l], K, N, wave, lane, row0, nw0, act, w0, w1, w2, acc); break;
-      default: fused_layer<1>(fn.W[l], K, N, wave, lane, row0, nw0, act, w0, w1, w2, acc); break;
+      case 4: fused_epilogue<4>(acc, bn, row0, nw0, r, q, actA, last, Y, ldy, m0, out_major); break;
+      case 3: fused_epilogue<3>(acc, bn, row0, nw0, r, q, actA, last, Y, ldy, m0, out_major); break;
+      case 2: fused_epilogue<2>(acc, bn, row0, nw0, r, q, actA, last, Y, ldy, m0, out_major); break;
+      default: fused_epilogue<1>(acc, bn, row0, nw0, r, q, actA, last, Y, ldy, m0, out_major); break;
     }
-    // (the layer ends with a barrier: every wave has read this layer's input)
-    switch (ntw) {
-      case 4: fused_epilogue<4>(acc, fn.b[l], row0, nw0, r, q, act, last, Y, ldy, m0, out_major); break;
-      case 3: fused_epilogue<3>(acc, fn.b[l], row0, nw0, r, q, act, last, Y, ldy, m0, out_major); break;
-      case 2: fused_epilogue<2>(acc, fn.b[l], row0, nw0, r, q, act, last, Y, ldy, m0, out_major); break;
-      default: fused_epilogue<1>(acc, fn.b[l], row0, nw0, r, q, act, last, Y, ldy, m0, out_major); break;
-    }
-    __syncthreads();  // the next layer's input is complete
+    // the next layer's input is complete: this wave's LDS writes are done
+    // (lgkmcnt(0); vmcnt and expcnt left at their maximum) and the barrier
+    // meets the others -- not __syncthreads, whose vmcnt(0) would drain the
+    // prefetched slices
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_s_barrier();
   }
 }
 
