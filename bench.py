@@ -91,6 +91,25 @@ def live_rms(ocean, a, b) -> float:
     return float(np.sqrt(np.mean(d * d)))
 
 
+def train_in_chunks(cp, epochs: int, guard: int, chunk: int = 240) -> tuple[list, list]:
+    """Train `epochs` epochs in calls of at most `chunk` epochs, logging after
+    each: a call keeps the network and its Adam moments, so the calls
+    continue one training (each call draws the same held-out split and
+    restarts the epoch shuffles from the configured seed)."""
+    tl, vl, done = [], [], 0
+    while done < epochs or not tl:
+        n = min(chunk, epochs - done)
+        a, b = cp.mlp_train(epochs=n, guard_mode=guard)
+        tl += a
+        vl += b
+        done += n
+        if epochs > chunk:
+            log(f"[bench] surrogate: {done}/{epochs} epochs, val loss {vl[-1]:.5f}")
+        if n == 0:
+            break
+    return tl, vl
+
+
 def surrogate_leg(ocean, cfg, args, plain_value: float) -> dict:
     """The learned closure on the same tile: train on the GPU on columns the
     plain path generates from an independent state draw on the same grid,
@@ -112,7 +131,7 @@ def surrogate_leg(ocean, cfg, args, plain_value: float) -> dict:
         cp.synchronize()
         t_col = time.perf_counter() - t0
         t1 = time.perf_counter()
-        tl, vl = cp.mlp_train(epochs=args.epochs, guard_mode=guard)
+        tl, vl = train_in_chunks(cp, args.epochs, guard)
         cp.synchronize()
         t_train = time.perf_counter() - t1
         cp.mlp_save(net_path)
