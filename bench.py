@@ -377,8 +377,9 @@ def main() -> int:
         copy_probe()
         return 0
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the column path shards by columns "
-                          "but no multi-GPU run is part of it", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the benchmark runs on one GPU: the column path shards by columns "
+                          "(independent tiles, no halo), and a multi-GPU run is not implemented",
+                          "n_gpus": args.gpus}))
         return 2
 
     if args.ensemble:
