@@ -36,10 +36,11 @@ void gemm(hipStream_t s, const GemmArgs& g) {
 }
 inline GemmEpi epi() { return GemmEpi{}; }
 
-// k-splits of a weight-gradient product over Mpad batch rows: at most 32
-// chunks of at least 64 rows, chunk length a multiple of 32
+// k-splits of a weight-gradient product over Mpad batch rows: at most 16
+// chunks of at least 128 rows, chunk length a multiple of 32 (32 chunks
+// doubled the partials k_grad_step reads for no faster product)
 inline void batch_splits(int Mpad, int* S, int* kc) {
-  *S = std::max(1, std::min(32, Mpad / 64));
+  *S = std::max(1, std::min(16, Mpad / 128));
   *kc = round_up((Mpad + *S - 1) / *S, 32);
 }
 
