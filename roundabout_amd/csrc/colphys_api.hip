@@ -1268,6 +1268,10 @@ int rdb_colphys_stats(rdb_colphys* h, rdb_colphys_counters* out) {
   out->algorithmic_bytes_per_step = 8.0 * (14.0 * (double)live + 3.0 * (double)wet) + 44.0 * (double)wet;
   out->live_levels = live;
   out->lds_levels = h->lds_levels;
+  if (h->closure != RDB_COLPHYS_CLOSURE_PLAIN) {
+    const int st = colphys::surr_settle(h);
+    if (st) return st;
+  }
   out->fallback_fraction =
       h->closure == RDB_COLPHYS_CLOSURE_PLAIN ? 1.0 : (double)h->last_fallback / (double)std::max(wet, 1LL);
   out->block_threads = h->block;

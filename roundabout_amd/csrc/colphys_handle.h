@@ -15,6 +15,9 @@ struct SurrState;
 int launch_step(rdb_colphys* h, double dt, int p0, int p1);
 int refresh_static(rdb_colphys* h, double dt);
 int surr_step(rdb_colphys* h, double dt);
+// the last surrogate step's fallback count and forward time, read from the
+// device (synchronises the stream when a step is pending)
+int surr_settle(rdb_colphys* h);
 void surr_free(SurrState* s);
 int surr_collect(rdb_colphys* h, double dt, int n_steps, int stride, long long* n_out);
 int surr_train(rdb_colphys* h, const rdb_colphys_mlp_config* cfg, float* train_loss, float* val_loss, int n_hist,

@@ -71,6 +71,11 @@ struct SurrState {
   int guard_mode = RDB_COLPHYS_GUARD_STRICT;
   int inf_rows = 0;
   hipEvent_t ev[2] = {nullptr, nullptr};  // around the forward GEMMs (timing on)
+  // a surrogate step leaves its fallback count (and, timing on, its event
+  // pair) on the device; surr_settle reads them when they are asked for
+  bool pending = false;
+  bool timed = false;
+  std::vector<SurrView> svs_up;  // the member views last uploaded (ensembles)
   ~SurrState();
   void free_dataset();
 };
@@ -792,21 +797,40 @@ int surr_step(rdb_colphys* h, double dt) {
 #undef LAUNCH
     SG_TRY(hipGetLastError());
   } else {
+    // the member views change only with the network's workspace: upload
+    // them (synchronously) when they do, and never from a buffer a later
+    // step could overwrite while the copy is pending
     std::vector<SurrView> svs;
     for (rdb_colphys* m : ps) svs.push_back(view_of(m));
-    SG_TRY(hipMemcpyAsync(h->d_svs, svs.data(), sizeof(SurrView) * svs.size(), hipMemcpyHostToDevice, h->stream));
+    if (svs.size() != S->svs_up.size() ||
+        std::memcmp(svs.data(), S->svs_up.data(), sizeof(SurrView) * svs.size()) != 0) {
+      SG_TRY(hipMemcpyAsync(h->d_svs, svs.data(), sizeof(SurrView) * svs.size(), hipMemcpyHostToDevice, h->stream));
+      SG_TRY(hipStreamSynchronize(h->stream));
+      S->svs_up = svs;
+    }
     int st = launch_step_all(h, dt, true);
     if (st) return st;
   }
+  // no synchronisation here: consecutive steps queue back to back, and the
+  // fallback count and the forward time are read by surr_settle
+  S->pending = true;
+  S->timed = h->timing;
+  return RDB_COLPHYS_OK;
+}
+
+int surr_settle(rdb_colphys* h) {
+  SurrState* S = h->surr;
+  if (!S || !S->pending) return RDB_COLPHYS_OK;
   unsigned int nf = 0;
   SG_TRY(hipMemcpyAsync(&nf, S->nfall, 4, hipMemcpyDeviceToHost, h->stream));
-  SG_TRY(hipStreamSynchronize(h->stream));  // also keeps svs alive until its copy is done
+  SG_TRY(hipStreamSynchronize(h->stream));
   h->last_fallback = nf;
-  if (h->timing) {
+  if (S->timed) {
     float ms = 0.f;
     SG_TRY(hipEventElapsedTime(&ms, S->ev[0], S->ev[1]));
     h->last_mlp_ms = ms;
   }
+  S->pending = false;
   return RDB_COLPHYS_OK;
 }
 
