@@ -316,6 +316,17 @@ int rdb_colphys_set_guard(rdb_colphys *h, int guard_mode);
 int rdb_mlp_eval(const int *widths, int n_widths, const float *params, int batch, const float *x, float *y);
 int rdb_mlp_loss_grad(const int *widths, int n_widths, const float *params, int batch, const float *x,
                       const float *t, const float *w, float *loss, float *grad);
+/* rdb_mlp_eval with the outputs output-major, yt (widths[n-1] x batch): the
+ * layout the surrogate closure's fused inference kernel writes for the
+ * column step. */
+int rdb_mlp_eval_t(const int *widths, int n_widths, const float *params, int batch, const float *x, float *yt);
+/* n_steps training steps of the learned closure on one batch (forward,
+ * backward, gradient reduction, Adam with bias correction, BF16 refresh of
+ * the weights and their transposes), starting from params with zero Adam
+ * moments; params_out receives the updated parameters. */
+int rdb_mlp_adam_steps(const int *widths, int n_widths, const float *params, int batch, const float *x,
+                       const float *t, const float *w, int n_steps, double lr, double beta1, double beta2,
+                       double eps, float *params_out);
 
 /* ---- batched ensemble (no counterpart in the reference) -----------------
  * Several independent ocean members, each with its own grid (nx x ny x nz),
@@ -335,6 +346,15 @@ int rdb_colphys_member_count(rdb_colphys *h, int *n_members);
 int rdb_colphys_member_set_field(rdb_colphys *h, int member, const char *name, const double *data, int n1, int n2,
                                  int n3);
 int rdb_colphys_member_get_field(rdb_colphys *h, int member, const char *name, double *out, int n1, int n2, int n3);
+
+/* ---- device routines on host arrays, for verification ------------------ */
+
+/* A7: the device cube root of the KPP path on n host arguments (y[i] =
+ * cbrt(x[i])), the routine rdb_sm_cbrt restates
+ * (src/shared_module_utilities/rdb_safe_math_pow.inc:107-142), so that it can
+ * be checked against the reference's golden bit table
+ * (tests/test_safe_math_golden.inc:557-606). */
+int rdb_colphys_cbrt(const double *x, double *y, int n);
 
 /* ---- error ring (mirrors rdb_ocean_last_error, rdb_ocean.h:302-314) ---- */
 int rdb_colphys_last_error(int idx, char *buf, int cap);

@@ -100,3 +100,19 @@ def loss_grad(widths, params, x, t, w, bf16=False):
             d = r(dx * gelu_grad(pres[li - 1]))
     flat = np.concatenate([np.concatenate([gW.ravel(), gb]) for gW, gb in grads])
     return loss, flat
+
+
+def adam_step(params, grad, m, v, t, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+    """One Adam update (Kingma & Ba 2015, Algorithm 1) in FP64.
+
+    ``m``, ``v`` are the first and second moment estimates before the step
+    and ``t`` the 1-based step count; returns (params, m, v) after it:
+    m <- beta1 m + (1 - beta1) g,  v <- beta2 v + (1 - beta2) g^2,
+    p <- p - lr (m / (1 - beta1^t)) / (sqrt(v / (1 - beta2^t)) + eps)."""
+    g = np.asarray(grad, dtype=np.float64)
+    m = beta1 * np.asarray(m, dtype=np.float64) + (1.0 - beta1) * g
+    v = beta2 * np.asarray(v, dtype=np.float64) + (1.0 - beta2) * g * g
+    c1 = 1.0 - beta1 ** t
+    c2 = 1.0 - beta2 ** t
+    p = np.asarray(params, dtype=np.float64) - lr * (m / c1) / (np.sqrt(v / c2) + eps)
+    return p, m, v

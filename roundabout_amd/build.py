@@ -8,6 +8,7 @@ without the packed-FP32 target feature.
 
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -41,15 +42,21 @@ def flags() -> list[str]:
 
 def build(verbose: bool = False) -> Path:
     LIB_DIR.mkdir(exist_ok=True)
-    objs = []
     cc = hipcc()
+    cmds, objs = [], []
     for src in SOURCES:
         obj = LIB_DIR / (Path(src).stem + ".o")
-        cmd = [cc, *flags(), "-c", str(CSRC / src), "-o", str(obj)]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append([cc, *flags(), "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(str(obj))
+    # the translation units compile independently: one hipcc per source
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
+        def run(cmd):
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            subprocess.run(cmd, check=True)
+        for f in [pool.submit(run, c) for c in cmds]:
+            f.result()
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs]
     if verbose:

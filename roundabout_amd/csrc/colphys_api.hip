@@ -1327,6 +1327,26 @@ int rdb_colphys_set_guard(rdb_colphys* h, int mode) {
   return colphys::surr_set_guard(h, mode);
 }
 
+int rdb_colphys_cbrt(const double* x, double* y, int n) {
+  if (n < 0 || n > (1 << 26) || (n > 0 && (!x || !y)))
+    return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "cbrt: n = %d outside [0, 2^26] or NULL arrays", n);
+  if (n == 0) return RDB_COLPHYS_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return push_error(RDB_COLPHYS_ERR_SETUP, "cbrt: no HIP device available");
+  double* d = nullptr;
+  HIP_TRY(hipMalloc((void**)&d, 16 * (size_t)n));
+  hipError_t e = hipMemcpy(d, x, 8 * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(colphys::k_cbrt_eval, grid1(n, 256), dim3(256), 0, 0, d, d + n, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(y, d + n, 8 * (size_t)n, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return push_error(RDB_COLPHYS_ERR_SETUP, "cbrt: %s", hipGetErrorString(e));
+  return RDB_COLPHYS_OK;
+}
+
 int rdb_colphys_last_error(int idx, char* buf, int cap) {
   if (idx < 0 || idx >= (int)g_ring.size() || cap <= 0 || !buf) return 0;
   const std::string& m = g_ring[(size_t)idx];

@@ -48,6 +48,7 @@ __global__ void k_pack_centre_y(const double* vf, double* dst, const int* col, i
                                 int ny);
 __global__ void k_stress_mag(const double* txc, const double* tyc, double* smag, const double* shelf,
                              double* tau_tot, int ncol);
+__global__ void k_cbrt_eval(const double* x, double* y, int n);
 __global__ void k_eos(StateView s, Params pr, int p0, int p1);
 __global__ void k_pp81(StateView s, Params pr, int p0, int p1);
 __global__ void k_kpp(StateView s, Params pr, int p0, int p1);

@@ -92,6 +92,12 @@ __global__ void k_stress_mag(const double* __restrict__ txc, const double* __res
   tau_tot[p] = s + shelf[p];  // stress_mag + stress_shelf (rdb_ocean_vmix.F90:1318)
 }
 
+// The device cube root of the KPP path (cbrt_kahan) over n arguments.
+__global__ void k_cbrt_eval(const double* __restrict__ x, double* __restrict__ y, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = cbrt_kahan(x[i]);
+}
+
 // ---------------------------------------------------------------------------
 // Staged kernels: one per reference routine, over packed columns [p0, p1).
 // ---------------------------------------------------------------------------

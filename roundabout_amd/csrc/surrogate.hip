@@ -719,6 +719,10 @@ int surr_train(rdb_colphys* h, const rdb_colphys_mlp_config* cfg, float* train_l
   SG_TRY(hipStreamSynchronize(h->stream));
   if (n_done) *n_done = done;
   S->guard_mode = cfg->guard_mode;
+  // the training batch may have resized the network's workspace: the next
+  // surrogate step sets up its inference rows (and the member views) again
+  S->inf_rows = 0;
+  S->svs_up.clear();
   return RDB_COLPHYS_OK;
 }
 
@@ -733,7 +737,9 @@ int surr_step(rdb_colphys* h, double dt) {
   const long long rows = wet_rows(h);
   if (rows > 0x7fffff00LL) return perr(RDB_COLPHYS_ERR_BAD_SHAPE, "surrogate: too many wet columns");
   const int Mpad = mlp::round_up((int)std::max<long long>(rows, 1), 128);  // the fused inference kernel's row block
-  if (S->inf_rows != Mpad) {
+  // the inference workspace must hold every wet row: training in between
+  // may have resized the network's buffers to its batch (Mcap)
+  if (S->inf_rows != Mpad || net.Mcap < Mpad) {
     int e = net.ensure_workspace(Mpad, false);
     if (e) return perr(e, "surrogate: %s", net.err.c_str());
     SG_TRY(hipMemsetAsync(net.X0, 0, 2 * (size_t)Mpad * net.pin[0], h->stream));
@@ -742,6 +748,8 @@ int surr_step(rdb_colphys* h, double dt) {
     if (!S->nfall) SG_TRY(hipMalloc((void**)&S->nfall, 64));
     S->inf_rows = Mpad;
   }
+  if (net.Mcap < Mpad || !S->guard || !S->nfall)
+    return perr(RDB_COLPHYS_ERR_SETUP, "surrogate: inference workspace of %d rows for %d wet rows", net.Mcap, Mpad);
   const std::vector<rdb_colphys*> ps = plans(h);
   SG_TRY(hipMemsetAsync(S->nfall, 0, 4, h->stream));
   const int ld = net.pin[0];
@@ -893,6 +901,7 @@ int surr_load(rdb_colphys* h, const char* path) {
   SG_TRY(hipMalloc((void**)&S->d_stats, 4 * stats.size()));
   SG_TRY(hipMemcpy(S->d_stats, stats.data(), 4 * stats.size(), hipMemcpyHostToDevice));
   S->inf_rows = 0;
+  S->svs_up.clear();
   return RDB_COLPHYS_OK;
 }
 

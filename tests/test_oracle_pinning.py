@@ -474,6 +474,103 @@ def test_nonlocal_telescopes_and_respects_vanished(orc):
     assert np.all(o2["gamma_t"] == 0) and np.all(o2["gamma_s"] == 0)
 
 
+def nonlocal_column_state(orc, qh):
+    """A mixed layer over a thermocline on non-uniform layers (bed first),
+    sheared in the top three layers, forced by wind and a surface flux: the
+    set-up of tests/test_ocean_kpp_nonlocal.F90:60-120 with unequal h."""
+    cfg = orc.default_config()
+    h = [40.0, 32.0, 26.0, 20.0, 16.0, 12.0, 9.0, 7.0]
+    rho = [1027.9, 1027.7, 1027.4, 1026.9, 1026.0, 1025.6, 1025.6, 1025.6]
+    nx, ny, nz = 5, 4, len(h)
+    st = column_grid(nx, ny, nz)
+    for k in range(nz):
+        st["h"][:, :, k] = h[k]
+        st["rho"][:, :, k] = rho[k]
+    st["u"][:, :, nz - 3:] = np.array([0.05, 0.12, 0.2])
+    st["smag"][...] = 0.1
+    st["Qh"][...] = qh
+    return cfg, st, run_pp81_kpp(orc, cfg, st)
+
+
+def test_nonlocal_cooling_sign_and_flux_divergence(orc):
+    # tests/test_ocean_kpp_nonlocal.F90:175-233: under cooling gamma_T < 0 in
+    # the boundary layer; applied alone, the non-local divergence warms the
+    # top layer (gamma(nz+1) = 0 minus a negative gamma(nz)) and cools the
+    # one below it.
+    cfg, st, o = nonlocal_column_state(orc, qh=-500.0)
+    h = st["h"]
+    nz = h.shape[2]
+    gt, gs = o["gamma_t"], o["gamma_s"]
+    assert gt.min() < 0.0
+    assert np.all(gt[:, :, 0] == 0) and np.all(gt[:, :, nz] == 0)
+    T = f3(np.full(h.shape, 14.0))
+    S = f3(np.full(h.shape, 35.0))
+    T0, S0 = T.copy(order="F"), S.copy(order="F")
+    dt = 100.0
+    orc.nonlocal_tendencies(dt, gt, gs, h, T, S)
+    dT = T - T0
+    assert np.all(dT[:, :, nz - 1] > 0.0), "cooling: the top layer must warm"
+    assert np.all(dT[:, :, nz - 2] < 0.0), "cooling: the layer below the top must cool"
+    # each layer's change is the flux divergence over its own thickness,
+    # h_k dT_k = dt (gamma(k+1) - gamma(k)): with unequal layers a dropped
+    # 1/h, or a flipped sign, breaks the column budget or these values
+    for k in range(nz):
+        want = 14.0 + dt * (gt[2, 1, k + 1] - gt[2, 1, k]) / h[2, 1, k]
+        assert T[2, 1, k] == pytest.approx(want, rel=2e-16, abs=0)
+    budget = (h * dT).sum(axis=2)  # dT carries ~1e-11 relative rounding of T - T0
+    assert np.max(np.abs(budget)) < 1e-10 * np.max(np.abs(h * dT))
+    # salinity follows gamma_s with the same divergence (zero here: Q_salt = 0)
+    assert np.all(gs == 0) and np.array_equal(S, S0)
+
+
+def test_nonlocal_closed_form_hand_built_gamma(orc):
+    # hand-built gamma on non-uniform layers with a vanished layer: the layer
+    # gets dt (F_up - F_dn) / (h + 1e-20), F_up = gamma(k+1), F_dn = gamma(k),
+    # a face touching a vanished layer carries nothing, and a vanished layer
+    # receives nothing (rdb_ocean_vmix.F90:1652-1714)
+    nx, ny, nz = 1, 1, 5
+    h = f3(np.array([50.0, 1e-4, 25.0, 10.0, 4.0]).reshape(1, 1, nz))
+    g = f3(np.array([0.0, -1e-6, -3e-6, -8e-6, -2e-6, 0.0]).reshape(1, 1, nz + 1))
+    T = f3(np.full((1, 1, nz), 10.0))
+    S = f3(np.full((1, 1, nz), 35.0))
+    orc.nonlocal_tendencies(3600.0, g, f3(np.zeros_like(g)), h, T, S)
+    # layer 0: its top face touches the vanished layer 1 -> no flux at all
+    assert T[0, 0, 0] == 10.0 and T[0, 0, 1] == 10.0
+    # layer 2: F_up = gamma(3) = -8e-6, F_dn = 0 (face 2 touches layer 1)
+    assert T[0, 0, 2] == pytest.approx(10.0 + 3600.0 * (-8e-6) / 25.0, rel=1e-15)
+    # layer 3: F_up = -2e-6, F_dn = -8e-6: net gain 6e-6 over 10 m
+    assert T[0, 0, 3] == pytest.approx(10.0 + 3600.0 * 6e-6 / 10.0, rel=1e-15)
+    # top layer: F_up = gamma(nz) closed at the surface (no k+1), F_dn = -2e-6
+    assert T[0, 0, 4] == pytest.approx(10.0 + 3600.0 * 2e-6 / 4.0, rel=1e-15)
+    assert np.all(S == 35.0)
+
+
+# ---------------------------------------------------------------- A16 stress magnitude
+def test_stress_mag_closed_forms(orc):
+    # rdb_ocean_surface_stress.F90:858-875: |tau| at a cell centre from the
+    # averages of its two x faces (i, i+1) and its two y faces (j, j+1)
+    nx, ny = 3, 2
+    tx = f3(np.zeros((nx + 1, ny)))
+    ty = f3(np.zeros((nx, ny + 1)))
+    # cell (1, 0): x faces 2 and 4 -> 3, y faces 0 and 8 -> 4: |tau| = 5 exactly
+    tx[1, 0], tx[2, 0] = 2.0, 4.0
+    ty[1, 0], ty[1, 1] = 0.0, 8.0
+    m = orc.stress_mag(tx, ty)
+    assert m[1, 0] == 5.0
+    # the x face at i = 2 is shared by cells 1 and 2; the y face at j = 1 by rows 0 and 1
+    assert m[2, 0] == 2.0          # (4 + 0) / 2 on x, nothing on y
+    assert m[1, 1] == 4.0          # (8 + 0) / 2 on y, nothing on x
+    assert m[0, 0] == 1.0          # (0 + 2) / 2 on x
+    assert m[0, 1] == 0.0 and m[2, 1] == 0.0
+    # a uniform stress at any angle keeps its magnitude (to rounding)
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        mag, ang = rng.lognormal(np.log(0.08), 0.5), rng.uniform(0, 2 * np.pi)
+        txu = f3(np.full((nx + 1, ny), mag * np.cos(ang)))
+        tyu = f3(np.full((nx, ny + 1), mag * np.sin(ang)))
+        assert np.allclose(orc.stress_mag(txu, tyu), mag, rtol=4e-16, atol=0)
+
+
 # ---------------------------------------------------------------- A2 surface flux
 def test_surface_deposit_closed_form(orc):
     cfg = orc.default_config()
