@@ -328,6 +328,21 @@ int rdb_mlp_adam_steps(const int *widths, int n_widths, const float *params, int
                        const float *t, const float *w, int n_steps, double lr, double beta1, double beta2,
                        double eps, float *params_out);
 
+/* rdb_mlp_eval without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels and results as rdb_mlp_eval; the weights,
+ * the padded input and output and the activations live in one 80 MiB
+ * scratch array per GPU that the library holds from load to process end
+ * (no hipMalloc, hipFree or other allocating routine is called).  A network
+ * and batch whose working set exceeds it (for the 381-256x4-149 closure
+ * network, batches above about 20000 rows) return RDB_COLPHYS_ERR_BAD_SHAPE
+ * before any work.  Runs on the legacy null stream and returns after the
+ * outputs are copied back.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (several host threads) race. */
+int rdb_mlp_eval_noalloc(const int *widths, int n_widths, const float *params, int batch, const float *x,
+                         float *y);
+
 /* ---- batched ensemble (no counterpart in the reference) -----------------
  * Several independent ocean members, each with its own grid (nx x ny x nz),
  * wet mask and optional depth sort key, in one handle.  rdb_colphys_step

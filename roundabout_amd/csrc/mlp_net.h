@@ -17,6 +17,25 @@
 
 namespace mlp {
 
+// The inference chain on buffers the caller owns: padded bf16 weights
+// (pout x pin, k contiguous), padded FP32 biases, and for the per-layer path
+// one activation buffer per hidden layer (Mpad x pout[l] bf16; layer l only
+// reads act[l - 1], so two buffers used alternately suffice).
+constexpr int LV_MAXL = 16;
+struct LayerView {
+  int nl;
+  int pin[LV_MAXL], pout[LV_MAXL];
+  const bf16* W[LV_MAXL];
+  const float* b[LV_MAXL];
+  bf16* act[LV_MAXL];
+};
+// Whether forward_view runs the fused single-launch kernel for this shape.
+bool fused_shape_ok(const LayerView& v, int Mpad);
+// X0 (Mpad x pin[0] bf16) -> Y, row-major (Mpad x pout[nl-1]) or, with
+// out_major, output-major (pout[nl-1] x Mpad); launches on s, allocates
+// nothing.
+hipError_t forward_view(const LayerView& v, const bf16* X0, float* Y, int Mpad, bool out_major, hipStream_t s);
+
 struct Net {
   int nl = 0;                      // linear layers
   std::vector<int> din, dout;      // unpadded in/out per layer
@@ -78,6 +97,7 @@ struct Net {
   // limits run as one fused launch (k_mlp_fused), else per-layer GEMMs.
   int forward(int Mpad, bool out_major = false);
   bool fused_ok(int Mpad) const;
+  LayerView view() const;  // the network's own weights and workspace
   // X0 -> X0T (feature-major copy of the inputs) for callers that fill X0
   // themselves.
   int transpose_input(int Mpad);

@@ -253,66 +253,95 @@ int Net::ensure_workspace(int Mpad, bool training) {
   return 0;
 }
 
-bool Net::fused_ok(int Mpad) const {
+static bool unfused_requested() {
   // RDB_MLP_UNFUSED=1 keeps inference on the per-layer GEMMs (A/B timing)
   static const bool unfused = [] {
     const char* e = std::getenv("RDB_MLP_UNFUSED");
     return e && e[0] == '1';
   }();
-  if (unfused || Mpad % 128 != 0 || nl > FP_MAXL) return false;
-  for (int l = 0; l < nl; ++l)
-    if (pin[l] > FP_MAXK || pout[l] > FP_MAXN) return false;
+  return unfused;
+}
+
+bool fused_shape_ok(const LayerView& v, int Mpad) {
+  if (unfused_requested() || Mpad % 128 != 0 || v.nl > FP_MAXL) return false;
+  for (int l = 0; l < v.nl; ++l)
+    if (v.pin[l] > FP_MAXK || v.pout[l] > FP_MAXN) return false;
   return true;
 }
 
-int Net::forward(int Mpad, bool out_major) {
-  if (fused_ok(Mpad)) {
+hipError_t forward_view(const LayerView& v, const bf16* X0, float* Y, int Mpad, bool out_major, hipStream_t s) {
+  if (v.nl < 1 || v.nl > LV_MAXL) return hipErrorInvalidValue;
+  if (fused_shape_ok(v, Mpad)) {
     // inference: one launch over all layers, activations in LDS
-    static bool lds_set = false;
+    static unsigned long long lds_set = 0;  // per device (devices 0..63)
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
     const size_t shmem = FP_LDS_BYTES;
-    if (!lds_set) {
-      NT_TRY(hipFuncSetAttribute((const void*)k_mlp_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-      lds_set = true;
+    if (dev >= 64 || !(lds_set >> dev & 1ull)) {
+      e = hipFuncSetAttribute((const void*)k_mlp_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+      if (e != hipSuccess) return e;
+      if (dev < 64) lds_set |= 1ull << dev;
     }
     FusedNet fn{};
-    fn.nl = nl;
-    for (int l = 0; l < nl; ++l) {
-      fn.W[l] = Wb + offWb[l];
-      fn.b[l] = p + offB[l];
-      fn.pin[l] = pin[l];
-      fn.pout[l] = pout[l];
+    fn.nl = v.nl;
+    for (int l = 0; l < v.nl; ++l) {
+      fn.W[l] = v.W[l];
+      fn.b[l] = v.b[l];
+      fn.pin[l] = v.pin[l];
+      fn.pout[l] = v.pout[l];
     }
-    const long long ldy = out_major ? (long long)Mpad : (long long)pout[nl - 1];
-    hipLaunchKernelGGL(k_mlp_fused, dim3((unsigned)(Mpad / 128)), dim3(64 * FP_WAVES), shmem, stream, fn,
-                       (const bf16*)X0,
-                       pin[0], Y, ldy, out_major ? 1 : 0);
-    NT_TRY(hipGetLastError());
-    return 0;
+    const long long ldy = out_major ? (long long)Mpad : (long long)v.pout[v.nl - 1];
+    hipLaunchKernelGGL(k_mlp_fused, dim3((unsigned)(Mpad / 128)), dim3(64 * FP_WAVES), shmem, s, fn, X0, v.pin[0], Y,
+                       ldy, out_major ? 1 : 0);
+    return hipGetLastError();
   }
   // per-layer GEMMs (batches the fused kernel does not take)
   const bf16* x = X0;
-  int ldx = pin[0];
-  for (int l = 0; l < nl; ++l) {
+  int ldx = v.pin[0];
+  for (int l = 0; l < v.nl; ++l) {
     GemmEpi ep = epi();
-    ep.bias = p + offB[l];
-    const bf16* W = Wb + offWb[l];
-    if (l < nl - 1) {
-      ep.Cb = act[l];
-      ep.ldcb = pout[l];
-      gemm<EPI_BIAS_GELU>(stream, gargs(x, ldx, W, pin[l], Mpad, pout[l], pin[l], ep));
-      x = act[l];
-      ldx = pout[l];
+    ep.bias = v.b[l];
+    if (l < v.nl - 1) {
+      ep.Cb = v.act[l];
+      ep.ldcb = v.pout[l];
+      gemm<EPI_BIAS_GELU>(s, gargs(x, ldx, v.W[l], v.pin[l], Mpad, v.pout[l], v.pin[l], ep));
+      x = v.act[l];
+      ldx = v.pout[l];
     } else if (out_major) {
       ep.Cf = Y;
       ep.ldcf = Mpad;
-      gemm<EPI_BIAS_F32_T>(stream, gargs(x, ldx, W, pin[l], Mpad, pout[l], pin[l], ep));
+      gemm<EPI_BIAS_F32_T>(s, gargs(x, ldx, v.W[l], v.pin[l], Mpad, v.pout[l], v.pin[l], ep));
     } else {
       ep.Cf = Y;
-      ep.ldcf = pout[l];
-      gemm<EPI_BIAS_F32>(stream, gargs(x, ldx, W, pin[l], Mpad, pout[l], pin[l], ep));
+      ep.ldcf = v.pout[l];
+      gemm<EPI_BIAS_F32>(s, gargs(x, ldx, v.W[l], v.pin[l], Mpad, v.pout[l], v.pin[l], ep));
     }
   }
-  NT_TRY(hipGetLastError());
+  return hipGetLastError();
+}
+
+LayerView Net::view() const {
+  LayerView v{};
+  v.nl = nl;
+  for (int l = 0; l < nl && l < LV_MAXL; ++l) {
+    v.pin[l] = pin[l];
+    v.pout[l] = pout[l];
+    v.W[l] = Wb + offWb[l];
+    v.b[l] = p + offB[l];
+    v.act[l] = l < (int)act.size() ? act[l] : nullptr;
+  }
+  return v;
+}
+
+bool Net::fused_ok(int Mpad) const { return fused_shape_ok(view(), Mpad); }
+
+int Net::forward(int Mpad, bool out_major) {
+  if (nl > LV_MAXL) {
+    err = "forward: more layers than the inference chain takes";
+    return 2;
+  }
+  NT_TRY(forward_view(view(), X0, Y, Mpad, out_major, stream));
   return 0;
 }
 

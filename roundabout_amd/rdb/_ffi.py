@@ -227,6 +227,8 @@ PROTOTYPES = [
      [_ip, ctypes.c_int, _fp, ctypes.c_int, _fp, _fp, _fp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
       ctypes.c_double, ctypes.c_double, _fp]),
     ("rdb_colphys_cbrt", ctypes.c_int, [_dp, _dp, ctypes.c_int]),
+    ("rdb_mlp_eval_noalloc", ctypes.c_int,
+     [_ip, ctypes.c_int, _fp, ctypes.c_int, _fp, _fp]),
 ]
 
 
