@@ -31,15 +31,31 @@ def build(force: bool = False) -> Path:
     return LIB_PATH
 
 
+def build_native(out_dir: str | os.PathLike) -> Path:
+    """Compile the timed CPU-baseline variant (-O3 -march=native, still no FMA
+    contraction) into out_dir, for the host this runs on."""
+    subprocess.run(["make", "-s", "-B", "-C", str(HERE), "native", f"OUT={Path(out_dir).resolve()}"], check=True)
+    return Path(out_dir).resolve() / "libcolphys_oracle_native.so"
+
+
 _LIB = None
+_LIB_OVERRIDE: Path | None = None
+
+
+def use_library(path: str | os.PathLike | None) -> None:
+    """Route subsequent oracle calls to another build of the same source
+    (None: the default build)."""
+    global _LIB, _LIB_OVERRIDE
+    _LIB = None
+    _LIB_OVERRIDE = Path(path) if path is not None else None
 
 
 def lib() -> ctypes.CDLL:
     global _LIB
     if _LIB is None:
-        if not LIB_PATH.exists():
+        if _LIB_OVERRIDE is None and not LIB_PATH.exists():
             build()
-        _LIB = ctypes.CDLL(str(LIB_PATH))
+        _LIB = ctypes.CDLL(str(_LIB_OVERRIDE or LIB_PATH))
         L = _LIB
         from roundabout_amd.rdb._ffi import Config
         cfgp = ctypes.POINTER(Config)
