@@ -472,6 +472,27 @@ int create_plan(const rdb_colphys_config* cfg, int nx, int ny, int nz, int n_pas
   h->nz = nz;
   h->n_passive = n_passive;
   h->precision = cfg->precision;
+  {
+    // the fused step may hold its Thomas state in all of a workgroup's LDS
+    // (160 KiB on gfx950): allowed once per device for every instantiation
+    static unsigned long long lds_allowed = 0;  // per device 0..63
+    int dev = 0, mx = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&mx, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || mx <= 0) {
+      delete h;
+      return push_error(RDB_COLPHYS_ERR_SETUP, "create: LDS size query failed");
+    }
+    if (dev >= 64 || !(lds_allowed >> dev & 1ull)) {
+      const hipError_t e = colphys::allow_full_lds(mx);
+      if (e != hipSuccess) {
+        delete h;
+        return push_error(RDB_COLPHYS_ERR_SETUP, "create: dynamic LDS of %d bytes refused: %s", mx,
+                          hipGetErrorString(e));
+      }
+      if (dev < 64) lds_allowed |= 1ull << dev;
+    }
+    h->lds_max = mx;
+  }
   // RDB_COLPHYS_BLOCK pins the launch shape (64, 128 or 256 threads) and
   // turns rdb_colphys_autotune into a no-op, so that a profiled run can
   // reproduce an autotuned configuration exactly.
@@ -483,7 +504,7 @@ int create_plan(const rdb_colphys_config* cfg, int nx, int ny, int nz, int n_pas
       const char* lv = std::getenv("RDB_COLPHYS_LDS_LEVELS");
       int m = lv ? std::atoi(lv) : 0;
       const size_t nq = (size_t)(3 + std::min(n_passive, 4) + (cfg->kt_bg != cfg->ks_bg ? 1 : 0));
-      if (m > 0 && (size_t)b * (size_t)m * nq * 8 <= 65536) h->lds_levels = m;
+      if (m > 0 && (size_t)b * (size_t)m * nq * 8 <= (size_t)h->lds_max) h->lds_levels = m;
     }
   }
   const int ncol = nx * ny;
@@ -1110,14 +1131,14 @@ int rdb_colphys_autotune(rdb_colphys* h, double dt) {
     }
   };
   // candidates: launch block x live levels of Thomas state held in LDS
-  // (dynamic LDS per block <= 64 KiB) x column packing (single grids: the
+  // (up to the device's LDS per workgroup) x column packing (single grids: the
   // current order, grid order, and deepest-first by live levels, which
   // keeps a wavefront's columns of one depth together)
   const size_t nq = (size_t)(3 + std::min(h->n_passive, 4) + (h->cfg.kt_bg != h->cfg.ks_bg ? 1 : 0));
   std::vector<std::pair<int, int>> cands;
   for (int b : {64, 128, 256}) {
-    int mmax = (int)(65536 / ((size_t)b * nq * 8));
-    for (int m : {0, 4, 8, 12, 16, 24}) {
+    int mmax = (int)((size_t)h->lds_max / ((size_t)b * nq * 8));
+    for (int m : {0, 4, 8, 12, 16, 24, 32}) {
       if (m <= mmax) cands.push_back({b, m});
     }
   }

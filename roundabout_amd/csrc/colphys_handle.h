@@ -62,6 +62,7 @@ struct rdb_colphys {
   long long timed = 0, steps = 0;
   int block = 256;
   int lds_levels = 0;  // live levels of Thomas state kept in LDS per lane
+  int lds_max = 65536;  // dynamic LDS a fused-step workgroup may take (the device's limit)
   long long live_levels = 0;  // sum over wet columns of live levels
   std::vector<int> col_live;  // live levels per grid column (from the last h_layer upload)
   bool block_pinned = false;

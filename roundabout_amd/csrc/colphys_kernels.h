@@ -82,5 +82,7 @@ template <typename R, int NTR, bool TWO_FACTOR, bool SURR>
 __global__ void k_column_step_ens(const StateView* views, const SurrView* svs, const int* n_wet, Params pr, double dt,
                                   const int2* work, int n_work, unsigned int* guard_count, int M);
 __global__ void k_totals(StateView s, int p0, int p1, unsigned long long* acc);
+// Allow every fused-step kernel lds_bytes of dynamic LDS (needed above 64 KiB).
+hipError_t allow_full_lds(int lds_bytes);
 
 }  // namespace colphys

@@ -13,6 +13,16 @@ namespace colphys {
 
 #define LV(arr, k) (arr)[(size_t)(k) * (size_t)P + (size_t)p]
 
+// Levels of loads the fused step keeps in flight ahead of its serial
+// sweeps (register ring depth).
+constexpr int PF = 6;
+// Sweep A usually stops a few levels below the surface: a shallower ring.
+constexpr int PFA = 3;
+// Cache policy of the fused step: the streamed inputs of sweep B and the
+// final outputs (kv, kt, ks, T, S, tracers) are nontemporal loads and
+// stores, so that L2 and the memory-side cache keep the Thomas state that
+// sweep B spills and sweep C reads back.
+
 // ---------------------------------------------------------------------------
 // Pack / unpack between the reference's full-grid layout (i fastest) and the
 // packed level-major layout.  col[p] = i + nx * j.
@@ -484,13 +494,33 @@ __device__ __forceinline__ void column_step(const StateView& s, const Params& pr
     R ri_prev = R(0), d_prev = d_centre_ref, h_b = R(0), d_centre_k = d_centre_ref;
     R h_up = R(h_sfc);
     int kcross = 0;
-    for (int k = k_sfc - 1; k >= 0; --k) {
-      double hk_d = LV(s.h, k);
+    // the levels below the surface layer come through a ring of PFA
+    // register slots (slot j: level k_sfc - 1 - j - PFA n), loaded without a
+    // branch on h so that the next levels are in flight while one is
+    // tested; a level below the bed re-reads level 0, unused
+    struct LevA {
+      double h, ub, vb, T, S;
+    };
+    auto load_a = [&](int k, LevA& q) {
+      const int kk = max(k, 0);
+      q.h = LV(s.h, kk);
+      q.ub = LV(s.ub, kk);
+      q.vb = LV(s.vb, kk);
+      q.T = LV(s.T, kk);
+      q.S = LV(s.S, kk);
+    };
+    LevA aring[PFA];
+#pragma unroll
+    for (int j = 0; j < PFA; ++j) load_a(k_sfc - 1 - j, aring[j]);
+    bool crossed = false;
+    auto level_a = [&](int k, LevA& q) {
+      double hk_d = q.h;
       R hk = R(hk_d);
       d_centre_k = d_centre_k + R(0.5) * (h_up + hk);
       h_up = hk;
-      R u_k = R(LV(s.ub, k)), v_k = R(LV(s.vb, k));
-      double rho_k = (hk_d > HV) ? eos_wright<double>(LV(s.T, k), LV(s.S, k), hk_d, pr.rho0, pr.p_ref) : pr.rho0;
+      R u_k = R(q.ub), v_k = R(q.vb);
+      double rho_k = (hk_d > HV) ? eos_wright<double>(q.T, q.S, hk_d, pr.rho0, pr.p_ref) : pr.rho0;
+      load_a(k - PFA, q);
       double b_k = -(GRAVITY * rho_k / pr.rho0);
       R delta_d = d_centre_k - d_centre_ref;
       R delta_b = R(b_ref - b_k);
@@ -504,10 +534,16 @@ __device__ __forceinline__ void column_step(const StateView& s, const Params& pr
         R frac = (fabs(denom) > R(1.0e-12)) ? (R(pr.ri_crit) - ri_prev) / denom : R(0.5);
         h_b = d_prev + frac * (d_centre_k - d_prev);
         kcross = k + 1;
-        break;
+        crossed = true;
+        return;
       }
       ri_prev = ri_bulk;
       d_prev = d_centre_k;
+    };
+    for (int k0 = k_sfc - 1; k0 >= 0 && !crossed; k0 -= PFA) {
+#pragma unroll
+      for (int j = 0; j < PFA; ++j)
+        if (!crossed && k0 - j >= 0) level_a(k0 - j, aring[j]);
     }
     if (kcross == 0) {
       h_b = R(0);
@@ -606,16 +642,40 @@ __device__ __forceinline__ void column_step(const StateView& s, const Params& pr
     kvv = fmn(fmx(kvv, pr.kv_bg), pr.kv_max);
     ktv = fmn(fmx(ktv, pr.kt_bg), pr.kd_max);
     ksv = fmn(fmx(ksv, pr.ks_bg), pr.kd_max);
-    LV(s.kv, kf) = kvv;
-    LV(s.kt, kf) = ktv;
-    LV(s.ks, kf) = ksv;
+    __builtin_nontemporal_store(kvv, &LV(s.kv, kf));
+    __builtin_nontemporal_store(ktv, &LV(s.kt, kf));
+    __builtin_nontemporal_store(ksv, &LV(s.ks, kf));
     kt_out = ktv;
     ks_out = ksv;
   };
 
-  double h0 = LV(s.h, kb);
-  double T0 = LV(s.T, kb), S0 = LV(s.S, kb);
-  R ub0 = R(LV(s.ub, kb)), vb0 = R(LV(s.vb, kb));
+  // Level inputs of sweep B travel through a ring of PF register slots:
+  // slot j holds level kb + j + PF n, and the loads of level k + PF are
+  // issued as soon as level k has been consumed, so PF - 1 levels of loads
+  // are in flight behind the serial elimination.  The loop is unrolled by
+  // PF so that every slot index is a constant; a load past the surface
+  // re-reads the top level (in bounds, unused).
+  struct LevIn {
+    double h, T, S, ub, vb;
+    double tr[NTR > 0 ? NTR : 1];
+  };
+  auto load_lev = [&](int k, LevIn& q) {
+    const int kk = min(k, nz - 1);
+    q.h = __builtin_nontemporal_load(&LV(s.h, kk));
+    q.T = __builtin_nontemporal_load(&LV(s.T, kk));
+    q.S = __builtin_nontemporal_load(&LV(s.S, kk));
+    q.ub = __builtin_nontemporal_load(&LV(s.ub, kk));
+    q.vb = __builtin_nontemporal_load(&LV(s.vb, kk));
+#pragma unroll
+    for (int t = 0; t < NTR; ++t) q.tr[t] = __builtin_nontemporal_load(&LV(trb + (size_t)t * tstride, kk));
+  };
+  LevIn ring[PF];
+#pragma unroll
+  for (int j = 0; j < PF; ++j) load_lev(kb + j, ring[j]);
+
+  double h0 = ring[0].h;
+  double T0 = ring[0].T, S0 = ring[0].S;
+  R ub0 = R(ring[0].ub), vb0 = R(ring[0].vb);
   double rho_0 = eos_wright<double>(T0, S0, h0, pr.rho0, pr.p_ref);
   double hm1 = 0.0;
   double Kt_below = 0.0, Ks_below = 0.0;  // interface kb
@@ -628,17 +688,18 @@ __device__ __forceinline__ void column_step(const StateView& s, const Params& pr
 #pragma unroll
   for (int t = 0; t < NTR; ++t) rtr[t] = 0.0;
 
-  for (int k = kb; k < nz; ++k) {
+  // one level of sweep B: `cur` holds level k, `nxt` level k + 1
+  auto level_b = [&](int k, LevIn& cur, const LevIn& nxt) {
     double h1 = 0.0, T1 = 0.0, S1 = 0.0;
     R ub1 = R(0), vb1 = R(0);
     double rho_1 = 0.0;
     double Kt_above = 0.0, Ks_above = 0.0;
     if (k + 1 < nz) {
-      h1 = LV(s.h, k + 1);
-      T1 = LV(s.T, k + 1);
-      S1 = LV(s.S, k + 1);
-      ub1 = R(LV(s.ub, k + 1));
-      vb1 = R(LV(s.vb, k + 1));
+      h1 = nxt.h;
+      T1 = nxt.T;
+      S1 = nxt.S;
+      ub1 = R(nxt.ub);
+      vb1 = R(nxt.vb);
       rho_1 = eos_wright<double>(T1, S1, h1, pr.rho0, pr.p_ref);
       iface(k + 1, ub0, vb0, ub1, vb1, h0, h1, rho_0, rho_1, Kt_above, Ks_above);
     }
@@ -678,9 +739,11 @@ __device__ __forceinline__ void column_step(const StateView& s, const Params& pr
     }
 #pragma unroll
     for (int t = 0; t < NTR; ++t) {
-      double x = LV(trb + (size_t)t * tstride, k);
+      double x = cur.tr[t];
       rtr[t] = (k == kb) ? x / piv_s : (x - a_s * rtr[t]) / piv_s;
     }
+    // level k is consumed: its slot takes level k + PF
+    load_lev(k + PF, cur);
     const int m = k - kb;
     if (m < M) {
       LSLOT(0, m) = cp_t;
@@ -706,6 +769,11 @@ __device__ __forceinline__ void column_step(const StateView& s, const Params& pr
     rho_0 = rho_1;
     Kt_below = Kt_above;
     Ks_below = Ks_above;
+  };
+  for (int k0 = kb; k0 < nz; k0 += PF) {
+#pragma unroll
+    for (int j = 0; j < PF; ++j)
+      if (k0 + j < nz) level_b(k0 + j, ring[j], ring[(j + 1) % PF]);
   }
 
   // ---------------- sweep C: back substitution + non-local ----------------
@@ -717,7 +785,28 @@ __device__ __forceinline__ void column_step(const StateView& s, const Params& pr
   double g_t_up = 0.0, g_s_up = 0.0;  // gamma at interface k+1
   R d_running = R(0);
   double h_up = 0.0;
-  for (int k = nz - 1; k >= kb; --k) {
+  // the spilled Thomas state of the levels above the LDS tier comes back
+  // through a ring of PF register slots, PF - 1 levels ahead of the back
+  // substitution (slot j holds level nz - 1 - j - PF n; a level inside the
+  // LDS tier, or below the bed, loads the lowest spilled level instead, in
+  // bounds and unused)
+  struct SpIn {
+    double cpt, cps, T, S;
+    double tr[NTR > 0 ? NTR : 1];
+  };
+  auto load_sp = [&](int k, SpIn& q) {
+    const int kk = min(max(k, kb + M), nz - 1);
+    q.cpt = LV(s.fc, kk);
+    if (TWO_FACTOR) q.cps = LV(s.fb, kk);
+    q.T = LV(s.T, kk);
+    q.S = LV(s.S, kk);
+#pragma unroll
+    for (int t = 0; t < NTR; ++t) q.tr[t] = LV(trb + (size_t)t * tstride, kk);
+  };
+  SpIn sring[PF];
+#pragma unroll
+  for (int j = 0; j < PF; ++j) load_sp(nz - 1 - j, sring[j]);
+  auto level_c = [&](int k, SpIn& q) {
     const int m = k - kb;
     double cpt, cps, xT, xS;
     double xt[NTR > 0 ? NTR : 1];
@@ -729,13 +818,15 @@ __device__ __forceinline__ void column_step(const StateView& s, const Params& pr
       for (int t = 0; t < NTR; ++t) xt[t] = LSLOT(3 + t, m);
       cps = TWO_FACTOR ? LSLOT(3 + NTR, m) : cpt;
     } else {
-      cpt = LV(s.fc, k);
-      cps = TWO_FACTOR ? LV(s.fb, k) : cpt;
-      xT = LV(s.T, k);
-      xS = LV(s.S, k);
+      cpt = q.cpt;
+      cps = TWO_FACTOR ? q.cps : cpt;
+      xT = q.T;
+      xS = q.S;
 #pragma unroll
-      for (int t = 0; t < NTR; ++t) xt[t] = LV(trb + (size_t)t * tstride, k);
+      for (int t = 0; t < NTR; ++t) xt[t] = q.tr[t];
     }
+    // level k is consumed: its slot takes level k - PF
+    load_sp(k - PF, q);
     if (k < nz - 1) {
       xT = xT - cpt * xT_up;
       xS = xS - cps * xS_up;
@@ -745,7 +836,7 @@ __device__ __forceinline__ void column_step(const StateView& s, const Params& pr
 #pragma unroll
     for (int t = 0; t < NTR; ++t) {
       xtr_up[t] = xt[t];
-      LV(trb + (size_t)t * tstride, k) = xt[t];
+      __builtin_nontemporal_store(xt[t], &LV(trb + (size_t)t * tstride, k));
     }
     xT_up = xT;
     xS_up = xS;
@@ -777,8 +868,13 @@ __device__ __forceinline__ void column_step(const StateView& s, const Params& pr
       g_s_up = g_s_dn;
       h_up = hk;
     }
-    LV(s.T, k) = xT;
-    LV(s.S, k) = xS;
+    __builtin_nontemporal_store(xT, &LV(s.T, k));
+    __builtin_nontemporal_store(xS, &LV(s.S, k));
+  };
+  for (int k0 = nz - 1; k0 >= kb; k0 -= PF) {
+#pragma unroll
+    for (int j = 0; j < PF; ++j)
+      if (k0 - j >= kb) level_c(k0 - j, sring[j]);
   }
   if (pr.guard && bad) atomicAdd(guard_count, bad);
 #undef LSLOT
@@ -871,5 +967,25 @@ INST(double, 3, false, true) INST(double, 4, false, true)
 INST(double, 0, true, true) INST(double, 1, true, true) INST(double, 2, true, true)
 INST(double, 3, true, true) INST(double, 4, true, true)
 #undef INST
+
+// Dynamic LDS above 64 KiB has to be allowed per kernel: allow every
+// fused-step instantiation the device's whole per-workgroup LDS.
+hipError_t allow_full_lds(int lds_bytes) {
+  hipError_t e = hipSuccess;
+#define ALLOW(R, N, TF, SU)                                                                                  \
+  if (e == hipSuccess)                                                                                       \
+    e = hipFuncSetAttribute((const void*)k_column_step<R, N, TF, SU>,                                        \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);                          \
+  if (e == hipSuccess)                                                                                       \
+    e = hipFuncSetAttribute((const void*)k_column_step_ens<R, N, TF, SU>,                                    \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+#define ALLOW_N(R, TF, SU) ALLOW(R, 0, TF, SU) ALLOW(R, 1, TF, SU) ALLOW(R, 2, TF, SU) ALLOW(R, 3, TF, SU) \
+  ALLOW(R, 4, TF, SU)
+  ALLOW_N(double, false, false) ALLOW_N(double, true, false) ALLOW_N(float, false, false)
+  ALLOW_N(float, true, false) ALLOW_N(double, false, true) ALLOW_N(double, true, true)
+#undef ALLOW_N
+#undef ALLOW
+  return e;
+}
 
 }  // namespace colphys
