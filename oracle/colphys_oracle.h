@@ -88,6 +88,22 @@ void orc_totals(int nx, int ny, int nz, const double *h, const double *c,
 
 int orc_num_threads(void);
 
+/* Momentum vertical viscosity (mom_oracle.c): the switches of
+ * diffuse_velocity_columns_impl (rdb_ocean_vdiff.F90:1416-1926). */
+typedef struct orc_mom_opts {
+  int do_stress, do_drag, do_top, solve_momentum, do_remnant;
+  int hvel_mom6, hvel_harmonic, hvel_upwind, bbl_glue, zlevel_faces, do_corner, use_harmonic;
+  double rho0, hbbl_visc, kv_bbl_bg, kv_prandtl;
+} orc_mom_opts;
+
+void orc_diffuse_velocity_columns(const orc_mom_opts *o, int nu, int nv, int nz, double dt, double *u_face,
+                                  const double *h_layer, const double *kv_centre, const double *wet_cell,
+                                  int x_face, int nx_cells, int ny_cells, const double *tau_face,
+                                  const double *lambda_bot, const double *lambda_top, const double *cover_face,
+                                  double *visc_rem_out, const double *kv_bbl_face, const double *bbl_thick_face,
+                                  const double *kv_corner, const int *k_top_face, const int *k_bot_face,
+                                  double *c_diag, double *rhs);
+
 #ifdef __cplusplus
 }
 #endif

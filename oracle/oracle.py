@@ -244,3 +244,94 @@ def set_threads(n: int) -> None:
         omp.omp_set_num_threads(int(n))
     except OSError:
         pass
+
+
+# ---------------------------------------------------------------- momentum
+class MomOpts(ctypes.Structure):
+    """orc_mom_opts: the switches of diffuse_velocity_columns_impl."""
+    _fields_ = [(n, ctypes.c_int) for n in ("do_stress", "do_drag", "do_top", "solve_momentum", "do_remnant",
+                                            "hvel_mom6", "hvel_harmonic", "hvel_upwind", "bbl_glue",
+                                            "zlevel_faces", "do_corner", "use_harmonic")] + [
+        (n, ctypes.c_double) for n in ("rho0", "hbbl_visc", "kv_bbl_bg", "kv_prandtl")]
+
+
+def _mom_lib():
+    L = lib()
+    if not getattr(L, "_mom_ready", False):
+        L.orc_diffuse_velocity_columns.argtypes = (
+            [ctypes.POINTER(MomOpts)] + [ctypes.c_int] * 3 + [ctypes.c_double] + [_dp] * 4 + [ctypes.c_int] * 3
+            + [_dp] * 8 + [_ip, _ip, _dp, _dp])
+        L._mom_ready = True
+    return L
+
+
+def _f(a, shape, fill=0.0):
+    """A Fortran-ordered float64 array of `shape`: `a` itself, or `fill`."""
+    if a is None:
+        return np.full(shape, fill, order="F")
+    a = np.asarray(a, dtype=np.float64)
+    assert a.shape == tuple(shape), (a.shape, shape)
+    return np.asfortranarray(a)
+
+
+def vdiff_apply_momentum(h, u, v, wet, dt, *, kv_source=None, K_v_momentum=0.0, use_harmonic=False,
+                         implicit_stress=False, implicit_drag=False, implicit_top_drag=False, tau_u=None,
+                         tau_v=None, lambda_bot_u=None, lambda_bot_v=None, lambda_top_u=None, lambda_top_v=None,
+                         cover_u=None, cover_v=None, rho0=None, remnant=False, remnant_only=False,
+                         kv_corner=None, kv_corner_prandtl=1.0, hvel_mom6=False, hvel_harmonic=False,
+                         hvel_upwind=True, bbl_glue=False, bbl_per_face=False, hbbl_visc=10.0, bbl_piston=3.0e-4,
+                         kv_bbl_bg=1.0e-4, kv_bbl_u=None, kv_bbl_v=None, bbl_thick_u=None, bbl_thick_v=None,
+                         zlevel_faces=False, k_top_u=None, k_bot_u=None, k_top_v=None, k_bot_v=None):
+    """The dispatch of vdiff_apply_momentum (rdb_ocean_vdiff.F90:455-718)
+    over diffuse_velocity_columns_impl.  h (nx, ny, nz), u (nx+1, ny, nz),
+    v (nx, ny+1, nz), wet (nx, ny); k_top/k_bot face indices 1-based as the
+    reference's ms%k_top_u (default nz and 1).  Returns (u, v, visc_rem_u,
+    visc_rem_v) as new arrays (visc_rem None unless `remnant`)."""
+    nx, ny, nz = h.shape
+    L = _mom_lib()
+    u = np.array(u, dtype=np.float64, order="F", copy=True)
+    v = np.array(v, dtype=np.float64, order="F", copy=True)
+    do_stress = implicit_stress and tau_u is not None and tau_v is not None
+    do_drag = implicit_drag and lambda_bot_u is not None and lambda_bot_v is not None
+    do_top = (implicit_top_drag and lambda_top_u is not None and lambda_top_v is not None and cover_u is not None
+              and cover_v is not None)
+    solve = not remnant_only
+    if not solve and not remnant:
+        return u, v, None, None
+    if do_stress and rho0 is None:
+        raise ValueError("implicit_stress with tau: rho0 is required")
+    do_corner = kv_corner is not None
+    if kv_source is None:
+        if (K_v_momentum <= 0.0 and not do_stress and not do_drag and not do_top and not remnant and not do_corner
+                and not bbl_glue):
+            return u, v, None, None
+        kv = np.full((nx, ny, nz + 1), K_v_momentum, order="F")
+        kv[:, :, 0] = 0.0
+        kv[:, :, nz] = 0.0
+    else:
+        kv = _f(kv_source, (nx, ny, nz + 1))
+    o = MomOpts(int(do_stress), int(do_drag), int(do_top), int(solve), int(remnant), int(hvel_mom6),
+                int(hvel_harmonic), int(hvel_upwind), int(bbl_glue), int(zlevel_faces), int(do_corner),
+                int(use_harmonic), float(rho0 if rho0 is not None else 1.0), float(hbbl_visc), float(kv_bbl_bg),
+                float(kv_corner_prandtl))
+    hf = _f(h, (nx, ny, nz))
+    wetf = _f(wet, (nx, ny))
+    cn = _f(kv_corner, (nx + 1, ny + 1, nz + 1))
+    rems = []
+    for x_face, arr, (nu, nv), tau, lb, lt, cov, kvb, bth, kt, kbt in (
+            (1, u, (nx + 1, ny), tau_u, lambda_bot_u, lambda_top_u, cover_u, kv_bbl_u, bbl_thick_u, k_top_u, k_bot_u),
+            (0, v, (nx, ny + 1), tau_v, lambda_bot_v, lambda_top_v, cover_v, kv_bbl_v, bbl_thick_v, k_top_v,
+             k_bot_v)):
+        if bbl_glue and not bbl_per_face:
+            kvb, bth = np.full((nu, nv), bbl_piston * hbbl_visc), np.full((nu, nv), hbbl_visc)
+        ktop = np.asfortranarray((np.full((nu, nv), nz) if kt is None else np.asarray(kt)).astype(np.int32) - 1)
+        kbot = np.asfortranarray((np.ones((nu, nv)) if kbt is None else np.asarray(kbt)).astype(np.int32) - 1)
+        rem = np.zeros((nu, nv, nz), order="F")
+        cs = np.zeros((nu, nv, nz), order="F")
+        ds = np.zeros((nu, nv, nz), order="F")
+        args = [_f(tau, (nu, nv)), _f(lb, (nu, nv)), _f(lt, (nu, nv)), _f(cov, (nu, nv)), rem,
+                _f(kvb, (nu, nv)), _f(bth, (nu, nv)), cn]
+        L.orc_diffuse_velocity_columns(ctypes.byref(o), nu, nv, nz, dt, _p(arr), _p(hf), _p(kv), _p(wetf), x_face,
+                                       nx, ny, *[_p(a) for a in args], _pi(ktop), _pi(kbot), _p(cs), _p(ds))
+        rems.append(rem if remnant else None)
+    return u, v, rems[0], rems[1]
