@@ -257,6 +257,55 @@ int rdb_colphys_heat_salt_totals(rdb_colphys *h, double *heat, double *salt);
 int rdb_colphys_set_timing(rdb_colphys *h, int enable_timing);
 int rdb_colphys_stats(rdb_colphys *h, rdb_colphys_counters *out);
 
+/* ---- momentum vertical viscosity ----------------------------------------
+ * Backward-Euler vertical viscosity of the face velocities u_face_x_layer and
+ * v_face_y_layer: vdiff_apply_momentum -> diffuse_velocity_columns_impl
+ * (src/parameterizations/vertical/rdb_ocean_vdiff.F90:455-718, 1416-2115),
+ * the other consumer of kv.  Knob names follow ocean_vdiff_t
+ * (rdb_ocean_vdiff.F90:77-280); rdb_colphys_momentum_config_default fills its
+ * defaults.  The face-located inputs of the folds are fields of the handle
+ * (rdb_colphys_set_field):
+ *   "lambda_bot_u" (nx+1, ny, 1), "lambda_bot_v" (nx, ny+1, 1)  bottom drag rate, 1/s
+ *   "lambda_top_u", "lambda_top_v"                              ice-shelf top drag rate
+ *   "cover_u", "cover_v"                                        face ice cover (0 open, 1 covered)
+ *   "kv_bbl_u", "kv_bbl_v", "bbl_thick_u", "bbl_thick_v"         per-face BBL (bbl_per_face)
+ *   "k_top_u", "k_bot_u", "k_top_v", "k_bot_v"                  ms%k_top_u etc., 1-based
+ *                                                               layer indices stored as double
+ *                                                               (default nz and 1)
+ *   "kv_corner" (nx+1, ny+1, nz+1)                              corner viscosity (use_corner)
+ * the wind stress is "tau_x"/"tau_y".  Outputs: the face velocities, and
+ * with `remnant` the viscous remnant "visc_rem_u" (nx+1, ny, nz) and
+ * "visc_rem_v" (nx, ny+1, nz) (get only). */
+typedef struct rdb_colphys_momentum_config {
+  int use_kv_source;     /* 1: the closure's kv (vmix); 0: K_v_momentum everywhere (1) */
+  int pad0;
+  double K_v_momentum;   /* scalar viscosity, m^2/s (0) */
+  int implicit_stress;   /* fold tau/rho0 into the top row (0) */
+  int implicit_drag;     /* fold lambda_bot into the bed row (0; needs lambda_bot_u/v) */
+  int implicit_top_drag; /* fold lambda_top and mask the wind by 1 - cover (0; needs lambda_top, cover) */
+  int hvel_mom6;         /* MOM6 face thickness and h_shear (0) */
+  int hvel_harmonic;     /* MOM6 HARMONIC_VISC branch (0) */
+  int hvel_upwind;       /* near-bed upwind blend of the harmonic branch (1) */
+  int bbl_glue;          /* MOM6 bottom-boundary-layer coupling (0; needs hvel_mom6) */
+  int bbl_per_face;      /* 1: kv_bbl/bbl_thick fields; 0: bbl_piston*hbbl_visc over hbbl_visc (0) */
+  double hbbl_visc;      /* 10 m */
+  double bbl_piston;     /* 3e-4 m/s */
+  double kv_bbl_bg;      /* 1e-4 m^2/s */
+  int zlevel_faces;      /* z-level partial-step faces: min thickness, cut couplings (0) */
+  int use_corner;        /* add kv_corner_prandtl * the corner viscosity (0) */
+  double kv_corner_prandtl; /* 1 */
+  int remnant;           /* also solve for the viscous remnant visc_rem_u/v (0) */
+  int remnant_only;      /* the remnant without changing u and v (0) */
+} rdb_colphys_momentum_config;
+
+int rdb_colphys_momentum_config_default(rdb_colphys_momentum_config *cfg);
+
+/* One backward-Euler step of vertical viscosity on the face velocities with
+ * the handle's kv (use_kv_source) or K_v_momentum, h_layer and wet mask; the
+ * face velocities the next column step reads are updated with it.  The
+ * tracer diffusivity option use_harmonic of the handle's config applies. */
+int rdb_colphys_vdiff_momentum(rdb_colphys *h, double dt, const rdb_colphys_momentum_config *cfg);
+
 /* ---- learned closure (no counterpart in the reference) -----------------
  * A multilayer perceptron predicts, per wet column, log10 kt and log10 kv at
  * the interior interfaces and the KPP boundary-layer depth from the column

@@ -20,7 +20,8 @@ CSRC = PKG / "csrc"
 LIB_DIR = PKG / "lib"
 LIB = LIB_DIR / "librdb_colphys.so"
 
-SOURCES = ["colphys_kernels.hip", "colphys_api.hip", "mlp_kernels.hip", "mlp_net.hip", "mlp_api.hip", "surrogate.hip"]
+SOURCES = ["colphys_kernels.hip", "colphys_api.hip", "mlp_kernels.hip", "mlp_net.hip", "mlp_api.hip", "surrogate.hip",
+           "momentum.hip"]
 ARCH = os.environ.get("RDB_COLPHYS_ARCH", "gfx950")
 
 

@@ -183,6 +183,7 @@ void release(rdb_colphys* h) {
   h->members.clear();
   colphys::surr_free(h->surr);
   h->surr = nullptr;
+  colphys::mom_release(h);
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
   for (hipEvent_t e : h->events) (void)hipEventDestroy(e);
@@ -760,8 +761,17 @@ int rdb_colphys_set_field(rdb_colphys* h, const char* name, const double* data, 
   if (!name || !data) return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "set_field: NULL name or data");
   FieldSpec spec;
   int tracer;
-  if (!lookup_field(name, h->n_passive, spec, tracer))
-    return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "set_field: unknown field '%s'", name);
+  if (!lookup_field(name, h->n_passive, spec, tracer)) {
+    int m1, m2, m3;
+    bool settable;
+    if (!colphys::mom_field_spec(h, name, &m1, &m2, &m3, &settable))
+      return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "set_field: unknown field '%s'", name);
+    if (!settable) return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "set_field: '%s' is read-only", name);
+    if (n1 != m1 || n2 != m2 || n3 != m3)
+      return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "set_field '%s': shape (%d,%d,%d), expected (%d,%d,%d)", name, n1,
+                        n2, n3, m1, m2, m3);
+    return colphys::mom_set_field(h, name, data, (size_t)m1 * m2 * m3);
+  }
   if (!spec.settable) return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "set_field: '%s' is read-only", name);
   const int nx = h->nx, ny = h->ny, nz = h->nz;
   int e1 = nx, e2 = ny, e3 = 1;
@@ -821,11 +831,13 @@ int rdb_colphys_set_field(rdb_colphys* h, const char* name, const double* data, 
     }
     case F_FACE_X:
       std::copy(data, data + n, h->h_u.begin());
+      if (int e = colphys::mom_face_set(h, true)) return e;
       hipLaunchKernelGGL(colphys::k_pack_centre_x, dim3(grid1(ncol, B).x, nz), b, 0, h->stream, h->d_stage, h->s.ub,
                          h->d_col, ncol, P, nz, nx, ny);
       break;
     case F_FACE_Y:
       std::copy(data, data + n, h->h_v.begin());
+      if (int e = colphys::mom_face_set(h, false)) return e;
       hipLaunchKernelGGL(colphys::k_pack_centre_y, dim3(grid1(ncol, B).x, nz), b, 0, h->stream, h->d_stage, h->s.vb,
                          h->d_col, ncol, P, nz, nx, ny);
       break;
@@ -863,8 +875,16 @@ int rdb_colphys_get_field(rdb_colphys* h, const char* name, double* out, int n1,
   if (!name || !out) return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "get_field: NULL name or buffer");
   FieldSpec spec;
   int tracer;
-  if (!lookup_field(name, h->n_passive, spec, tracer))
-    return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "get_field: unknown field '%s'", name);
+  if (!lookup_field(name, h->n_passive, spec, tracer)) {
+    int m1, m2, m3;
+    bool settable;
+    if (!colphys::mom_field_spec(h, name, &m1, &m2, &m3, &settable))
+      return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "get_field: unknown field '%s'", name);
+    if (n1 != m1 || n2 != m2 || n3 != m3)
+      return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "get_field '%s': shape (%d,%d,%d), expected (%d,%d,%d)", name, n1,
+                        n2, n3, m1, m2, m3);
+    return colphys::mom_get_field(h, name, out, (size_t)m1 * m2 * m3);
+  }
   const int nx = h->nx, ny = h->ny, nz = h->nz;
   int e1 = nx, e2 = ny, e3 = 1;
   switch (spec.kind) {
@@ -881,8 +901,14 @@ int rdb_colphys_get_field(rdb_colphys* h, const char* name, double* out, int n1,
                       n2, n3, e1, e2, e3);
   const size_t n = (size_t)e1 * e2 * e3;
   switch (spec.kind) {
-    case F_FACE_X: std::copy(h->h_u.begin(), h->h_u.end(), out); return RDB_COLPHYS_OK;
-    case F_FACE_Y: std::copy(h->h_v.begin(), h->h_v.end(), out); return RDB_COLPHYS_OK;
+    case F_FACE_X:
+      if (int e = colphys::mom_face_get(h, true)) return e;
+      std::copy(h->h_u.begin(), h->h_u.end(), out);
+      return RDB_COLPHYS_OK;
+    case F_FACE_Y:
+      if (int e = colphys::mom_face_get(h, false)) return e;
+      std::copy(h->h_v.begin(), h->h_v.end(), out);
+      return RDB_COLPHYS_OK;
     case F_TAU_X: std::copy(h->h_taux.begin(), h->h_taux.end(), out); return RDB_COLPHYS_OK;
     case F_TAU_Y: std::copy(h->h_tauy.begin(), h->h_tauy.end(), out); return RDB_COLPHYS_OK;
     default: break;
@@ -1346,6 +1372,16 @@ int rdb_colphys_mlp_load(rdb_colphys* h, const char* path) {
 int rdb_colphys_set_guard(rdb_colphys* h, int mode) {
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "set_guard: bad handle");
   return colphys::surr_set_guard(h, mode);
+}
+
+int rdb_colphys_vdiff_momentum(rdb_colphys* h, double dt, const rdb_colphys_momentum_config* cfg) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "vdiff_momentum: bad handle");
+  if (!h->members.empty())
+    return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "vdiff_momentum: an ensemble handle is reached per member "
+                      "(rdb_colphys_member_*) and stepped with rdb_colphys_step");
+  if (!cfg) return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "vdiff_momentum: NULL config");
+  if (!(dt > 0.0) || !std::isfinite(dt)) return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "vdiff_momentum: dt = %g", dt);
+  return colphys::mom_solve(h, dt, cfg);
 }
 
 int rdb_colphys_cbrt(const double* x, double* y, int n) {

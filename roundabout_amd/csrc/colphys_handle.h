@@ -11,6 +11,15 @@
 
 namespace colphys {
 struct SurrState;
+struct MomState;
+// momentum vertical viscosity (momentum.hip)
+int mom_solve(rdb_colphys* h, double dt, const rdb_colphys_momentum_config* c);
+void mom_release(rdb_colphys* h);
+int mom_face_set(rdb_colphys* h, bool x_face);
+int mom_face_get(rdb_colphys* h, bool x_face);
+int mom_field_spec(rdb_colphys* h, const char* name, int* e1, int* e2, int* e3, bool* settable);
+int mom_set_field(rdb_colphys* h, const char* name, const double* data, size_t n);
+int mom_get_field(rdb_colphys* h, const char* name, double* out, size_t n);
 // shared by the C ABI translation units
 int launch_step(rdb_colphys* h, double dt, int p0, int p1);
 int refresh_static(rdb_colphys* h, double dt);
@@ -73,6 +82,9 @@ struct rdb_colphys {
   // learned closure (trained or loaded network, dataset, guard state)
   double* d_coriolis = nullptr;
   colphys::SurrState* surr = nullptr;
+  // momentum vertical viscosity: face velocities and fold inputs on the
+  // device, created on first use
+  colphys::MomState* mom = nullptr;
   long long last_fallback = 0;
   double last_mlp_ms = 0.0;
   double mlp_flop = 0.0;  // per step, unpadded layer widths

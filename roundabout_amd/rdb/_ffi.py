@@ -107,6 +107,32 @@ class MlpConfig(ctypes.Structure):
     ]
 
 
+class MomentumConfig(ctypes.Structure):
+    """ctypes mirror of ``rdb_colphys_momentum_config``."""
+
+    _fields_ = [
+        ("use_kv_source", ctypes.c_int),
+        ("pad0", ctypes.c_int),
+        ("K_v_momentum", ctypes.c_double),
+        ("implicit_stress", ctypes.c_int),
+        ("implicit_drag", ctypes.c_int),
+        ("implicit_top_drag", ctypes.c_int),
+        ("hvel_mom6", ctypes.c_int),
+        ("hvel_harmonic", ctypes.c_int),
+        ("hvel_upwind", ctypes.c_int),
+        ("bbl_glue", ctypes.c_int),
+        ("bbl_per_face", ctypes.c_int),
+        ("hbbl_visc", ctypes.c_double),
+        ("bbl_piston", ctypes.c_double),
+        ("kv_bbl_bg", ctypes.c_double),
+        ("zlevel_faces", ctypes.c_int),
+        ("use_corner", ctypes.c_int),
+        ("kv_corner_prandtl", ctypes.c_double),
+        ("remnant", ctypes.c_int),
+        ("remnant_only", ctypes.c_int),
+    ]
+
+
 GUARD_STRICT = 0
 GUARD_SURFACE = 1
 GUARD_RANGE = 2
@@ -227,6 +253,8 @@ PROTOTYPES = [
      [_ip, ctypes.c_int, _fp, ctypes.c_int, _fp, _fp, _fp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
       ctypes.c_double, ctypes.c_double, _fp]),
     ("rdb_colphys_cbrt", ctypes.c_int, [_dp, _dp, ctypes.c_int]),
+    ("rdb_colphys_momentum_config_default", ctypes.c_int, [ctypes.POINTER(MomentumConfig)]),
+    ("rdb_colphys_vdiff_momentum", ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(MomentumConfig)]),
     ("rdb_mlp_eval_noalloc", ctypes.c_int,
      [_ip, ctypes.c_int, _fp, ctypes.c_int, _fp, _fp]),
 ]

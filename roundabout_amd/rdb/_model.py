@@ -21,6 +21,9 @@ _ip = ctypes.POINTER(ctypes.c_int)
 LAYER_FIELDS = ("h_layer", "temperature", "salinity", "rho_layer")
 IFACE_FIELDS = ("kv", "kt", "ks", "gamma_t", "gamma_s")
 COLUMN_FIELDS = ("stress_shelf", "Q_heat", "Q_salt", "bl_depth", "stress_mag", "b0", "kbl", "coriolis")
+# face-located inputs of the momentum solve's folds (rdb_colphys_vdiff_momentum)
+U_FACE_2D = ("lambda_bot_u", "lambda_top_u", "cover_u", "kv_bbl_u", "bbl_thick_u", "k_top_u", "k_bot_u")
+V_FACE_2D = ("lambda_bot_v", "lambda_top_v", "cover_v", "kv_bbl_v", "bbl_thick_v", "k_top_v", "k_bot_v")
 
 
 class ColumnPhysics:
@@ -109,10 +112,16 @@ class ColumnPhysics:
             return nx + 1, ny, nz
         if name == "v_face_y_layer":
             return nx, ny + 1, nz
-        if name == "tau_x":
+        if name in ("tau_x",) + U_FACE_2D:
             return nx + 1, ny, 1
-        if name == "tau_y":
+        if name in ("tau_y",) + V_FACE_2D:
             return nx, ny + 1, 1
+        if name == "visc_rem_u":
+            return nx + 1, ny, nz
+        if name == "visc_rem_v":
+            return nx, ny + 1, nz
+        if name == "kv_corner":
+            return nx + 1, ny + 1, nz + 1
         return nx, ny, 1  # 2-D column fields; unknown names get NOT_FOUND from the library
 
     def set_field(self, name: str, data) -> None:
@@ -128,7 +137,7 @@ class ColumnPhysics:
         out = np.empty((n1, n2, n3), order="F")
         st = self._lib.rdb_colphys_get_field(self._handle(), name.encode(), out.ctypes.data_as(_dp), n1, n2, n3)
         _ffi.check(st, self._lib)
-        if name in COLUMN_FIELDS or name in ("tau_x", "tau_y"):
+        if name in COLUMN_FIELDS or name in ("tau_x", "tau_y") + U_FACE_2D + V_FACE_2D:
             out = out[:, :, 0]
         if name == "kbl":
             out = out.astype(np.int32)
@@ -158,6 +167,18 @@ class ColumnPhysics:
         if st not in (_ffi.OK, _ffi.GUARD_TRIPPED):
             _ffi.check(st, self._lib)
         return st
+
+    def vdiff_momentum(self, dt: float, **overrides) -> None:
+        """Backward-Euler vertical viscosity of the face velocities
+        (vdiff_apply_momentum); keyword arguments override
+        rdb_colphys_momentum_config (use_kv_source=1 by default)."""
+        cfg = _ffi.MomentumConfig()
+        _ffi.check(self._lib.rdb_colphys_momentum_config_default(ctypes.byref(cfg)), self._lib)
+        for k, v in overrides.items():
+            if not hasattr(cfg, k):
+                raise AttributeError(f"rdb_colphys_momentum_config has no field {k!r}")
+            setattr(cfg, k, v)
+        _ffi.check(self._lib.rdb_colphys_vdiff_momentum(self._handle(), float(dt), ctypes.byref(cfg)), self._lib)
 
     def synchronize(self) -> None:
         _ffi.check(self._lib.rdb_colphys_synchronize(self._handle()), self._lib)
