@@ -143,6 +143,28 @@ def cpu_baseline(ocean, budget_s: float, crop_cols: tuple[int, int]):
     }
 
 
+def momentum_leg(cp, ocean, reps: int = 10) -> dict:
+    """The momentum vertical viscosity (rdb_colphys_vdiff_momentum, the
+    closure's kv, ocean_vdiff_t defaults) on the same tile: wall time per
+    call over `reps` calls, each including its natural-order copies of h and
+    kv and the refresh of the centred velocities."""
+    cp.vdiff_momentum(DT)  # first call: allocates the face buffers
+    cp.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        cp.vdiff_momentum(DT)
+    cp.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    nx, ny, nz = ocean.nx, ocean.ny, ocean.nz
+    faces = (nx + 1) * ny + nx * (ny + 1)
+    # per face and level: u read and written, h and kv of the two cells
+    alg = faces * nz * 6 * 8
+    return {"ms_per_call": ms, "face_columns": faces, "nz": nz,
+            "face_column_steps_per_s": faces / (ms * 1e-3),
+            "algorithmic_GBs": alg / (ms * 1e-3) / 1e9,
+            "config": "kv of the plain closure, ocean_vdiff_t defaults (no folds)"}
+
+
 def live_rms(ocean, a, b) -> float:
     live = (ocean.h > 1.5e-4) & (ocean.wet > 0.5)[:, :, None]
     d = (a - b)[live]
@@ -530,6 +552,9 @@ def main() -> int:
         },
         "heat_salt_totals": [heat, salt],
     }
+    if args.precision == "fp64":
+        log("[bench] momentum vertical viscosity ...")
+        out["momentum"] = momentum_leg(cp, ocean)
     cp.close()
     if not args.no_surrogate and args.precision == "fp64":
         log("[bench] learned closure leg ...")

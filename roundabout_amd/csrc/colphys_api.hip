@@ -811,6 +811,7 @@ int rdb_colphys_set_field(rdb_colphys* h, const char* name, const double* data, 
       h->col_live[(size_t)c] = nz - kb;
     }
     h->live_levels = live;
+    ++h->h_gen;
   }
   HIP_TRY(hipMemcpyAsync(h->d_stage, data, 8 * n, hipMemcpyHostToDevice, h->stream));
   const int ncol = h->n_all, P = h->P, B = 256;

@@ -85,6 +85,7 @@ struct rdb_colphys {
   // momentum vertical viscosity: face velocities and fold inputs on the
   // device, created on first use
   colphys::MomState* mom = nullptr;
+  long long h_gen = 0;  // bumped whenever h_layer is set (natural-order copies follow it)
   long long last_fallback = 0;
   double last_mlp_ms = 0.0;
   double mlp_flop = 0.0;  // per step, unpadded layer widths

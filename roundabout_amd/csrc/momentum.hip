@@ -299,6 +299,7 @@ struct MomState {
   bool faces_on_device = false;  // uf/vf hold the current face velocities
   bool u_host_stale = false, v_host_stale = false;
   double* h_nat = nullptr;   // [nz][C]
+  long long h_gen = -1;      // the handle's h_gen h_nat was copied at (-1: never)
   double* kv_nat = nullptr;  // [nz + 1][C]
   double* wet_nat = nullptr;
   double* tau_u = nullptr;
@@ -493,13 +494,18 @@ int mom_solve(rdb_colphys* h, double dt, const rdb_colphys_momentum_config* c) {
   }
   // the cell fields in natural order (the packed state is permuted by depth)
   const int ncol = h->n_all, P = h->P, B = 256;
-  hipLaunchKernelGGL(k_unpack_layer, dim3(mgrid(ncol, B).x, (unsigned)nz), dim3(B), 0, h->stream, h->s.h, m->h_nat,
-                     h->d_col, ncol, P, (int)nz, (long long)C);
+  // (h and the wet mask only when h_layer was set since the last copy; the
+  // wet mask is fixed at create)
+  if (m->h_gen != h->h_gen) {
+    hipLaunchKernelGGL(k_unpack_layer, dim3(mgrid(ncol, B).x, (unsigned)nz), dim3(B), 0, h->stream, h->s.h,
+                       m->h_nat, h->d_col, ncol, P, (int)nz, (long long)C);
+    hipLaunchKernelGGL(k_unpack_layer, dim3(mgrid(ncol, B).x, 1), dim3(B), 0, h->stream, h->s.wet, m->wet_nat,
+                       h->d_col, ncol, P, 1, (long long)C);
+    m->h_gen = h->h_gen;
+  }
   if (c->use_kv_source)
     hipLaunchKernelGGL(k_unpack_layer, dim3(mgrid(ncol, B).x, (unsigned)(nz + 1)), dim3(B), 0, h->stream, h->s.kv,
                        m->kv_nat, h->d_col, ncol, P, (int)nz + 1, (long long)C);
-  hipLaunchKernelGGL(k_unpack_layer, dim3(mgrid(ncol, B).x, 1), dim3(B), 0, h->stream, h->s.wet, m->wet_nat, h->d_col,
-                     ncol, P, 1, (long long)C);
   MOM_TRY("unpack", hipGetLastError());
   if (do_stress) {
     MOM_TRY("tau upload", hipMemcpyAsync(m->tau_u, h->h_taux.data(), 8 * m->Fu, hipMemcpyHostToDevice, h->stream));

@@ -23,6 +23,8 @@ program colphys_fortran_check
    integer :: i, j, k, m
    integer(c_int) :: n_mem, enx(2), eny(2), enz(2)
    type(c_ptr) :: ewet(2)
+   type(rdb_colphys_momentum_config_t) :: mcfg
+   real(c_double) :: u1(nx + 1, ny, nz), hf(nz), m0, m1, mabs
 
    call get_command_argument(1, mode)
    st = rdb_colphys_config_default(cfg)
@@ -87,6 +89,26 @@ program colphys_fortran_check
    st = colphys_get(h, "temperature", T1)
    st = colphys_get(h, "kt", kt1)
    st = rdb_colphys_heat_salt_totals(h, heat, salt)
+   ! momentum vertical viscosity with the closure's kv: the flux form keeps
+   ! sum(hf * u) of every interior u-face column
+   st = rdb_colphys_momentum_config_default(mcfg)
+   if (st /= 0 .or. mcfg%use_kv_source /= 1 .or. mcfg%hbbl_visc /= 10.0_c_double) stop 13
+   st = rdb_colphys_vdiff_momentum(h, 3600.0_c_double, mcfg)
+   if (st /= 0) then
+      print '(a)', trim(colphys_last_error())
+      stop 14
+   end if
+   if (colphys_get(h, "u_face_x_layer", u1) /= 0) stop 15
+   if (all(u1 == u)) stop 16
+   do j = 1, ny
+      do i = 2, nx
+         hf = 0.5_c_double*(hl(i - 1, j, :) + hl(i, j, :))
+         m0 = sum(hf*u(i, j, :))
+         m1 = sum(hf*u1(i, j, :))
+         mabs = sum(abs(hf*u(i, j, :)))
+         if (abs(m1 - m0) > 1.0e-13_c_double*mabs) stop 17
+      end do
+   end do
    st = rdb_colphys_destroy(h)
    if (c_associated(h)) stop 12
 

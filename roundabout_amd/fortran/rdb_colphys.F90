@@ -55,6 +55,19 @@ module rdb_colphys
       real(c_double) :: val_fraction
    end type rdb_colphys_mlp_config_t
 
+   !> Mirror of rdb_colphys_momentum_config (vdiff_apply_momentum knobs).
+   type, bind(c), public :: rdb_colphys_momentum_config_t
+      integer(c_int) :: use_kv_source, pad0
+      real(c_double) :: K_v_momentum
+      integer(c_int) :: implicit_stress, implicit_drag, implicit_top_drag
+      integer(c_int) :: hvel_mom6, hvel_harmonic, hvel_upwind, bbl_glue, bbl_per_face
+      real(c_double) :: hbbl_visc, bbl_piston, kv_bbl_bg
+      integer(c_int) :: zlevel_faces, use_corner
+      real(c_double) :: kv_corner_prandtl
+      integer(c_int) :: remnant, remnant_only
+   end type rdb_colphys_momentum_config_t
+
+   public :: rdb_colphys_momentum_config_default, rdb_colphys_vdiff_momentum
    public :: rdb_colphys_ensemble_create, rdb_colphys_member_count
    public :: rdb_colphys_member_set_field, rdb_colphys_member_get_field
    public :: colphys_member_set, colphys_member_get
@@ -231,6 +244,19 @@ module rdb_colphys
          import :: c_int, c_ptr
          type(c_ptr), value :: handle
          integer(c_int), value :: mode
+      end function
+      integer(c_int) function rdb_colphys_momentum_config_default(cfg) &
+         bind(c, name="rdb_colphys_momentum_config_default")
+         import :: c_int, rdb_colphys_momentum_config_t
+         type(rdb_colphys_momentum_config_t), intent(out) :: cfg
+      end function
+      !> vdiff_apply_momentum (rdb_ocean_vdiff.F90:455-718) on the handle's
+      !> face velocities, with its kv or K_v_momentum.
+      integer(c_int) function rdb_colphys_vdiff_momentum(handle, dt, cfg) bind(c, name="rdb_colphys_vdiff_momentum")
+         import :: c_int, c_double, c_ptr, rdb_colphys_momentum_config_t
+         type(c_ptr), value :: handle
+         real(c_double), value :: dt
+         type(rdb_colphys_momentum_config_t), intent(in) :: cfg
       end function
       integer(c_int) function rdb_colphys_last_error(idx, buf, cap) bind(c, name="rdb_colphys_last_error")
          import :: c_int, c_char
