@@ -170,6 +170,90 @@ int rdb_colphys_create(const rdb_colphys_config *cfg, int nx_tot, int ny_tot,
 /* Idempotent: NULL or an already-destroyed handle is a no-op success. */
 int rdb_colphys_destroy(rdb_colphys **handle);
 
+/* ---- create from namelist text (mirrors rdb_ocean_create_from_string,
+ *      include/rdb_ocean.h:108-109) ---------------------------------------
+ * nml_text need not be NUL-terminated; nml_len is authoritative.  Read:
+ *   &ocean_vmix_nml  (src/core/config/rdb_config_register_vmix.F90:738-870):
+ *       use_kpp, pp81_nu0, pp81_nu_bg (also kv_bg), pp81_kappa_bg (also
+ *       kt_bg and ks_bg), pp81_alpha, shear2_floor, kpp_ri_crit,
+ *       kpp_cs_nonlocal, kpp_c_vt2, kv_max, kd_max, vmix_guard,
+ *       harmonic_visc, kv_ml_invz2, kd_smooth_iterations; switches of
+ *       branches this engine does not run (use_closure = .false.,
+ *       interior_closure = 'none', direct_stress, bkgnd_profile,
+ *       bkgnd_henyey, bkgnd_prandtl != -1, buoyancy_coeffs = 'eos') return
+ *       RDB_COLPHYS_ERR_CONFIG_VALIDATE; the driver's time-step and
+ *       advection knobs are accepted and have no effect;
+ *   &ocean_eos_nml   eos ('wright' only), c_p, p_ref;
+ *   &ocean_ic_nml    rho_0;
+ *   &ocean_vdiff_nml (rdb_colphys_momentum_config_from_string) the momentum
+ *       switches implicit_stress, implicit_drag, implicit_top_drag,
+ *       hvel_mom6, hvel_harmonic, hvel_upwind, bbl_glue, hbbl_visc,
+ *       bbl_piston, accel_visc_rem (-> remnant).
+ * Other groups of a model namelist are skipped; an unknown key in a group
+ * read here is RDB_COLPHYS_ERR_CONFIG_PARSE.  Unlisted knobs keep the
+ * reference defaults.  The grid, wet mask, k_top and sort key are given as
+ * for rdb_colphys_create. */
+int rdb_colphys_config_from_string(const char *nml_text, int nml_len, rdb_colphys_config *cfg);
+int rdb_colphys_create_from_string(const char *nml_text, int nml_len, int nx_tot, int ny_tot, int nz,
+                                   int n_passive, const double *wet, const int *k_top, const int *sort_key,
+                                   rdb_colphys **handle_out);
+
+/* ---- queries (mirror rdb_ocean_get_time / get_step_count /
+ *      get_grid_info, include/rdb_ocean.h:283-293) ------------------------
+ * Time is the sum of dt over the steps taken (rdb_colphys_step and the
+ * plain steps of rdb_colphys_mlp_collect); nghost is 0 (the extents are the
+ * caller's full arrays). */
+int rdb_colphys_get_time(rdb_colphys *h, double *t_out);
+int rdb_colphys_get_step_count(rdb_colphys *h, int *step_out);
+int rdb_colphys_get_grid_info(rdb_colphys *h, int *nx, int *ny, int *nz, int *nghost);
+
+/* ---- host pointers with a generation (mirror rdb_ocean_refresh_host and
+ *      the rdb_ocean_get_*_ptr getters, include/rdb_ocean.h:321-406) ------
+ * Every getter returns a pointer into a full-extent host mirror of the
+ * field (reference layout, double) with its extents and generation = the
+ * handle's step count at the call, WITHOUT copying from the device (the
+ * first request of a field creates and fills its mirror).  When the
+ * generation differs from the one last seen, call rdb_colphys_refresh_host
+ * before reading: it copies every mirror older than the device state (and
+ * is free when all are current).  rho_layer is the Wright EOS of the
+ * current T, S and h.  The pointers stay valid until destroy. */
+int rdb_colphys_refresh_host(rdb_colphys *h);
+int rdb_colphys_get_h_layer_ptr(rdb_colphys *h, void **ptr, int *nx, int *ny, int *nz, int *gen);
+int rdb_colphys_get_u_face_x_layer_ptr(rdb_colphys *h, void **ptr, int *nx, int *ny, int *nz, int *gen);
+int rdb_colphys_get_v_face_y_layer_ptr(rdb_colphys *h, void **ptr, int *nx, int *ny, int *nz, int *gen);
+int rdb_colphys_get_rho_layer_ptr(rdb_colphys *h, void **ptr, int *nx, int *ny, int *nz, int *gen);
+int rdb_colphys_get_kv_ptr(rdb_colphys *h, void **ptr, int *nx, int *ny, int *nz, int *gen);
+int rdb_colphys_get_kt_ptr(rdb_colphys *h, void **ptr, int *nx, int *ny, int *nz, int *gen);
+int rdb_colphys_get_ks_ptr(rdb_colphys *h, void **ptr, int *nx, int *ny, int *nz, int *gen);
+int rdb_colphys_get_wet_t_ptr(rdb_colphys *h, void **ptr, int *nx, int *ny, int *gen);
+int rdb_colphys_get_tau_x_ptr(rdb_colphys *h, void **ptr, int *nx, int *ny, int *gen);
+int rdb_colphys_get_tau_y_ptr(rdb_colphys *h, void **ptr, int *nx, int *ny, int *gen);
+int rdb_colphys_get_q_heat_ptr(rdb_colphys *h, void **ptr, int *nx, int *ny, int *gen);
+int rdb_colphys_get_q_salt_ptr(rdb_colphys *h, void **ptr, int *nx, int *ny, int *gen);
+
+/* ---- tracers by name (mirror rdb_ocean_get_tracer_count /
+ *      list_tracers / get_tracer_ptr, include/rdb_ocean.h:393-406) -------
+ * The registry is temperature, salinity, tracer_1 .. tracer_<n_passive>;
+ * list_tracers has snprintf semantics and returns the full name length. */
+int rdb_colphys_get_tracer_count(rdb_colphys *h, int *count_out);
+int rdb_colphys_list_tracers(rdb_colphys *h, int idx, char *buf, int cap);
+int rdb_colphys_get_tracer_ptr(rdb_colphys *h, const char *name, int name_len, void **ptr, int *nx, int *ny,
+                               int *nz, int *gen);
+
+/* ---- narrow setters (mirror rdb_ocean_set_h/u/v/wind/heat_flux/
+ *      salt_flux/tracer, include/rdb_ocean.h:408-446) ---------------------
+ * Each pushes only what it changes.  Shapes as the reference's: nx, ny, nz
+ * are the cell extents, face arrays carry the +1 in their staggered
+ * dimension. */
+int rdb_colphys_set_h(rdb_colphys *h, const double *h_data, int nx, int ny, int nz);
+int rdb_colphys_set_u(rdb_colphys *h, const double *u_data, int nx, int ny, int nz);
+int rdb_colphys_set_v(rdb_colphys *h, const double *v_data, int nx, int ny, int nz);
+int rdb_colphys_set_wind(rdb_colphys *h, const double *taux_data, const double *tauy_data, int nx, int ny);
+int rdb_colphys_set_heat_flux(rdb_colphys *h, const double *q_data, int nx, int ny);
+int rdb_colphys_set_salt_flux(rdb_colphys *h, const double *q_data, int nx, int ny);
+int rdb_colphys_set_tracer(rdb_colphys *h, const char *name, int name_len, const double *data, int nx, int ny,
+                           int nz);
+
 /* ---- state transfer (mirrors the rdb_ocean_set_* / get_*_ptr pairs,
  *      include/rdb_ocean.h:338-446, with explicit copies) -------------------
  *
@@ -195,6 +279,7 @@ int rdb_colphys_destroy(rdb_colphys **handle);
  *                                     -1 where the surrogate closure was used),
  *                                     stored as double
  *   "coriolis"       (nx, ny, 1)      Coriolis parameter f (surrogate input)
+ *   "wet_t"          (nx, ny, 1)      get only; the wet mask given at create
  * A shape mismatch returns RDB_COLPHYS_ERR_BAD_SHAPE, an unknown name
  * RDB_COLPHYS_ERR_NOT_FOUND.
  */
@@ -232,6 +317,16 @@ int rdb_colphys_nonlocal(rdb_colphys *h, double dt);
  * tracer diffusion -> non-local.  One launch per step over all wet columns,
  * replayed from a captured hipGraph. */
 int rdb_colphys_step(rdb_colphys *h, double dt, int n_steps);
+
+/* The vdiff budget slots heat_budget_vdiff and salt_budget_vdiff
+ * (src/core/rdb_multilayer_state.F90:403-410): with enable = 1 every tracer
+ * solve (fused step or rdb_colphys_diffuse_tracers) adds the per-cell
+ * content change h*c_new - h*c_old of T and S to them, exactly as
+ * apply_factored_tracer does (rdb_ocean_vdiff.F90:1397-1405).  Enabling
+ * allocates them zeroed; they are read and reset through get/set_field
+ * "heat_budget_vdiff" / "salt_budget_vdiff" (nx, ny, nz).  Off by default
+ * (the step then moves no budget bytes); single grids only. */
+int rdb_colphys_set_budgets(rdb_colphys *h, int enable);
 
 /* Block until the handle's stream is idle. */
 int rdb_colphys_synchronize(rdb_colphys *h);
@@ -299,6 +394,9 @@ typedef struct rdb_colphys_momentum_config {
 } rdb_colphys_momentum_config;
 
 int rdb_colphys_momentum_config_default(rdb_colphys_momentum_config *cfg);
+/* The &ocean_vdiff_nml switches over the defaults (see
+ * rdb_colphys_config_from_string). */
+int rdb_colphys_momentum_config_from_string(const char *nml_text, int nml_len, rdb_colphys_momentum_config *cfg);
 
 /* One backward-Euler step of vertical viscosity on the face velocities with
  * the handle's kv (use_kv_source) or K_v_momentum, h_layer and wet mask; the

@@ -442,23 +442,47 @@ void orc_apply_factored(int nx, int ny, int nz, double *x, const double *a,
 
 /* A11 vdiff_apply_tracers, two-pass dispatch (kt -> T; ks -> S + passives):
  * rdb_ocean_vdiff.F90:1107-1226. */
-void orc_vdiff_tracers(int nx, int ny, int nz, int use_harmonic, double dt,
-                       const double *kt, const double *ks, const double *h,
-                       double *T, double *S, int n_passive, double **passive) {
+/* The budget of apply_factored_tracer (rdb_ocean_vdiff.F90:1397-1405):
+ * budget += h*c_new - h*c_old per cell, the two products standalone. */
+static void add_budget(size_t n, const double *h, const double *c_old, const double *c_new, double *budget) {
+#pragma omp parallel for schedule(static)
+  for (size_t q = 0; q < n; ++q) {
+    double p_new = h[q] * c_new[q];
+    double p_old = h[q] * c_old[q];
+    budget[q] = budget[q] + (p_new - p_old);
+  }
+}
+
+/* A11 with the heat/salt budget slots (NULL: not kept). */
+void orc_vdiff_tracers_budget(int nx, int ny, int nz, int use_harmonic, double dt, const double *kt,
+                              const double *ks, const double *h, double *T, double *S, int n_passive,
+                              double **passive, double *heat_budget, double *salt_budget) {
   size_t n = (size_t)nx * ny * nz;
+  double *old = (heat_budget || salt_budget) ? (double *)malloc(sizeof(double) * n) : NULL;
   double *a = (double *)malloc(sizeof(double) * n);
   double *b = (double *)malloc(sizeof(double) * n);
   double *c = (double *)malloc(sizeof(double) * n);
   double *rhs = (double *)malloc(sizeof(double) * n);
   orc_build_factorize(nx, ny, nz, use_harmonic, dt, kt, h, a, b, c);
+  if (heat_budget) memcpy(old, T, sizeof(double) * n);
   orc_apply_factored(nx, ny, nz, T, a, b, c, rhs);
+  if (heat_budget) add_budget(n, h, old, T, heat_budget);
   orc_build_factorize(nx, ny, nz, use_harmonic, dt, ks, h, a, b, c);
+  if (salt_budget) memcpy(old, S, sizeof(double) * n);
   orc_apply_factored(nx, ny, nz, S, a, b, c, rhs);
+  if (salt_budget) add_budget(n, h, old, S, salt_budget);
   for (int t = 0; t < n_passive; ++t) orc_apply_factored(nx, ny, nz, passive[t], a, b, c, rhs);
+  free(old);
   free(a);
   free(b);
   free(c);
   free(rhs);
+}
+
+void orc_vdiff_tracers(int nx, int ny, int nz, int use_harmonic, double dt,
+                       const double *kt, const double *ks, const double *h,
+                       double *T, double *S, int n_passive, double **passive) {
+  orc_vdiff_tracers_budget(nx, ny, nz, use_harmonic, dt, kt, ks, h, T, S, n_passive, passive, NULL, NULL);
 }
 
 /* nonlocal_face_flux: rdb_ocean_vmix.F90:1694-1714. */
@@ -502,8 +526,8 @@ int orc_column_step(const rdb_colphys_config *cfg, orc_state *st, double dt) {
   int status = orc_assemble(cfg, nx, ny, nz, st->kv, st->kt, st->ks);
   orc_surface_flux(cfg, nx, ny, nz, dt, st->T, st->S, st->h, st->wet, st->Q_heat,
                    st->Q_salt, st->k_top);
-  orc_vdiff_tracers(nx, ny, nz, cfg->use_harmonic, dt, st->kt, st->ks, st->h, st->T,
-                    st->S, st->n_passive, st->passive);
+  orc_vdiff_tracers_budget(nx, ny, nz, cfg->use_harmonic, dt, st->kt, st->ks, st->h, st->T,
+                           st->S, st->n_passive, st->passive, st->heat_budget, st->salt_budget);
   if (cfg->use_kpp) orc_nonlocal(nx, ny, nz, dt, st->gamma_t, st->gamma_s, st->h, st->T, st->S);
   return status;
 }

@@ -61,6 +61,10 @@ void orc_vdiff_tracers(int nx, int ny, int nz, int use_harmonic, double dt,
                        const double *kt, const double *ks, const double *h,
                        double *T, double *S, int n_passive, double **passive);
 
+void orc_vdiff_tracers_budget(int nx, int ny, int nz, int use_harmonic, double dt, const double *kt,
+                              const double *ks, const double *h, double *T, double *S, int n_passive,
+                              double **passive, double *heat_budget, double *salt_budget);
+
 void orc_nonlocal(int nx, int ny, int nz, double dt, const double *gamma_t,
                   const double *gamma_s, const double *h, double *T,
                   double *S);
@@ -76,6 +80,8 @@ typedef struct orc_state {
   int *kbl;
   /* scratch for the tracer solve */
   double *a, *b, *c, *rhs;
+  /* heat/salt_budget_vdiff slots (NULL: not kept) */
+  double *heat_budget, *salt_budget;
 } orc_state;
 
 /* One column step in the order of apply_deferred_thermo

@@ -124,6 +124,9 @@ bool lookup_field(const char* name, int n_passive, FieldSpec& spec, int& tracer)
       {"b0", {F_COL, false, true}},
       {"kbl", {F_COL_INT, false, true}},
       {"coriolis", {F_COL, true, true}},
+      {"wet_t", {F_COL, false, true}},
+      {"heat_budget_vdiff", {F_LAYER, true, true}},
+      {"salt_budget_vdiff", {F_LAYER, true, true}},
   };
   for (const E& e : table)
     if (std::strcmp(e.n, name) == 0) {
@@ -162,6 +165,9 @@ double* layer_ptr(rdb_colphys* h, const char* name, int tracer) {
   if (!std::strcmp(name, "stress_mag")) return h->d_smag;
   if (!std::strcmp(name, "b0")) return s.b0;
   if (!std::strcmp(name, "coriolis")) return h->d_coriolis;
+  if (!std::strcmp(name, "wet_t")) return s.wet;
+  if (!std::strcmp(name, "heat_budget_vdiff")) return s.budT;
+  if (!std::strcmp(name, "salt_budget_vdiff")) return s.budS;
   return nullptr;
 }
 
@@ -773,6 +779,8 @@ int rdb_colphys_set_field(rdb_colphys* h, const char* name, const double* data, 
     return colphys::mom_set_field(h, name, data, (size_t)m1 * m2 * m3);
   }
   if (!spec.settable) return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "set_field: '%s' is read-only", name);
+  if (tracer < 0 && (spec.kind == F_LAYER || spec.kind == F_IFACE || spec.kind == F_COL) && !layer_ptr(h, name, tracer))
+    return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "set_field: '%s' is not kept (rdb_colphys_set_budgets)", name);
   const int nx = h->nx, ny = h->ny, nz = h->nz;
   int e1 = nx, e2 = ny, e3 = 1;
   switch (spec.kind) {
@@ -865,6 +873,16 @@ int rdb_colphys_set_field(rdb_colphys* h, const char* name, const double* data, 
   }
   h->static_dirty = true;
   HIP_TRY(hipStreamSynchronize(h->stream));  // the caller may reuse its buffer
+  ++h->state_ver;
+  {
+    // a host mirror of this field takes the new values directly (as the
+    // reference's setters write the host array they push)
+    auto it = h->mirrors.find(name);
+    if (it != h->mirrors.end() && it->second.size() == n) {
+      std::copy(data, data + n, it->second.begin());
+      h->mirror_ver[name] = h->state_ver;
+    }
+  }
   return RDB_COLPHYS_OK;
 }
 
@@ -915,6 +933,8 @@ int rdb_colphys_get_field(rdb_colphys* h, const char* name, double* out, int n1,
     default: break;
   }
   const int ncol = h->n_all, P = h->P, B = 256;
+  if (spec.kind != F_COL_INT && !layer_ptr(h, name, tracer))
+    return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "get_field: '%s' is not kept (rdb_colphys_set_budgets)", name);
   if (spec.kind == F_COL_INT) {
     hipLaunchKernelGGL(colphys::k_unpack_int, grid1(ncol, B), dim3(B), 0, h->stream, h->s.kbl, h->d_stage, h->d_col,
                        ncol);
@@ -935,6 +955,7 @@ int rdb_colphys_eos(rdb_colphys* h) {
   if (!h->members.empty())
     return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "eos: an ensemble handle is reached per member (rdb_colphys_member_*) "
                       "and stepped with rdb_colphys_step");
+  ++h->state_ver;
   dim3 g(grid1(h->n_all, h->block).x, (unsigned)h->nz);
   hipLaunchKernelGGL(colphys::k_eos, g, dim3(h->block), 0, h->stream, h->s, h->pr, 0, h->n_all);
   return launch_check("k_eos");
@@ -946,6 +967,7 @@ int rdb_colphys_coefficients(rdb_colphys* h, double dt) {
   if (!h->members.empty())
     return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "coefficients: an ensemble handle is reached per member (rdb_colphys_member_*) "
                       "and stepped with rdb_colphys_step");
+  ++h->state_ver;
   int st = staged_coefficients(h, 0, h->n_all);
   if (st) return st;
   h->static_dirty = true;  // the staged pass rewrote every interface
@@ -966,6 +988,7 @@ int rdb_colphys_surface_flux(rdb_colphys* h, double dt) {
   if (!h->members.empty())
     return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "surface_flux: an ensemble handle is reached per member (rdb_colphys_member_*) "
                       "and stepped with rdb_colphys_step");
+  ++h->state_ver;
   if (!h->cfg.has_heat && !h->cfg.has_salt) return RDB_COLPHYS_OK;
   hipLaunchKernelGGL(colphys::k_surface_flux, grid1(h->n_all, h->block), dim3(h->block), 0, h->stream, h->s, h->pr,
                      dt, 0, h->n_all);
@@ -977,6 +1000,7 @@ int rdb_colphys_diffuse_tracers(rdb_colphys* h, double dt) {
   if (!h->members.empty())
     return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "diffuse_tracers: an ensemble handle is reached per member (rdb_colphys_member_*) "
                       "and stepped with rdb_colphys_step");
+  ++h->state_ver;
   hipLaunchKernelGGL(colphys::k_vdiff, grid1(h->n_all, h->block), dim3(h->block), 0, h->stream, h->s, h->pr, dt, 0,
                      h->n_all);
   return launch_check("k_vdiff");
@@ -987,6 +1011,7 @@ int rdb_colphys_nonlocal(rdb_colphys* h, double dt) {
   if (!h->members.empty())
     return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "nonlocal: an ensemble handle is reached per member (rdb_colphys_member_*) "
                       "and stepped with rdb_colphys_step");
+  ++h->state_ver;
   if (!h->cfg.use_kpp) return RDB_COLPHYS_OK;
   dim3 g(grid1(h->n_all, h->block).x, (unsigned)h->nz);
   hipLaunchKernelGGL(colphys::k_nonlocal, g, dim3(h->block), 0, h->stream, h->s, dt, 0, h->n_all);
@@ -1052,6 +1077,8 @@ int rdb_colphys_step(rdb_colphys* h, double dt, int n_steps) {
     }
   }
   h->steps += n_steps;
+  h->t_sim += dt * n_steps;
+  ++h->state_ver;
   if (h->cfg.vmix_guard) {
     unsigned int bad = 0;
     st = read_guard(h, &bad);
@@ -1115,6 +1142,7 @@ int repack(rdb_colphys* h, const std::vector<int>& order) {
   for (double* a : {sv.h, sv.ub, sv.vb, sv.T, sv.S, sv.rho}) perm_d(a, nz);
   perm_d(sv.tr, nz * std::max(h->n_passive, 1));
   for (double* a : {sv.kv, sv.kt, sv.ks, sv.gamma_t, sv.gamma_s}) perm_d(a, nz + 1);
+  for (double* a : {sv.budT, sv.budS}) perm_d(a, nz);
   for (double* a : {sv.tau_tot, sv.Qh, sv.Qs, sv.wet, sv.bl, sv.b0, h->d_txc, h->d_tyc, h->d_smag, h->d_shelf,
                     h->d_coriolis})
     perm_d(a, 1);
@@ -1140,16 +1168,18 @@ int rdb_colphys_autotune(rdb_colphys* h, double dt) {
   // back up the evolving state (T, S, passives, bl_depth) of every grid
   const std::vector<rdb_colphys*> ps = colphys::plans(h);
   size_t nstate = 0;
-  for (rdb_colphys* m : ps) nstate += (size_t)m->nz * (size_t)m->P * (size_t)(2 + m->n_passive) + (size_t)m->P;
+  // (and the budget slots, which the candidate steps add to)
+  for (rdb_colphys* m : ps)
+    nstate += (size_t)m->nz * (size_t)m->P * (size_t)(2 + m->n_passive + (m->s.budT ? 2 : 0)) + (size_t)m->P;
   double* bak = nullptr;
   HIP_TRY(hipMalloc((void**)&bak, 8 * std::max<size_t>(nstate, 1)));
   auto save = [&](bool restore) {
     size_t off = 0;
     for (rdb_colphys* m : ps) {
       const size_t L = (size_t)m->nz * (size_t)m->P;
-      double* parts[4] = {m->s.T, m->s.S, m->s.tr, m->s.bl};
-      size_t sizes[4] = {L, L, L * (size_t)m->n_passive, (size_t)m->P};
-      for (int q = 0; q < 4; ++q) {
+      double* parts[6] = {m->s.T, m->s.S, m->s.tr, m->s.bl, m->s.budT, m->s.budS};
+      size_t sizes[6] = {L, L, L * (size_t)m->n_passive, (size_t)m->P, m->s.budT ? L : 0, m->s.budS ? L : 0};
+      for (int q = 0; q < 6; ++q) {
         if (sizes[q])
           (void)hipMemcpyAsync(restore ? parts[q] : bak + off, restore ? bak + off : parts[q], 8 * sizes[q],
                                hipMemcpyDeviceToDevice, h->stream);
@@ -1375,6 +1405,24 @@ int rdb_colphys_set_guard(rdb_colphys* h, int mode) {
   return colphys::surr_set_guard(h, mode);
 }
 
+int rdb_colphys_set_budgets(rdb_colphys* h, int enable) {
+  if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "set_budgets: bad handle");
+  if (!h->members.empty())
+    return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "set_budgets: the budget slots are kept for single grids only");
+  if (enable && !h->s.budT) {
+    const size_t L = (size_t)h->nz * (size_t)h->P;
+    int st = dalloc(h, (void**)&h->s.budT, 8 * L);  // zeroed
+    if (!st) st = dalloc(h, (void**)&h->s.budS, 8 * L);
+    if (!st) st = dalloc(h, (void**)&h->s.pold, 16 * L);
+    if (st) return st;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  } else if (!enable) {
+    h->s.budT = h->s.budS = h->s.pold = nullptr;  // the buffers stay allocated until destroy
+  }
+  drop_graph(h);  // the captured launches carry the state view
+  return RDB_COLPHYS_OK;
+}
+
 int rdb_colphys_vdiff_momentum(rdb_colphys* h, double dt, const rdb_colphys_momentum_config* cfg) {
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "vdiff_momentum: bad handle");
   if (!h->members.empty())
@@ -1413,3 +1461,7 @@ int rdb_colphys_last_error(int idx, char* buf, int cap) {
 }
 
 }  // extern "C"
+
+namespace colphys {
+bool handle_valid(rdb_colphys* h) { return valid(h); }
+}  // namespace colphys

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -20,6 +21,7 @@ int mom_face_get(rdb_colphys* h, bool x_face);
 int mom_field_spec(rdb_colphys* h, const char* name, int* e1, int* e2, int* e3, bool* settable);
 int mom_set_field(rdb_colphys* h, const char* name, const double* data, size_t n);
 int mom_get_field(rdb_colphys* h, const char* name, double* out, size_t n);
+bool handle_valid(rdb_colphys* h);
 // shared by the C ABI translation units
 int launch_step(rdb_colphys* h, double dt, int p0, int p1);
 int refresh_static(rdb_colphys* h, double dt);
@@ -86,6 +88,14 @@ struct rdb_colphys {
   // device, created on first use
   colphys::MomState* mom = nullptr;
   long long h_gen = 0;  // bumped whenever h_layer is set (natural-order copies follow it)
+  // the rdb_ocean.h host surface: simulation time, and host mirrors of the
+  // fields the pointer getters hand out, each current as of state version
+  // mirror_ver[name]; every entry point that changes the device state bumps
+  // state_ver
+  double t_sim = 0.0;
+  long long state_ver = 0;
+  std::map<std::string, std::vector<double>> mirrors;
+  std::map<std::string, long long> mirror_ver;
   long long last_fallback = 0;
   double last_mlp_ms = 0.0;
   double mlp_flop = 0.0;  // per step, unpadded layer widths

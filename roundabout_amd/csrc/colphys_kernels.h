@@ -33,6 +33,11 @@ struct StateView {
   int* ktop;        // [P] 0-based first-live scan start
   int* kb;          // [P] first live layer from the bed (nz: none)
   int* fdirty;      // [P] lowest bed-filler interface holding a KPP value
+  // vdiff budget slots heat/salt_budget_vdiff (null: not kept): the content
+  // change h*c_new - h*c_old of the tracer solve, accumulated per cell
+  double* budT;     // [nz][P]
+  double* budS;     // [nz][P]
+  double* pold;     // [2][nz][P] h*c_old of T and S between the two sweeps
 };
 
 __global__ void k_pack_layer(const double* src, double* dst, const int* col, int ncol, int P, int nlev,

@@ -291,9 +291,14 @@ __device__ void factorize_column(const StateView& s, const double* __restrict__ 
   }
 }
 
-// A13 forward sweep + back substitution of one tracer against stored pivots.
-__device__ void apply_column(const StateView& s, double* __restrict__ x, int p) {
+// A13 forward sweep + back substitution of one tracer against stored pivots;
+// with a budget slot, budget += h*c_new - h*c_old per cell
+// (apply_factored_tracer, rdb_ocean_vdiff.F90:1351-1414).
+__device__ void apply_column(const StateView& s, double* __restrict__ x, int p, double* budget = nullptr,
+                             double* pold = nullptr) {
   const int P = s.P, nz = s.nz;
+  if (budget)
+    for (int k = 0; k < nz; ++k) LV(pold, k) = LV(s.h, k) * LV(x, k);
   double r = LV(x, 0) / LV(s.fb, 0);
   LV(x, 0) = r;
   for (int k = 1; k < nz; ++k) {
@@ -304,6 +309,8 @@ __device__ void apply_column(const StateView& s, double* __restrict__ x, int p) 
     r = LV(x, k) - LV(s.fc, k) * r;
     LV(x, k) = r;
   }
+  if (budget)
+    for (int k = 0; k < nz; ++k) LV(budget, k) = LV(budget, k) + (LV(s.h, k) * LV(x, k) - LV(pold, k));
 }
 
 // A11 two-pass dispatch: kt -> T; ks -> S and passives (rdb_ocean_vdiff.F90:1107-1226).
@@ -311,9 +318,9 @@ __global__ void k_vdiff(StateView s, Params pr, double dt, int p0, int p1) {
   int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= p1) return;
   factorize_column(s, s.kt, dt, pr.use_harmonic, p);
-  apply_column(s, s.T, p);
+  apply_column(s, s.T, p, s.budT, s.pold);
   factorize_column(s, s.ks, dt, pr.use_harmonic, p);
-  apply_column(s, s.S, p);
+  apply_column(s, s.S, p, s.budS, s.pold);
   for (int t = 0; t < s.n_passive; ++t) apply_column(s, s.tr + (size_t)t * (size_t)s.nz * (size_t)s.P, p);
 }
 
@@ -730,6 +737,10 @@ __device__ __forceinline__ void column_step(const StateView& s, const Params& pr
       if (pr.has_heat) xT = conc_add_source(xT, h0, cellT);
       if (pr.has_salt) xS = conc_add_source(xS, h0, cellS);
     }
+    if (s.budT) {  // h*c_old of the tracer solve (after the surface deposit)
+      LV(s.pold, k) = h0 * xT;
+      LV(s.pold + tstride, k) = h0 * xS;
+    }
     if (k == kb) {
       rT = xT / piv_t;
       rS = xS / piv_s;
@@ -840,6 +851,11 @@ __device__ __forceinline__ void column_step(const StateView& s, const Params& pr
     }
     xT_up = xT;
     xS_up = xS;
+    if (s.budT) {  // budget += h*c_new - h*c_old, before the non-local tendency
+      const double hb = LV(s.h, k);
+      LV(s.budT, k) = LV(s.budT, k) + (hb * xT - LV(s.pold, k));
+      LV(s.budS, k) = LV(s.budS, k) + (hb * xS - LV(s.pold + tstride, k));
+    }
     if (nl_on && k <= k_sfc && k >= nl_lo) {
       // gamma at interface k (bottom of layer k), KPP pass 2 arithmetic
       double hk = LV(s.h, k);

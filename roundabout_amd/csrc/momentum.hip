@@ -572,6 +572,7 @@ int mom_solve(rdb_colphys* h, double dt, const rdb_colphys_momentum_config* c) {
     m->u_host_stale = m->v_host_stale = true;
     h->static_dirty = true;  // land-column outputs depend on u and v
   }
+  ++h->state_ver;
   return 0;
 }
 

@@ -541,6 +541,8 @@ int surr_collect(rdb_colphys* h, double dt, int n_steps, int stride, long long* 
     SG_TRY(hipGetLastError());
     S->n += ncols;
     h->steps += 1;
+    h->t_sim += dt;
+    ++h->state_ver;
   }
   SG_TRY(hipStreamSynchronize(h->stream));
   if (n_out) *n_out = S->n;

@@ -254,10 +254,39 @@ PROTOTYPES = [
       ctypes.c_double, ctypes.c_double, _fp]),
     ("rdb_colphys_cbrt", ctypes.c_int, [_dp, _dp, ctypes.c_int]),
     ("rdb_colphys_momentum_config_default", ctypes.c_int, [ctypes.POINTER(MomentumConfig)]),
+    ("rdb_colphys_momentum_config_from_string", ctypes.c_int,
+     [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(MomentumConfig)]),
+    ("rdb_colphys_config_from_string", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(Config)]),
+    ("rdb_colphys_create_from_string", ctypes.c_int,
+     [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _ip, _ip, _hp]),
+    ("rdb_colphys_get_time", ctypes.c_int, [_vp, _dp]),
+    ("rdb_colphys_get_step_count", ctypes.c_int, [_vp, _ip]),
+    ("rdb_colphys_get_grid_info", ctypes.c_int, [_vp, _ip, _ip, _ip, _ip]),
+    ("rdb_colphys_refresh_host", ctypes.c_int, [_vp]),
+    ("rdb_colphys_get_tracer_count", ctypes.c_int, [_vp, _ip]),
+    ("rdb_colphys_list_tracers", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
+    ("rdb_colphys_get_tracer_ptr", ctypes.c_int,
+     [_vp, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), _ip, _ip, _ip, _ip]),
+    ("rdb_colphys_set_h", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("rdb_colphys_set_u", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("rdb_colphys_set_v", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("rdb_colphys_set_wind", ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, ctypes.c_int]),
+    ("rdb_colphys_set_heat_flux", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int]),
+    ("rdb_colphys_set_salt_flux", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int]),
+    ("rdb_colphys_set_tracer", ctypes.c_int,
+     [_vp, ctypes.c_char_p, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("rdb_colphys_set_budgets", ctypes.c_int, [_vp, ctypes.c_int]),
     ("rdb_colphys_vdiff_momentum", ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(MomentumConfig)]),
     ("rdb_mlp_eval_noalloc", ctypes.c_int,
      [_ip, ctypes.c_int, _fp, ctypes.c_int, _fp, _fp]),
 ]
+# pointer getters into host mirrors: (handle, void **ptr, extents..., gen)
+for _name in ("h_layer", "u_face_x_layer", "v_face_y_layer", "rho_layer", "kv", "kt", "ks"):
+    PROTOTYPES.append((f"rdb_colphys_get_{_name}_ptr", ctypes.c_int,
+                       [_vp, ctypes.POINTER(ctypes.c_void_p), _ip, _ip, _ip, _ip]))
+for _name in ("wet_t", "tau_x", "tau_y", "q_heat", "q_salt"):
+    PROTOTYPES.append((f"rdb_colphys_get_{_name}_ptr", ctypes.c_int,
+                       [_vp, ctypes.POINTER(ctypes.c_void_p), _ip, _ip, _ip]))
 
 
 def _candidates():

@@ -18,21 +18,58 @@ from . import _ffi
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
-LAYER_FIELDS = ("h_layer", "temperature", "salinity", "rho_layer")
+LAYER_FIELDS = ("h_layer", "temperature", "salinity", "rho_layer", "heat_budget_vdiff", "salt_budget_vdiff")
 IFACE_FIELDS = ("kv", "kt", "ks", "gamma_t", "gamma_s")
-COLUMN_FIELDS = ("stress_shelf", "Q_heat", "Q_salt", "bl_depth", "stress_mag", "b0", "kbl", "coriolis")
+COLUMN_FIELDS = ("stress_shelf", "Q_heat", "Q_salt", "bl_depth", "stress_mag", "b0", "kbl", "coriolis", "wet_t")
 # face-located inputs of the momentum solve's folds (rdb_colphys_vdiff_momentum)
 U_FACE_2D = ("lambda_bot_u", "lambda_top_u", "cover_u", "kv_bbl_u", "bbl_thick_u", "k_top_u", "k_bot_u")
 V_FACE_2D = ("lambda_bot_v", "lambda_top_v", "cover_v", "kv_bbl_v", "bbl_thick_v", "k_top_v", "k_bot_v")
+
+
+class HostField:
+    """A window onto one host mirror of the library (rdb_colphys_get_*_ptr):
+    it remembers the outer step count its contents were last known good at
+    (the generation) and, on access, calls rdb_colphys_refresh_host when the
+    handle has stepped since -- the lazy, silent re-sync of the reference's
+    python/rdb Field (python/rdb/_field.py).  Entry points that change the
+    state without a step (setters, the momentum solve) keep the step count:
+    call ``refresh_host`` after them, as with the reference."""
+
+    def __init__(self, cp, name):
+        self._cp = cp
+        self.name = name
+        self._arr, self._gen = cp.host_pointer(name)
+
+    @property
+    def generation(self) -> int:
+        return self._gen
+
+    @property
+    def values(self) -> np.ndarray:
+        now = self._cp.step_count
+        if now != self._gen:
+            self._cp.refresh_host()
+            self._gen = now
+        return self._arr
 
 
 class ColumnPhysics:
     """One column-physics plan on the GPU (``rdb_colphys_create``)."""
 
     def __init__(self, nx: int, ny: int, nz: int, wet, *, n_passive: int = 0, config=None,
-                 closures=(), k_top=None, sort_key=None, lib=None):
+                 closures=(), k_top=None, sort_key=None, lib=None, namelist=None):
+        """``namelist``: Fortran namelist text (&ocean_vmix_nml, &ocean_eos_nml,
+        &ocean_ic_nml) to create from instead of ``config``, as
+        rdb_ocean_create_from_string does (rdb_colphys_create_from_string)."""
         self._lib = lib or _ffi.load()
-        cfg = config if config is not None else _ffi.default_config()
+        if namelist is not None:
+            if config is not None or closures:
+                raise ValueError("give either namelist text or config/closures")
+            text = namelist.encode() if isinstance(namelist, str) else bytes(namelist)
+            cfg = _ffi.Config()
+            _ffi.check(self._lib.rdb_colphys_config_from_string(text, len(text), ctypes.byref(cfg)), self._lib)
+        else:
+            cfg = config if config is not None else _ffi.default_config()
         for c in closures:
             c.apply(cfg)
         self.config = cfg
@@ -54,10 +91,12 @@ class ColumnPhysics:
 
     # ------------------------------------------------------------- factory
     @classmethod
-    def from_ocean(cls, ocean, config=None, closures=(), sort: bool = True, lib=None) -> "ColumnPhysics":
+    def from_ocean(cls, ocean, config=None, closures=(), sort: bool = True, lib=None,
+                   namelist=None) -> "ColumnPhysics":
         """Create a plan for a ``roundabout_amd.synth.Ocean`` and upload it."""
         cp = cls(ocean.nx, ocean.ny, ocean.nz, ocean.wet, n_passive=ocean.n_passive, config=config,
-                 closures=closures, k_top=ocean.k_top, sort_key=ocean.n_live if sort else None, lib=lib)
+                 closures=closures, k_top=ocean.k_top, sort_key=ocean.n_live if sort else None, lib=lib,
+                 namelist=namelist)
         cp.upload(ocean)
         return cp
 
@@ -142,6 +181,76 @@ class ColumnPhysics:
         if name == "kbl":
             out = out.astype(np.int32)
         return out
+
+    # -------------------------------------------- rdb_ocean.h host surface
+    _PTR3 = ("h_layer", "u_face_x_layer", "v_face_y_layer", "rho_layer", "kv", "kt", "ks")
+    _PTR2 = {"wet_t": "wet_t", "tau_x": "tau_x", "tau_y": "tau_y", "Q_heat": "q_heat", "Q_salt": "q_salt"}
+
+    def host_pointer(self, name: str):
+        """(read-only numpy view of the host mirror, generation) of a field,
+        through the rdb_colphys_get_*_ptr getters; tracers by registry name."""
+        p = ctypes.c_void_p()
+        n = [ctypes.c_int() for _ in range(4)]
+        h = self._handle()
+        if name in self._PTR3:
+            st = getattr(self._lib, f"rdb_colphys_get_{name}_ptr")(h, ctypes.byref(p), *[ctypes.byref(x) for x in n])
+            shape = (n[0].value, n[1].value, n[2].value)
+            gen = n[3].value
+        elif name in self._PTR2:
+            st = getattr(self._lib, f"rdb_colphys_get_{self._PTR2[name]}_ptr")(
+                h, ctypes.byref(p), ctypes.byref(n[0]), ctypes.byref(n[1]), ctypes.byref(n[3]))
+            shape = (n[0].value, n[1].value)
+            gen = n[3].value
+        else:
+            b = name.encode()
+            st = self._lib.rdb_colphys_get_tracer_ptr(h, b, len(b), ctypes.byref(p), *[ctypes.byref(x) for x in n])
+            shape = (n[0].value, n[1].value, n[2].value)
+            gen = n[3].value
+        _ffi.check(st, self._lib)
+        count = int(np.prod(shape))
+        arr = np.ndarray(shape, dtype=np.float64, buffer=(ctypes.c_double * count).from_address(p.value),
+                         order="F")
+        arr.flags.writeable = False
+        return arr, gen
+
+    def field(self, name: str) -> "HostField":
+        """A lazily synchronised view of a field (see HostField)."""
+        return HostField(self, name)
+
+    def refresh_host(self) -> None:
+        _ffi.check(self._lib.rdb_colphys_refresh_host(self._handle()), self._lib)
+
+    @property
+    def time(self) -> float:
+        t = ctypes.c_double()
+        _ffi.check(self._lib.rdb_colphys_get_time(self._handle(), ctypes.byref(t)), self._lib)
+        return t.value
+
+    @property
+    def step_count(self) -> int:
+        n = ctypes.c_int()
+        _ffi.check(self._lib.rdb_colphys_get_step_count(self._handle(), ctypes.byref(n)), self._lib)
+        return n.value
+
+    @property
+    def grid_info(self) -> tuple[int, int, int, int]:
+        n = [ctypes.c_int() for _ in range(4)]
+        _ffi.check(self._lib.rdb_colphys_get_grid_info(self._handle(), *[ctypes.byref(x) for x in n]), self._lib)
+        return tuple(x.value for x in n)
+
+    def tracer_names(self) -> list[str]:
+        cnt = ctypes.c_int()
+        _ffi.check(self._lib.rdb_colphys_get_tracer_count(self._handle(), ctypes.byref(cnt)), self._lib)
+        out = []
+        for i in range(cnt.value):
+            buf = ctypes.create_string_buffer(64)
+            self._lib.rdb_colphys_list_tracers(self._handle(), i, buf, 64)
+            out.append(buf.value.decode())
+        return out
+
+    def set_budgets(self, enable: bool) -> None:
+        """Keep heat/salt_budget_vdiff (fields of the same names)."""
+        _ffi.check(self._lib.rdb_colphys_set_budgets(self._handle(), int(bool(enable))), self._lib)
 
     # --------------------------------------------------------------- compute
     def eos(self) -> None:
