@@ -25,6 +25,9 @@ program colphys_fortran_check
    type(c_ptr) :: ewet(2)
    type(rdb_colphys_momentum_config_t) :: mcfg
    real(c_double) :: u1(nx + 1, ny, nz), hf(nz), m0, m1, mabs
+   real(c_double) :: hb(nx, ny, nz), tnow
+   integer(c_int) :: nsteps
+   character(len=*), parameter :: nml = "&ocean_vmix_nml use_kpp = .true., pp81_alpha = 5.0 /"
 
    call get_command_argument(1, mode)
    st = rdb_colphys_config_default(cfg)
@@ -78,14 +81,26 @@ program colphys_fortran_check
    end if
 
    ! gpu: fused step vs the staged entry points on two identical handles
-   st = rdb_colphys_create(cfg, nx, ny, nz, 0, wet, c_null_ptr, c_null_ptr, h)
+   ! (the first one created from namelist text, with the budget slots kept)
+   st = rdb_colphys_create_from_string(nml, len(nml, c_int), nx, ny, nz, 0, wet, c_null_ptr, c_null_ptr, h)
    if (st /= 0) then
       print '(a)', trim(colphys_last_error())
       stop 10
    end if
    call upload()
+   if (rdb_colphys_set_budgets(h, 1_c_int) /= 0) stop 18
    st = rdb_colphys_step(h, 3600.0_c_double, 3)
    if (st /= 0) stop 11
+   if (rdb_colphys_get_step_count(h, nsteps) /= 0 .or. nsteps /= 3) stop 19
+   if (rdb_colphys_get_time(h, tnow) /= 0 .or. tnow /= 3.0_c_double*3600.0_c_double) stop 19
+   ! the vdiff heat budget telescopes in every column
+   if (colphys_get(h, "heat_budget_vdiff", hb) /= 0) stop 40
+   do j = 1, ny
+      do i = 1, nx
+         if (abs(sum(hb(i, j, :))) > 1.0e-12_c_double*sum(abs(hl(i, j, :)*T(i, j, :)))) stop 41
+      end do
+   end do
+   if (maxval(abs(hb)) <= 0.0_c_double) stop 42
    st = colphys_get(h, "temperature", T1)
    st = colphys_get(h, "kt", kt1)
    st = rdb_colphys_heat_salt_totals(h, heat, salt)

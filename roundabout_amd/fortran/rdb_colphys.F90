@@ -68,6 +68,8 @@ module rdb_colphys
    end type rdb_colphys_momentum_config_t
 
    public :: rdb_colphys_momentum_config_default, rdb_colphys_vdiff_momentum
+   public :: rdb_colphys_create_from_string, rdb_colphys_set_budgets, rdb_colphys_refresh_host
+   public :: rdb_colphys_get_step_count, rdb_colphys_get_time
    public :: rdb_colphys_ensemble_create, rdb_colphys_member_count
    public :: rdb_colphys_member_set_field, rdb_colphys_member_get_field
    public :: colphys_member_set, colphys_member_get
@@ -257,6 +259,37 @@ module rdb_colphys
          type(c_ptr), value :: handle
          real(c_double), value :: dt
          type(rdb_colphys_momentum_config_t), intent(in) :: cfg
+      end function
+      !> rdb_ocean_create_from_string analogue: the config from namelist text.
+      integer(c_int) function rdb_colphys_create_from_string(nml_text, nml_len, nx, ny, nz, n_passive, wet, &
+                                                             k_top, sort_key, handle) &
+         bind(c, name="rdb_colphys_create_from_string")
+         import :: c_int, c_double, c_ptr, c_char
+         character(kind=c_char), intent(in) :: nml_text(*)
+         integer(c_int), value :: nml_len, nx, ny, nz, n_passive
+         real(c_double), intent(in) :: wet(*)
+         type(c_ptr), value :: k_top, sort_key
+         type(c_ptr), intent(out) :: handle
+      end function
+      !> Keep heat/salt_budget_vdiff (read with colphys_get).
+      integer(c_int) function rdb_colphys_set_budgets(handle, enable) bind(c, name="rdb_colphys_set_budgets")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int), value :: enable
+      end function
+      integer(c_int) function rdb_colphys_refresh_host(handle) bind(c, name="rdb_colphys_refresh_host")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function
+      integer(c_int) function rdb_colphys_get_step_count(handle, n) bind(c, name="rdb_colphys_get_step_count")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+         integer(c_int), intent(out) :: n
+      end function
+      integer(c_int) function rdb_colphys_get_time(handle, t) bind(c, name="rdb_colphys_get_time")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: handle
+         real(c_double), intent(out) :: t
       end function
       integer(c_int) function rdb_colphys_last_error(idx, buf, cap) bind(c, name="rdb_colphys_last_error")
          import :: c_int, c_char
