@@ -364,6 +364,7 @@ void drop_graph(rdb_colphys* h) {
   h->graph_exec = nullptr;
   h->graph = nullptr;
   h->graph_n = 0;
+  colphys::surr_invalidate(h);
 }
 
 int ensure_events(rdb_colphys* h, int n) {
@@ -1024,26 +1025,10 @@ int rdb_colphys_step(rdb_colphys* h, double dt, int n_steps) {
   int st = refresh_static(h, dt);
   if (st) return st;
   if (h->closure == RDB_COLPHYS_CLOSURE_SURROGATE) {
-    // features -> MLP (MFMA GEMMs) -> fused step with the predicted
-    // coefficients, eagerly launched; events bracket the whole sequence
-    if (h->timing) {
-      st = ensure_events(h, 2);
-      if (st) return st;
-      HIP_TRY(hipEventRecord(h->events[0], h->stream));
-    }
-    for (int i = 0; i < n_steps; ++i) {
-      st = colphys::surr_step(h, dt);
-      if (st) return st;
-    }
-    if (h->timing) {
-      HIP_TRY(hipEventRecord(h->events[1], h->stream));
-      HIP_TRY(hipEventSynchronize(h->events[1]));
-      float ms = 0.f;
-      HIP_TRY(hipEventElapsedTime(&ms, h->events[0], h->events[1]));
-      h->last_ms = ms / n_steps;
-      h->sum_ms += ms;
-      h->timed += n_steps;
-    }
+    // features -> fused MLP forward -> column step with the predicted
+    // coefficients, n steps captured in one hipGraph and replayed
+    st = colphys::surr_run(h, dt, n_steps);
+    if (st) return st;
   } else if (h->n_passive > 4) {
     // beyond the fused kernel's tracer slots: the staged sequence, per step
     for (int i = 0; i < n_steps; ++i) {
@@ -1464,4 +1449,14 @@ int rdb_colphys_last_error(int idx, char* buf, int cap) {
 
 namespace colphys {
 bool handle_valid(rdb_colphys* h) { return valid(h); }
+}  // namespace colphys
+
+namespace colphys {
+int step_events(rdb_colphys* h, hipEvent_t* e0, hipEvent_t* e1) {
+  int st = ensure_events(h, 2);
+  if (st) return st;
+  *e0 = h->events[0];
+  *e1 = h->events[1];
+  return RDB_COLPHYS_OK;
+}
 }  // namespace colphys

@@ -25,7 +25,12 @@ bool handle_valid(rdb_colphys* h);
 // shared by the C ABI translation units
 int launch_step(rdb_colphys* h, double dt, int p0, int p1);
 int refresh_static(rdb_colphys* h, double dt);
-int surr_step(rdb_colphys* h, double dt);
+// n surrogate steps, captured in a hipGraph and replayed (surrogate.hip)
+int surr_run(rdb_colphys* h, double dt, int n);
+// a launch argument of the captured surrogate step changed (the state view)
+void surr_invalidate(rdb_colphys* h);
+// the handle's step event pair (created on first use)
+int step_events(rdb_colphys* h, hipEvent_t* e0, hipEvent_t* e1);
 // the last surrogate step's fallback count and forward time, read from the
 // device (synchronises the stream when a step is pending)
 int surr_settle(rdb_colphys* h);

@@ -35,6 +35,9 @@ bool fused_shape_ok(const LayerView& v, int Mpad);
 // out_major, output-major (pout[nl-1] x Mpad); launches on s, allocates
 // nothing.
 hipError_t forward_view(const LayerView& v, const bf16* X0, float* Y, int Mpad, bool out_major, hipStream_t s);
+// What forward_view sets up on first use on a device (the fused kernel's
+// dynamic LDS allowance); call before capturing a forward into a graph.
+hipError_t prepare_forward();
 
 struct Net {
   int nl = 0;                      // linear layers

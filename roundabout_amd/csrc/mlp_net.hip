@@ -269,20 +269,27 @@ bool fused_shape_ok(const LayerView& v, int Mpad) {
   return true;
 }
 
+hipError_t prepare_forward() {
+  // the fused kernel's dynamic LDS, allowed once per device (0..63)
+  static unsigned long long lds_set = 0;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev >= 64 || !(lds_set >> dev & 1ull)) {
+    e = hipFuncSetAttribute((const void*)k_mlp_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)FP_LDS_BYTES);
+    if (e != hipSuccess) return e;
+    if (dev < 64) lds_set |= 1ull << dev;
+  }
+  return hipSuccess;
+}
+
 hipError_t forward_view(const LayerView& v, const bf16* X0, float* Y, int Mpad, bool out_major, hipStream_t s) {
   if (v.nl < 1 || v.nl > LV_MAXL) return hipErrorInvalidValue;
   if (fused_shape_ok(v, Mpad)) {
     // inference: one launch over all layers, activations in LDS
-    static unsigned long long lds_set = 0;  // per device (devices 0..63)
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
+    hipError_t e = prepare_forward();
     if (e != hipSuccess) return e;
     const size_t shmem = FP_LDS_BYTES;
-    if (dev >= 64 || !(lds_set >> dev & 1ull)) {
-      e = hipFuncSetAttribute((const void*)k_mlp_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-      if (e != hipSuccess) return e;
-      if (dev < 64) lds_set |= 1ull << dev;
-    }
     FusedNet fn{};
     fn.nl = v.nl;
     for (int l = 0; l < v.nl; ++l) {

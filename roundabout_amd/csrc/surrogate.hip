@@ -76,6 +76,23 @@ struct SurrState {
   bool pending = false;
   bool timed = false;
   std::vector<SurrView> svs_up;  // the member views last uploaded (ensembles)
+  // ensembles: the feature kernel's member table and work list (one launch)
+  struct FeatMemberHost {
+    const double* cor;
+    long long row_off;
+    int n, lev_off;
+  };
+  void* d_fmem = nullptr;
+  int2* d_fwork = nullptr;
+  int n_fwork = 0;
+  // the captured surrogate step: rebuilt when dt, the step count, the launch
+  // shape or any buffer its launches name (prep_gen) changes
+  long long prep_gen = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  double g_dt = 0.0;
+  int g_n = 0, g_block = 0, g_lds = -1;
+  long long g_prep = -1;
   ~SurrState();
   void free_dataset();
 };
@@ -332,10 +349,12 @@ __global__ void k_norm_targets(const float* T, const float* W, long long n, int 
 // in LDS.  Every value is the
 // same arithmetic as feature(), so training and inference features agree
 // bit for bit.
-__global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double* cor, int n, int din, int ld,
-                                                     const float* st, bf16* X0, unsigned char* guard,
-                                                     unsigned int* nfall, int guard_mode, double rho0, double p_ref,
-                                                     int nzn, int lev_off, double b0T, double b0S) {
+// The block of FC packed columns [p0, p0 + FC) of one grid (single grids:
+// the launch's block index times FC; ensembles: one work item of a member).
+__device__ __forceinline__ void features_block(int p0, const StateView& s, const double* cor, int n, int din,
+                                               int ld, const float* st, bf16* X0, unsigned char* guard,
+                                               unsigned int* nfall, int guard_mode, double rho0, double p_ref,
+                                               int nzn, int lev_off, double b0T, double b0S) {
   __shared__ bf16 tile[384][FC + 1];
   __shared__ unsigned char flags[FW][FC];
   __shared__ double sdepth[FW][FC];
@@ -351,7 +370,7 @@ __global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double
   const float* stdv = sst + din;
   const float* mn = sst + 2 * din;
   const float* mx = sst + 3 * din;
-  const int p = blockIdx.x * FC + c;
+  const int p = p0 + c;
   const int P = s.P, nz = s.nz;
   unsigned char fl = 0;
   auto put = [&](int f, float v) {
@@ -429,13 +448,49 @@ __global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double
   __syncthreads();  // the forcing features are in the tile
   // cooperative row stores: the FC rows of this block, 2 bf16 per lane,
   // consecutive lanes along a row
-  const int rows = min(FC, n - (int)blockIdx.x * FC);
+  const int rows = min(FC, n - p0);
   const int half = ld / 2;
   for (int idx = tid; idx < rows * half; idx += FB) {
     const int r = idx / half, c2 = idx - r * half;
     const uint32_t lo = tile[2 * c2][r], hi = tile[2 * c2 + 1][r];
-    ((uint32_t*)(X0 + (size_t)(blockIdx.x * FC + r) * (size_t)ld))[c2] = lo | (hi << 16);
+    ((uint32_t*)(X0 + (size_t)(p0 + r) * (size_t)ld))[c2] = lo | (hi << 16);
   }
+}
+
+__global__ void __launch_bounds__(FB) k_features_infer(StateView s, const double* cor, int n, int din, int ld,
+                                                     const float* st, bf16* X0, unsigned char* guard,
+                                                     unsigned int* nfall, int guard_mode, double rho0, double p_ref,
+                                                     int nzn, int lev_off, double b0T, double b0S) {
+  features_block(blockIdx.x * FC, s, cor, n, din, ld, st, X0, guard, nfall, guard_mode, rho0, p_ref, nzn, lev_off,
+                 b0T, b0S);
+}
+
+// Every member of an ensemble in one launch: block w takes the FC columns
+// of work item (member, first packed column) work[w].
+struct FeatMember {
+  const double* cor;  // the member's Coriolis field (packed)
+  long long row_off;  // the member's first row in the network batch
+  int n;              // wet columns
+  int lev_off;        // network levels below the member's bed
+};
+
+__global__ void __launch_bounds__(FB) k_features_infer_ens(const StateView* __restrict__ views,
+                                                         const FeatMember* __restrict__ mem,
+                                                         const int2* __restrict__ work, int din, int ld,
+                                                         const float* st, bf16* X0, unsigned char* guard,
+                                                         unsigned int* nfall, int guard_mode, double rho0,
+                                                         double p_ref, int nzn, double b0T, double b0S) {
+  const int2 w = work[blockIdx.x];
+  const FeatMember m = mem[w.x];
+  features_block(w.y, views[w.x], m.cor, m.n, din, ld, st, X0 + (size_t)m.row_off * (size_t)ld, guard + m.row_off,
+                 nfall, guard_mode, rho0, p_ref, nzn, m.lev_off, b0T, b0S);
+}
+
+// the fallback counter is cleared by a kernel, so that the step can be
+// captured and replayed (no memset node in the graph)
+__global__ void k_clear_u32(unsigned int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
 }
 
 inline dim3 gs(long long n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
@@ -446,6 +501,10 @@ inline dim3 gs_capped(long long n, int b) { return dim3((unsigned)std::min<long 
 
 SurrState::~SurrState() {
   free_dataset();
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (d_fmem) (void)hipFree(d_fmem);
+  if (d_fwork) (void)hipFree(d_fwork);
   if (d_stats) (void)hipFree(d_stats);
   if (guard) (void)hipFree(guard);
   if (nfall) (void)hipFree(nfall);
@@ -728,7 +787,11 @@ int surr_train(rdb_colphys* h, const rdb_colphys_mlp_config* cfg, float* train_l
   return RDB_COLPHYS_OK;
 }
 
-int surr_step(rdb_colphys* h, double dt) {
+// Everything a surrogate step needs that is not a kernel launch: checks,
+// the inference workspace, the guard and fallback buffers, the member views
+// and the ensemble feature tables.  Run before a step is launched or
+// captured; a change of any buffer the launches name bumps prep_gen.
+int surr_prepare(rdb_colphys* h) {
   SurrState* S = h->surr;
   if (!S || !S->have_net || !S->d_stats) return perr(RDB_COLPHYS_ERR_NOT_INITIALISED, "surrogate: no trained network");
   mlp::Net& net = S->net;
@@ -749,42 +812,84 @@ int surr_step(rdb_colphys* h, double dt) {
     SG_TRY(hipMalloc((void**)&S->guard, (size_t)Mpad));
     if (!S->nfall) SG_TRY(hipMalloc((void**)&S->nfall, 64));
     S->inf_rows = Mpad;
+    ++S->prep_gen;
   }
   if (net.Mcap < Mpad || !S->guard || !S->nfall)
     return perr(RDB_COLPHYS_ERR_SETUP, "surrogate: inference workspace of %d rows for %d wet rows", net.Mcap, Mpad);
-  const std::vector<rdb_colphys*> ps = plans(h);
-  SG_TRY(hipMemsetAsync(S->nfall, 0, 4, h->stream));
+  SG_TRY(mlp::prepare_forward());
+  if (!h->members.empty()) {
+    const std::vector<rdb_colphys*> ps = plans(h);
+    const float* tmean = S->d_stats + 4 * S->din;
+    const float* tstd = tmean + S->dout;
+    // the member views change only with the network's workspace: upload
+    // them (synchronously) when they do, never from a buffer a later step
+    // could overwrite while the copy is pending
+    std::vector<SurrView> svs;
+    for (rdb_colphys* m : ps)
+      svs.push_back(SurrView{net.Y + m->row_off, (long long)Mpad, tmean, tstd, S->guard + m->row_off, nzn - m->nz, 0});
+    if (svs.size() != S->svs_up.size() ||
+        std::memcmp(svs.data(), S->svs_up.data(), sizeof(SurrView) * svs.size()) != 0) {
+      SG_TRY(hipMemcpyAsync(h->d_svs, svs.data(), sizeof(SurrView) * svs.size(), hipMemcpyHostToDevice, h->stream));
+      SG_TRY(hipStreamSynchronize(h->stream));
+      S->svs_up = svs;
+      ++S->prep_gen;
+    }
+    if (!S->d_fwork) {
+      std::vector<SurrState::FeatMemberHost> fm;
+      std::vector<int2> work;
+      for (size_t mi = 0; mi < ps.size(); ++mi) {
+        rdb_colphys* m = ps[mi];
+        fm.push_back({m->d_coriolis, m->row_off, m->n_wet, nzn - m->nz});
+        for (int p0 = 0; p0 < m->n_wet; p0 += FC) work.push_back(make_int2((int)mi, p0));
+      }
+      static_assert(sizeof(SurrState::FeatMemberHost) == sizeof(FeatMember), "feature member table layout");
+      SG_TRY(hipMalloc(&S->d_fmem, sizeof(FeatMember) * std::max<size_t>(fm.size(), 1)));
+      SG_TRY(hipMalloc((void**)&S->d_fwork, sizeof(int2) * std::max<size_t>(work.size(), 1)));
+      SG_TRY(hipMemcpy(S->d_fmem, fm.data(), sizeof(FeatMember) * fm.size(), hipMemcpyHostToDevice));
+      SG_TRY(hipMemcpy(S->d_fwork, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice));
+      S->n_fwork = (int)work.size();
+      ++S->prep_gen;
+    }
+    int st = ensure_work(h);  // the persistent step's work list, outside any capture
+    if (st) return st;
+  }
+  h->mlp_flop = 0.0;
+  for (int l = 0; l < net.nl; ++l) h->mlp_flop += 2.0 * (double)net.din[l] * (double)net.dout[l];
+  h->mlp_flop *= (double)rows;
+  return RDB_COLPHYS_OK;
+}
+
+// One surrogate step as kernel launches only (capturable): clear the
+// fallback counter, features and guard flags of every wet column, the fused
+// network forward, the column step with the predicted coefficients.
+int surr_launch(rdb_colphys* h, double dt) {
+  SurrState* S = h->surr;
+  mlp::Net& net = S->net;
+  const int nzn = net_levels(h);
+  const int Mpad = S->inf_rows;
   const int ld = net.pin[0];
-  for (rdb_colphys* m : ps) {
-    if (m->n_wet == 0) continue;
-    hipLaunchKernelGGL(k_features_infer, gs(m->n_wet, FC), dim3(FB), 0, h->stream, m->s, m->d_coriolis, m->n_wet,
-                       S->din, ld, S->d_stats, net.X0 + (size_t)m->row_off * (size_t)ld, S->guard + m->row_off,
-                       S->nfall, S->guard_mode, h->pr.rho0, h->pr.p_ref, nzn, nzn - m->nz, b0_T(h->pr), b0_S(h->pr));
+  hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, h->stream, S->nfall, 1);
+  if (h->members.empty()) {
+    if (h->n_wet > 0)
+      hipLaunchKernelGGL(k_features_infer, gs(h->n_wet, FC), dim3(FB), 0, h->stream, h->s, h->d_coriolis, h->n_wet,
+                         S->din, ld, S->d_stats, net.X0, S->guard, S->nfall, S->guard_mode, h->pr.rho0, h->pr.p_ref,
+                         nzn, 0, b0_T(h->pr), b0_S(h->pr));
+  } else if (S->n_fwork > 0) {
+    hipLaunchKernelGGL(k_features_infer_ens, dim3((unsigned)S->n_fwork), dim3(FB), 0, h->stream, h->d_views,
+                       (const FeatMember*)S->d_fmem, S->d_fwork, S->din, ld, S->d_stats, net.X0, S->guard, S->nfall,
+                       S->guard_mode, h->pr.rho0, h->pr.p_ref, nzn, b0_T(h->pr), b0_S(h->pr));
   }
   SG_TRY(hipGetLastError());
-  if (h->timing) {
-    if (!S->ev[0]) SG_TRY(hipEventCreate(&S->ev[0]));
-    if (!S->ev[1]) SG_TRY(hipEventCreate(&S->ev[1]));
-    SG_TRY(hipEventRecord(S->ev[0], h->stream));
-  }
-  int e = net.forward(Mpad, true);
-  if (e) return perr(e, "surrogate: %s", net.err.c_str());
-  if (h->timing) SG_TRY(hipEventRecord(S->ev[1], h->stream));
-  double flop = 0.0;
-  for (int l = 0; l < net.nl; ++l) flop += 2.0 * (double)net.din[l] * (double)net.dout[l];
-  h->mlp_flop = flop * (double)rows;
+  SG_TRY(mlp::forward_view(net.view(), net.X0, net.Y, Mpad, true, h->stream));
+  if (!h->members.empty()) return launch_step_all(h, dt, true);
   const float* tmean = S->d_stats + 4 * S->din;
   const float* tstd = tmean + S->dout;
-  auto view_of = [&](rdb_colphys* m) {
-    return SurrView{net.Y + m->row_off, (long long)Mpad, tmean, tstd, S->guard + m->row_off, nzn - m->nz, 0};
-  };
-  if (h->members.empty()) {
-    const SurrView sv = view_of(h);
-    const bool two = (h->cfg.kt_bg != h->cfg.ks_bg);
-    const int ntr = h->n_passive;
-    dim3 g((unsigned)((h->n_wet + h->block - 1) / h->block)), b((unsigned)h->block);
-    const int M = h->lds_levels;
-    const size_t shmem = (size_t)h->block * (size_t)M * (size_t)(3 + std::min(ntr, 4) + (two ? 1 : 0)) * 8;
+  const SurrView sv{net.Y, (long long)Mpad, tmean, tstd, S->guard, 0, 0};
+  const bool two = (h->cfg.kt_bg != h->cfg.ks_bg);
+  const int ntr = h->n_passive;
+  dim3 g((unsigned)((h->n_wet + h->block - 1) / h->block)), b((unsigned)h->block);
+  const int M = h->lds_levels;
+  const size_t shmem = (size_t)h->block * (size_t)M * (size_t)(3 + std::min(ntr, 4) + (two ? 1 : 0)) * 8;
 #define LAUNCH(N, TF)                                                                                              \
   hipLaunchKernelGGL((k_column_step<double, N, TF, true>), g, b, shmem, h->stream, h->s, h->pr, dt, 0, h->n_wet, \
                      h->d_guard, M, sv)
@@ -796,33 +901,73 @@ int surr_step(rdb_colphys* h, double dt) {
     case 3: LAUNCH(3, TF); break;   \
     default: LAUNCH(4, TF); break;  \
   }
-    if (h->n_wet > 0) {
-      if (two) {
-        BY_NTR(true)
-      } else {
-        BY_NTR(false)
-      }
+  if (h->n_wet > 0) {
+    if (two) {
+      BY_NTR(true)
+    } else {
+      BY_NTR(false)
     }
+  }
 #undef BY_NTR
 #undef LAUNCH
-    SG_TRY(hipGetLastError());
-  } else {
-    // the member views change only with the network's workspace: upload
-    // them (synchronously) when they do, and never from a buffer a later
-    // step could overwrite while the copy is pending
-    std::vector<SurrView> svs;
-    for (rdb_colphys* m : ps) svs.push_back(view_of(m));
-    if (svs.size() != S->svs_up.size() ||
-        std::memcmp(svs.data(), S->svs_up.data(), sizeof(SurrView) * svs.size()) != 0) {
-      SG_TRY(hipMemcpyAsync(h->d_svs, svs.data(), sizeof(SurrView) * svs.size(), hipMemcpyHostToDevice, h->stream));
-      SG_TRY(hipStreamSynchronize(h->stream));
-      S->svs_up = svs;
+  SG_TRY(hipGetLastError());
+  return RDB_COLPHYS_OK;
+}
+
+// n surrogate steps: prepared, captured once into a hipGraph (rebuilt when
+// dt, n, the launch shape or a buffer changes) and replayed.  With timing on
+// the replay is bracketed by events on the handle's stream, and the network
+// forward is timed on its own right after (one more forward over the last
+// step's features: it rewrites the same outputs).  Nothing here waits for
+// the device: the fallback count is read by surr_settle.
+int surr_run(rdb_colphys* h, double dt, int n) {
+  int st = surr_prepare(h);
+  if (st) return st;
+  SurrState* S = h->surr;
+  if (!S->gexec || S->g_dt != dt || S->g_n != n || S->g_prep != S->prep_gen || S->g_block != h->block ||
+      S->g_lds != h->lds_levels) {
+    if (S->gexec) (void)hipGraphExecDestroy(S->gexec);
+    if (S->graph) (void)hipGraphDestroy(S->graph);
+    S->gexec = nullptr;
+    S->graph = nullptr;
+    SG_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < n && !st; ++i) st = surr_launch(h, dt);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (st) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
     }
-    int st = launch_step_all(h, dt, true);
-    if (st) return st;
+    SG_TRY(e);
+    S->graph = g;
+    SG_TRY(hipGraphInstantiate(&S->gexec, g, nullptr, nullptr, 0));
+    S->g_dt = dt;
+    S->g_n = n;
+    S->g_prep = S->prep_gen;
+    S->g_block = h->block;
+    S->g_lds = h->lds_levels;
   }
-  // no synchronisation here: consecutive steps queue back to back, and the
-  // fallback count and the forward time are read by surr_settle
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (h->timing) {
+    st = step_events(h, &e0, &e1);
+    if (st) return st;
+    SG_TRY(hipEventRecord(e0, h->stream));
+  }
+  SG_TRY(hipGraphLaunch(S->gexec, h->stream));
+  if (h->timing) {
+    SG_TRY(hipEventRecord(e1, h->stream));
+    SG_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    SG_TRY(hipEventElapsedTime(&ms, e0, e1));
+    h->last_ms = ms / n;
+    h->sum_ms += ms;
+    h->timed += n;
+    if (!S->ev[0]) SG_TRY(hipEventCreate(&S->ev[0]));
+    if (!S->ev[1]) SG_TRY(hipEventCreate(&S->ev[1]));
+    SG_TRY(hipEventRecord(S->ev[0], h->stream));
+    SG_TRY(mlp::forward_view(S->net.view(), S->net.X0, S->net.Y, S->inf_rows, true, h->stream));
+    SG_TRY(hipEventRecord(S->ev[1], h->stream));
+  }
   S->pending = true;
   S->timed = h->timing;
   return RDB_COLPHYS_OK;
@@ -910,8 +1055,14 @@ int surr_load(rdb_colphys* h, const char* path) {
 int surr_set_guard(rdb_colphys* h, int mode) {
   if (mode < RDB_COLPHYS_GUARD_STRICT || mode > RDB_COLPHYS_GUARD_RANGE)
     return perr(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "set_guard: unknown mode %d", mode);
-  surr_get(h)->guard_mode = mode;
+  SurrState* S = surr_get(h);
+  if (S->guard_mode != mode) ++S->prep_gen;  // a launch argument of the captured step
+  S->guard_mode = mode;
   return RDB_COLPHYS_OK;
+}
+
+void surr_invalidate(rdb_colphys* h) {
+  if (h->surr) ++h->surr->prep_gen;
 }
 
 bool surr_ready(rdb_colphys* h) { return h->surr && h->surr->have_net && h->surr->d_stats; }
