@@ -518,6 +518,66 @@ int rdb_colphys_member_get_field(rdb_colphys *h, int member, const char *name, d
  * (tests/test_safe_math_golden.inc:557-606). */
 int rdb_colphys_cbrt(const double *x, double *y, int n);
 
+/* ---- conservative vertical (ALE) remap, on device arrays --------------- */
+
+/* Reconstruction methods (src/core/rdb_constants.F90:208-218).  Any other
+ * code runs PLM, as remap_column's default branch does
+ * (src/ALE/rdb_remap_column.F90:371-372). */
+#define RDB_REMAP_PCM 1
+#define RDB_REMAP_PLM 2
+#define RDB_REMAP_PPM 3
+#define RDB_REMAP_PPM_H4 4
+#define RDB_REMAP_PQM 5
+
+/* These entry points take DEVICE pointers (a Fortran caller passes its
+ * device-resident arrays, e.g. under host_data use_device) in the natural
+ * layout: (nx, ny, nz), i fastest, k = 1 at the bed.  They enqueue their
+ * kernels on `stream` (a hipStream_t; NULL = the null stream) and return
+ * without waiting, except rdb_remap_scan_preconditions, which returns its
+ * scalars.  The library keeps one grow-only work buffer set per device:
+ * calls on one device must be ordered (one stream, or synchronised between
+ * streams).  The thicknesses must be non-negative with matching column
+ * totals (the reference's precondition, rdb_remap_column.F90:120-166);
+ * rdb_remap_scan_preconditions checks it. */
+
+/* ocean_remap_tracer_field (src/ALE/rdb_ocean_remap.F90:236-319): remap one
+ * tracer concentration from h_old to h_new, concentration form: donor
+ * column over the fillers (h <= 1.5e-4 m), the column remap with `method`,
+ * each filler's excess added back, the target bounded by its contributors'
+ * stencil range, an empty target taking the mean of the source cell at its
+ * bottom interface.  c_out may equal c_in (in place, as the reference).
+ * bnd_extrap / nonunif: &vcoord_nml remap_boundary_extrap /
+ * remap_nonuniform_weights.  budget (optional, accumulated): the per-cell
+ * h_new*c_new - h_old*c_old increment of the heat/salt_budget_remap slots;
+ * clamp_out (optional, overwritten): h_new*(c_bounded - c_raw), the
+ * c_new_out/clamp_out test hooks of the reference. */
+int rdb_remap_tracer_field(int method, int nx, int ny, int nz, const double *h_old, const double *h_new,
+                           const double *c_in, double *c_out, int bnd_extrap, int nonunif, double *budget,
+                           double *clamp_out, void *stream);
+
+/* remap_x_face_velocity (axis 0, u is (nx+1, ny, nz)) and
+ * remap_y_face_velocity (axis 1, v is (nx, ny+1, nz))
+ * (src/ALE/rdb_ocean_remap.F90:967-1129), in place: face thicknesses are the
+ * mean of the two abutting cells (the wall faces take their cell's), or,
+ * with zlevel_faces, their minimum with closed layers dropped; conserve_ke
+ * rescales the baroclinic anomaly (rescale_anomaly_ke, :1131-1180). */
+int rdb_remap_face_velocity(int axis, int nx, int ny, int nz, const double *h_old, const double *h_new, double *u,
+                            int method, int conserve_ke, int zlevel_faces, int bnd_extrap, int nonunif,
+                            void *stream);
+
+/* remap_column (src/ALE/rdb_remap_column.F90:328-375) on ncol independent
+ * columns, arrays (ncol, nz) with the column index fastest.  q_new may
+ * equal q_old. */
+int rdb_remap_columns(int method, int ncol, int nz, const double *dz_old, const double *dz_new, const double *q_old,
+                      double *q_new, int bnd_extrap, int nonunif, void *stream);
+
+/* ocean_remap_scan_preconditions (src/ALE/rdb_ocean_remap.F90:1348-1405):
+ * the number of columns with a negative thickness or a column-total
+ * mismatch above rel_tol, the largest relative mismatch and the most
+ * negative thickness (0 if none).  Synchronises `stream`. */
+int rdb_remap_scan_preconditions(int nx, int ny, int nz, const double *h_old, const double *h_new, double rel_tol,
+                                 int *n_bad, double *worst_rel, double *worst_neg, void *stream);
+
 /* ---- error ring (mirrors rdb_ocean_last_error, rdb_ocean.h:302-314) ---- */
 int rdb_colphys_last_error(int idx, char *buf, int cap);
 

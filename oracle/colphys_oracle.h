@@ -110,6 +110,25 @@ void orc_diffuse_velocity_columns(const orc_mom_opts *o, int nu, int nv, int nz,
                                   const double *kv_corner, const int *k_top_face, const int *k_bot_face,
                                   double *c_diag, double *rhs);
 
+/* remap_oracle.c: the conservative vertical remap (src/ALE/
+ * rdb_remap_column.F90, src/ALE/rdb_ocean_remap.F90).  Column arrays are
+ * 0-based, index 0 at the bed. */
+int orc_remap_preconditions_ok(int nz, const double *dz_old, const double *dz_new, double rel_tol);
+void orc_remap_column(int method, int nz, const double *dz_old, const double *dz_new, const double *q_old,
+                      double *q_new, int bnd_extrap, int nonunif);
+void orc_pqm_end_value_h4(const double *dz, const double *u, double *csys);
+void orc_vl_column_conc(int nz, const double *h, const double *c_in, double *c_out);
+void orc_remap_tracer_column(int method, int nz, const double *h_old, const double *h_new, double *c,
+                             int bnd_extrap, int nonunif, double *c_new_out, double *clamp_out);
+void orc_remap_tracer_field(int method, int nx, int ny, int nz, const double *h_old, const double *h_new,
+                            double *c, int bnd_extrap, int nonunif, double *budget, double *c_new_out,
+                            double *clamp_out);
+void orc_remap_face_velocity(int axis, int nx, int ny, int nz, const double *h_old, const double *h_new,
+                             double *u, int method, int conserve_ke, int zlevel_faces, int bnd_extrap,
+                             int nonunif);
+void orc_remap_scan_preconditions(int nx, int ny, int nz, const double *h_old, const double *h_new, double rel_tol,
+                                  int *n_bad, double *worst_rel, double *worst_neg);
+
 #ifdef __cplusplus
 }
 #endif

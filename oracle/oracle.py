@@ -350,3 +350,104 @@ def vdiff_apply_momentum(h, u, v, wet, dt, *, kv_source=None, K_v_momentum=0.0, 
                                        nx, ny, *[_p(a) for a in args], _pi(ktop), _pi(kbot), _p(cs), _p(ds))
         rems.append(rem if remnant else None)
     return u, v, rems[0], rems[1]
+
+
+# ---- conservative vertical remap (remap_oracle.c) --------------------------
+
+REMAP_PCM, REMAP_PLM, REMAP_PPM, REMAP_PPM_H4, REMAP_PQM = 1, 2, 3, 4, 5
+
+
+def _remap_lib():
+    L = lib()
+    if not getattr(L, "_remap_ready", False):
+        i, d = ctypes.c_int, ctypes.c_double
+        L.orc_remap_preconditions_ok.restype = i
+        L.orc_remap_preconditions_ok.argtypes = [i, _dp, _dp, d]
+        L.orc_remap_column.argtypes = [i, i, _dp, _dp, _dp, _dp, i, i]
+        L.orc_pqm_end_value_h4.argtypes = [_dp, _dp, _dp]
+        L.orc_vl_column_conc.argtypes = [i, _dp, _dp, _dp]
+        L.orc_remap_tracer_column.argtypes = [i, i, _dp, _dp, _dp, i, i, _dp, _dp]
+        L.orc_remap_tracer_field.argtypes = [i, i, i, i, _dp, _dp, _dp, i, i, _dp, _dp, _dp]
+        L.orc_remap_face_velocity.argtypes = [i, i, i, i, _dp, _dp, _dp, i, i, i, i, i]
+        L.orc_remap_scan_preconditions.argtypes = [i, i, i, _dp, _dp, d, _ip, _dp, _dp]
+        L._remap_ready = True
+    return L
+
+
+def _col(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def remap_column(method, dz_old, dz_new, q_old, bnd_extrap=False, nonunif=False):
+    """remap_column (rdb_remap_column.F90:328-375) of one column (index 0 at
+    the bed); returns q_new."""
+    dz_old, dz_new, q_old = _col(dz_old), _col(dz_new), _col(q_old)
+    q_new = np.zeros_like(q_old)
+    _remap_lib().orc_remap_column(int(method), q_old.size, _p(dz_old), _p(dz_new), _p(q_old), _p(q_new),
+                                  int(bnd_extrap), int(nonunif))
+    return q_new
+
+
+def remap_preconditions_ok(dz_old, dz_new, rel_tol):
+    dz_old, dz_new = _col(dz_old), _col(dz_new)
+    return bool(_remap_lib().orc_remap_preconditions_ok(dz_old.size, _p(dz_old), _p(dz_new), float(rel_tol)))
+
+
+def pqm_end_value_h4(dz, u):
+    dz, u = _col(dz), _col(u)
+    csys = np.zeros(4)
+    _remap_lib().orc_pqm_end_value_h4(_p(dz), _p(u), _p(csys))
+    return csys
+
+
+def vl_column_conc(h, c):
+    h, c = _col(h), _col(c)
+    out = np.zeros_like(c)
+    _remap_lib().orc_vl_column_conc(c.size, _p(h), _p(c), _p(out))
+    return out
+
+
+def remap_tracer_column(method, h_old, h_new, c, bnd_extrap=False, nonunif=False):
+    """ocean_remap_tracer_column (rdb_ocean_remap.F90:1425-1457) with the
+    field pass's knobs: returns (c_new, clamp)."""
+    h_old, h_new, c = _col(h_old), _col(h_new), _col(c).copy()
+    cn, cl = np.zeros_like(c), np.zeros_like(c)
+    _remap_lib().orc_remap_tracer_column(int(method), c.size, _p(h_old), _p(h_new), _p(c), int(bnd_extrap),
+                                         int(nonunif), _p(cn), _p(cl))
+    return cn, cl
+
+
+def remap_tracer_field(method, h_old, h_new, c, bnd_extrap=False, nonunif=False, budget=None):
+    """ocean_remap_tracer_field (rdb_ocean_remap.F90:236-319) on Fortran
+    (nx, ny, nz) arrays: returns (c_new, clamp); ``budget`` is accumulated in
+    place."""
+    nx, ny, nz = c.shape
+    h_old, h_new = _f(h_old, (nx, ny, nz)), _f(h_new, (nx, ny, nz))
+    cw = np.array(c, dtype=np.float64, order="F", copy=True)
+    clamp = np.zeros((nx, ny, nz), order="F")
+    if budget is not None:
+        assert budget.flags.f_contiguous and budget.dtype == np.float64
+    _remap_lib().orc_remap_tracer_field(int(method), nx, ny, nz, _p(h_old), _p(h_new), _p(cw), int(bnd_extrap),
+                                        int(nonunif), _p(budget), None, _p(clamp))
+    return cw, clamp
+
+
+def remap_face_velocity(axis, h_old, h_new, u, method, conserve_ke=False, zlevel_faces=False, bnd_extrap=False,
+                        nonunif=False):
+    """remap_x_face_velocity (axis 0) / remap_y_face_velocity (axis 1)
+    (rdb_ocean_remap.F90:967-1129); returns the remapped face field."""
+    nx, ny, nz = h_old.shape
+    shp = (nx + 1, ny, nz) if axis == 0 else (nx, ny + 1, nz)
+    uw = np.array(_f(u, shp), copy=True, order="F")
+    _remap_lib().orc_remap_face_velocity(int(axis), nx, ny, nz, _p(_f(h_old, (nx, ny, nz))),
+                                         _p(_f(h_new, (nx, ny, nz))), _p(uw), int(method), int(conserve_ke),
+                                         int(zlevel_faces), int(bnd_extrap), int(nonunif))
+    return uw
+
+
+def remap_scan_preconditions(h_old, h_new, rel_tol):
+    nx, ny, nz = h_old.shape
+    n_bad, wr, wn = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+    _remap_lib().orc_remap_scan_preconditions(nx, ny, nz, _p(_f(h_old, (nx, ny, nz))), _p(_f(h_new, (nx, ny, nz))),
+                                              float(rel_tol), ctypes.byref(n_bad), ctypes.byref(wr), ctypes.byref(wn))
+    return n_bad.value, wr.value, wn.value

@@ -279,6 +279,15 @@ PROTOTYPES = [
     ("rdb_colphys_vdiff_momentum", ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(MomentumConfig)]),
     ("rdb_mlp_eval_noalloc", ctypes.c_int,
      [_ip, ctypes.c_int, _fp, ctypes.c_int, _fp, _fp]),
+    # conservative vertical remap on device arrays (pointers as void *)
+    ("rdb_remap_tracer_field", ctypes.c_int,
+     [ctypes.c_int] * 4 + [_vp] * 4 + [ctypes.c_int] * 2 + [_vp] * 3),
+    ("rdb_remap_face_velocity", ctypes.c_int,
+     [ctypes.c_int] * 4 + [_vp] * 3 + [ctypes.c_int] * 5 + [_vp]),
+    ("rdb_remap_columns", ctypes.c_int,
+     [ctypes.c_int] * 3 + [_vp] * 4 + [ctypes.c_int] * 2 + [_vp]),
+    ("rdb_remap_scan_preconditions", ctypes.c_int,
+     [ctypes.c_int] * 3 + [_vp] * 2 + [ctypes.c_double, _ip, _dp, _dp, _vp]),
 ]
 # pointer getters into host mirrors: (handle, void **ptr, extents..., gen)
 for _name in ("h_layer", "u_face_x_layer", "v_face_y_layer", "rho_layer", "kv", "kt", "ks"):
