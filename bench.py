@@ -43,13 +43,14 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def read_traffic(workload: str):
+def read_traffic(workload: str, path: Path | None = None):
     """HBM bytes per step launch measured by rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, each scaled by the copy-kernel calibration), if recorded."""
-    if not TRAFFIC_FILE.exists():
+    path = path or TRAFFIC_FILE
+    if not path.exists():
         return None
     vals = {}
-    for line in TRAFFIC_FILE.read_text().splitlines():
+    for line in path.read_text().splitlines():
         if ":" in line and not line.startswith("#"):
             k, v = line.split(":", 1)
             vals[k.strip()] = v.strip()
@@ -163,6 +164,87 @@ def momentum_leg(cp, ocean, reps: int = 10) -> dict:
             "face_column_steps_per_s": faces / (ms * 1e-3),
             "algorithmic_GBs": alg / (ms * 1e-3) / 1e9,
             "config": "kv of the plain closure, ocean_vdiff_t defaults (no folds)"}
+
+
+def remap_leg(ocean, seed: int, reps: int = 5, cpu: bool = True) -> dict:
+    """The conservative vertical (ALE) remap on the same tile: T and S
+    remapped from the Lagrangian thicknesses after a step onto the z*
+    target of the same column totals (synth.ale_pair), PPM with the default
+    knobs and the heat/salt remap budget slots, as ocean_apply_ale_remap_centres
+    runs it; then the face-velocity pass.  Kernel time from HIP events on the
+    null stream the kernels run on, averaged over `reps` calls after one
+    untimed call."""
+    from roundabout_amd import synth
+    from roundabout_amd.rdb import remap as R
+
+    t0 = time.perf_counter()
+    ho, hn = synth.ale_pair(ocean, seed)
+    d_ho, d_hn = R.DeviceArray.from_host(ho), R.DeviceArray.from_host(hn)
+    log(f"[bench] remap: ALE pair built and uploaded in {time.perf_counter() - t0:.1f} s")
+    nx, ny, nz = ocean.nx, ocean.ny, ocean.nz
+    cells = nx * ny * nz
+    timer = R.Timer()
+
+    def timed(fn):
+        fn()
+        R.synchronize()
+        timer.start()
+        for _ in range(reps):
+            fn()
+        return timer.stop_ms() / reps
+
+    fields = {}
+    for name, fld in (("T", ocean.T), ("S", ocean.S)):
+        d_c = R.DeviceArray.from_host(fld)
+        d_o, d_b = R.DeviceArray(d_c.shape), R.DeviceArray.zeros(d_c.shape)
+        ms_out = timed(lambda: R.ocean_remap_tracer_field(d_ho, d_hn, d_c, R.REMAP_PPM, budget=d_b, c_out=d_o))
+        ms_in = timed(lambda: R.ocean_remap_tracer_field(d_ho, d_hn, d_c, R.REMAP_PPM, budget=d_b))
+        fields[name] = {"kernel_ms": ms_out, "in_place_ms": ms_in}
+        for d in (d_c, d_o, d_b):
+            d.free()
+    # per cell: h_old, h_new and c read, c_new written, the budget slot read and written
+    bytes_field = cells * 6 * 8
+    ms_field = float(np.mean([f["kernel_ms"] for f in fields.values()]))
+    achieved = bytes_field / (ms_field * 1e-3) / 1e9
+    rng = np.random.default_rng(seed)
+    d_u = R.DeviceArray.from_host(rng.normal(0.0, 0.1, (nx + 1, ny, nz)))
+    d_v = R.DeviceArray.from_host(rng.normal(0.0, 0.1, (nx, ny + 1, nz)))
+    ms_faces = timed(lambda: R.ocean_apply_ale_remap_faces(d_ho, d_hn, d_u, d_v, R.REMAP_PPM))
+    face_cells = ((nx + 1) * ny + nx * (ny + 1)) * nz
+    n_bad, worst_rel, worst_neg = R.ocean_remap_scan_preconditions(d_ho, d_hn)
+    for d in (d_u, d_v, d_ho, d_hn):
+        d.free()
+    out = {
+        "workload": f"{nx}x{ny}x{nz} tile, T and S, z* target of a {1.7:.1f} m free-surface anomaly over "
+                    f"5 %-jittered Lagrangian thicknesses; PPM, default knobs, budget slots on",
+        "kernel": "k_walk<CellCols, PPM, field> (one lane per column, register window merge walk)",
+        "kernel_ms_per_field": ms_field,
+        "fields": fields,
+        "column_remaps_per_s": nx * ny / (ms_field * 1e-3),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "algorithmic_bytes_per_launch": bytes_field,
+                     "bytes_per_cell": 48,
+                     "traffic": read_traffic(f"tile {nx}x{ny}x{nz}", TRAFFIC_FILE.parent / "traffic_remap_field_tile.txt")},
+        "faces_ms": ms_faces,
+        "face_cells": face_cells,
+        "preconditions": {"n_bad": n_bad, "worst_rel": worst_rel, "worst_neg": worst_neg},
+    }
+    if cpu:
+        from oracle import oracle as O
+        sub = synth.crop(ocean, 0, min(360, nx), 0, min(270, ny))
+        sho, shn = ho[:sub.nx, :sub.ny].copy(order="F"), hn[:sub.nx, :sub.ny].copy(order="F")
+        O.set_threads(int(os.environ.get("OMP_NUM_THREADS", host_cpu()["cpus_allowed"])))
+        O.remap_tracer_field(R.REMAP_PPM, sho, shn, sub.T)
+        n, t1 = 0, time.perf_counter()
+        while time.perf_counter() - t1 < 3.0:
+            O.remap_tracer_field(R.REMAP_PPM, sho, shn, sub.T, budget=np.zeros_like(sho, order="F"))
+            n += 1
+        dt = (time.perf_counter() - t1) / n
+        out["cpu_baseline"] = {"value": sub.nx * sub.ny / dt, "unit": "column-remaps/s", "cores": O.num_threads(),
+                               "kind": "port",
+                               "sample": f"oracle remap_tracer_field (PPM, budget) on the {sub.nx}x{sub.ny}x{nz} "
+                                         f"corner crop, {n} calls in {dt * n:.1f} s"}
+    return out
 
 
 def live_rms(ocean, a, b) -> float:
@@ -450,6 +532,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-closure leg")
+    ap.add_argument("--no-remap", action="store_true", help="skip the ALE remap leg")
+    ap.add_argument("--remap-only", action="store_true", help="run only the ALE remap leg (for profiling)")
     ap.add_argument("--ensemble", action="store_true",
                     help="the 16-member batched ensemble (about 10.5 M columns) instead of the single tile")
     ap.add_argument("--guard", default="surface", choices=["strict", "surface", "range"])
@@ -477,6 +561,9 @@ def main() -> int:
 
     t0 = time.perf_counter()
     ocean = synth.WORKLOADS[args.workload](args.seed)
+    if args.remap_only:
+        print(json.dumps({"remap": remap_leg(ocean, args.seed, cpu=not args.no_cpu)}), flush=True)
+        return 0
     log(f"[bench] generated {args.workload} {ocean.nx}x{ocean.ny}x{ocean.nz}, wet {ocean.n_wet} "
         f"in {time.perf_counter() - t0:.1f} s")
     cfg = default_config()
@@ -556,6 +643,9 @@ def main() -> int:
         log("[bench] momentum vertical viscosity ...")
         out["momentum"] = momentum_leg(cp, ocean)
     cp.close()
+    if not args.no_remap:
+        log("[bench] ALE remap leg ...")
+        out["remap"] = remap_leg(ocean, args.seed, cpu=not args.no_cpu)
     if not args.no_surrogate and args.precision == "fp64":
         log("[bench] learned closure leg ...")
         out["surrogate"] = surrogate_leg(ocean, cfg, args, value)

@@ -91,7 +91,8 @@ def build_fortran(verbose: bool = False) -> Path | None:
     cmds = [
         [fc, "-O2", f"-J{mod_dir}", "-c", str(FORTRAN_DIR / "rdb_colphys.F90"), "-o", str(obj)],
         [fc, "-O2", f"-J{mod_dir}", str(FORTRAN_DIR / "colphys_fortran_check.F90"), str(obj),
-         f"-L{LIB_DIR}", "-lrdb_colphys", "-Wl,-rpath,$ORIGIN/../lib", "-o", str(FORTRAN_CHECK)],
+         f"-L{LIB_DIR}", "-lrdb_colphys", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN/../lib",
+         "-o", str(FORTRAN_CHECK)],
     ]
     for cmd in cmds:
         if verbose:

@@ -580,29 +580,40 @@ __global__ __launch_bounds__(TPB) void k_walk(Cols cols, long ncol, int nz, int 
   if (col >= ncol) return;
   cols.bind(col);
 
+  // Source cells arrive through a two-deep prefetch of their raw (h, c)
+  // (loaded two advances before the window takes them), so the load
+  // latency overlaps the reconstruction of the cells in between.
+  struct Raw {
+    double h, c;
+  };
+  auto raw = [&](int m) -> Raw {
+    if (m >= nz) return Raw{0.0, 0.0};
+    return Raw{cols.src_h(m), cols.val(m)};
+  };
   // donor lookahead: the nearest live layer at or above the load front
   int la = -1;
   double la_c = 0.0, last_live = 0.0;
   bool have_live = false;
-  auto load = [&](int m) -> Win {
-    Win w{0.0, 0.0, 0.0};
+  auto finish = [&](int m, Raw r) -> Win {
+    Win w{r.h, r.c, r.c};
     if (m >= nz) return w;
-    w.h = cols.src_h(m);
-    w.co = cols.val(m);
-    w.cd = w.co;
     if (FIELD) {
       if (w.h > HV) {
         last_live = w.co;
         have_live = true;
       } else {
         if (la < m) {
+          // scan up for the next live layer, eight independent loads a round
           la = nz;
-          for (int j = m + 1; j < nz; ++j)
-            if (cols.src_h(j) > HV) {
-              la = j;
-              la_c = cols.val(j);
-              break;
-            }
+          for (int j0 = m + 1; j0 < nz && la == nz; j0 += 8) {
+            double hh[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) hh[u] = (j0 + u < nz) ? cols.src_h(j0 + u) : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+              if (la == nz && hh[u] > HV) la = j0 + u;
+          }
+          if (la < nz) la_c = cols.val(la);
         }
         // the nearest live layer above; none above: the topmost live one
         // (the last live layer loaded); no live layer: 0
@@ -621,9 +632,10 @@ __global__ __launch_bounds__(TPB) void k_walk(Cols cols, long ncol, int nz, int 
   };
 
   Win w0{0.0, 0.0, 0.0};
-  Win w1 = load(0);
-  Win w2 = load(1);
-  Win w3 = load(2);
+  Win w1 = finish(0, raw(0));
+  Win w2 = finish(1, raw(1));
+  Win w3 = finish(2, raw(2));
+  Raw pa = raw(3), pb = raw(4);
   const double cd0 = w1.cd;
   int s = 0;
   double zs_lo = 0.0, zs_hi = 0.0 + w1.h;
@@ -723,16 +735,24 @@ __global__ __launch_bounds__(TPB) void k_walk(Cols cols, long ncol, int nz, int 
     w0 = w1;
     w1 = w2;
     w2 = w3;
-    w3 = load(s + 2);
+    w3 = finish(s + 2, pa);
+    pa = pb;
+    pb = raw(s + 4);
     zs_lo = zs_hi;
     zs_hi = zs_lo + w1.h;
     if (s < nz) recon();
   };
   recon();
 
+  // target thicknesses arrive two targets ahead
+  double hn_a = cols.dst_h(0), hn_b = (nz > 1) ? cols.dst_h(1) : 0.0;
   double zn_hi = 0.0;
   for (int k = 0; k < nz; ++k) {
-    const double hn = cols.dst_h(k);
+    const double hn = hn_a;
+    hn_a = hn_b;
+    hn_b = (k + 2 < nz) ? cols.dst_h(k + 2) : 0.0;
+    const long at = col + ncol * (long)k;
+    const double b_old = (FIELD && budget) ? budget[at] : 0.0;
     const double zn_lo = zn_hi;
     zn_hi = zn_lo + hn;
     // the first source cell whose top lies above the target's bottom
@@ -742,7 +762,10 @@ __global__ __launch_bounds__(TPB) void k_walk(Cols cols, long ncol, int nz, int 
     const double z_first = zs_lo;
     double integral = 0.0, c_lo = 0.0, c_hi = 0.0;
     int n_hit = 0;
-    bool fill = false;
+    // filler excesses s*overlap of the contributors, in source order; more
+    // than two in one target (never on a z* or z-level grid) re-walk
+    double ex0 = 0.0, ex1 = 0.0;
+    int n_ex = 0;
     if (hn > 0.0 && s < nz) {
       for (;;) {
         const double z_lo = dmax(zn_lo, zs_lo);
@@ -781,9 +804,16 @@ __global__ __launch_bounds__(TPB) void k_walk(Cols cols, long ncol, int nz, int 
               }
             }
             if (w1.h <= HV) {
-              lo = lo + (w1.co - w1.cd);
-              hi = hi + (w1.co - w1.cd);
-              if (w1.co - w1.cd != 0.0) fill = true;
+              const double sx = w1.co - w1.cd;
+              lo = lo + sx;
+              hi = hi + sx;
+              if (sx != 0.0) {
+                if (n_ex == 0)
+                  ex0 = sx * ov;
+                else if (n_ex == 1)
+                  ex1 = sx * ov;
+                ++n_ex;
+              }
             }
             if (n_hit == 0) {
               c_lo = lo;
@@ -804,14 +834,16 @@ __global__ __launch_bounds__(TPB) void k_walk(Cols cols, long ncol, int nz, int 
       q = cd0;
     else
       q = (hn > 0.0) ? integral / hn : 0.0;
-    const long at = col + ncol * (long)k;
     if (!FIELD) {
       __builtin_nontemporal_store(q, &out[at]);
       continue;
     }
-    if (fill) {
-      // remap_add_filler_excess: each filler contributor's own excess,
-      // after the division, in source order
+    // remap_add_filler_excess: each filler contributor's own excess, after
+    // the division, in source order
+    if (n_ex <= 2) {
+      if (n_ex > 0) q = q + ex0 / hn;
+      if (n_ex > 1) q = q + ex1 / hn;
+    } else {
       double zj_lo = z_first;
       for (int j = s_first; j <= s; ++j) {
         const double hj = cols.src_h(j);
@@ -835,9 +867,28 @@ __global__ __launch_bounds__(TPB) void k_walk(Cols cols, long ncol, int nz, int 
     __builtin_nontemporal_store(cn, &out[at]);
     if (clamp) clamp[at] = hn * (cn - c_raw);
     if (budget) {
+      // the source cell of level k: from the window when it is there
+      double h_k, c_k;
+      const int d = k - s;
+      if (d == -1) {
+        h_k = w0.h;
+        c_k = w0.co;
+      } else if (d == 0) {
+        h_k = w1.h;
+        c_k = w1.co;
+      } else if (d == 1) {
+        h_k = w2.h;
+        c_k = w2.co;
+      } else if (d == 2) {
+        h_k = w3.h;
+        c_k = w3.co;
+      } else {
+        h_k = cols.src_h(k);
+        c_k = cols.val(k);
+      }
       const double p_new = hn * cn;
-      const double p_old = cols.src_h(k) * cols.val(k);
-      budget[at] = budget[at] + (p_new - p_old);
+      const double p_old = h_k * c_k;
+      budget[at] = b_old + (p_new - p_old);
     }
   }
 }

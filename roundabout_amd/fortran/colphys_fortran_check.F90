@@ -5,7 +5,8 @@
 !>                                 the outputs' checksums and a parity line
 !>                                 against the staged entry points, and a
 !>                                 two-member ensemble whose members equal
-!>                                 the single-grid step bitwise
+!>                                 the single-grid step bitwise, and an ALE
+!>                                 remap through device pointers
 program colphys_fortran_check
    use, intrinsic :: iso_c_binding
    use rdb_colphys
@@ -16,7 +17,7 @@ program colphys_fortran_check
    character(len=8) :: mode
    integer, parameter :: nx = 16, ny = 12, nz = 8
    real(c_double), target :: wet(nx, ny)
-   real(c_double) :: hl(nx, ny, nz), T(nx, ny, nz), S(nx, ny, nz)
+   real(c_double), target :: hl(nx, ny, nz), T(nx, ny, nz), S(nx, ny, nz)
    real(c_double) :: u(nx + 1, ny, nz), v(nx, ny + 1, nz), tx(nx + 1, ny, 1), ty(nx, ny + 1, 1)
    real(c_double) :: qh(nx, ny, 1), qs(nx, ny, 1), T1(nx, ny, nz), T2(nx, ny, nz), kt1(nx, ny, nz + 1)
    real(c_double) :: kt2(nx, ny, nz + 1), heat, salt
@@ -28,6 +29,31 @@ program colphys_fortran_check
    real(c_double) :: hb(nx, ny, nz), tnow
    integer(c_int) :: nsteps
    character(len=*), parameter :: nml = "&ocean_vmix_nml use_kpp = .true., pp81_alpha = 5.0 /"
+   real(c_double), target :: hn(nx, ny, nz), cr(nx, ny, nz), bud(nx, ny, nz)
+   type(c_ptr) :: d_ho, d_hn, d_c, d_b
+   integer(c_int) :: n_bad
+   real(c_double) :: w_rel, w_neg, c0, c1
+   ! the HIP runtime's allocation and copy, for device arrays of this host
+   interface
+      integer(c_int) function hipMalloc(p, nbytes) bind(c, name="hipMalloc")
+         import :: c_int, c_ptr, c_size_t
+         type(c_ptr), intent(out) :: p
+         integer(c_size_t), value :: nbytes
+      end function
+      integer(c_int) function hipFree(p) bind(c, name="hipFree")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: p
+      end function
+      integer(c_int) function hipMemcpy(dst, src, nbytes, kind) bind(c, name="hipMemcpy")
+         import :: c_int, c_ptr, c_size_t
+         type(c_ptr), value :: dst, src
+         integer(c_size_t), value :: nbytes
+         integer(c_int), value :: kind
+      end function
+      integer(c_int) function hipDeviceSynchronize() bind(c, name="hipDeviceSynchronize")
+         import :: c_int
+      end function
+   end interface
 
    call get_command_argument(1, mode)
    st = rdb_colphys_config_default(cfg)
@@ -173,6 +199,53 @@ program colphys_fortran_check
       end if
    end do
    st = rdb_colphys_destroy(h)
+
+   ! ALE remap of T through device pointers onto a grid whose bed layer gave
+   ! 5 m to the surface layer: content conserved per column (the bounding
+   ! moves nothing on this smooth profile), the precondition scan clean
+   hn = hl
+   do j = 1, ny
+      do i = 1, nx
+         if (hl(i, j, 1) > 1.0_c_double) then
+            hn(i, j, 1) = hl(i, j, 1) - 5.0_c_double
+            hn(i, j, nz) = hl(i, j, nz) + 5.0_c_double
+         end if
+      end do
+   end do
+   cr = T
+   bud = 0.0_c_double
+   if (hipMalloc(d_ho, int(8*nx*ny*nz, c_size_t)) /= 0) stop 50
+   if (hipMalloc(d_hn, int(8*nx*ny*nz, c_size_t)) /= 0) stop 50
+   if (hipMalloc(d_c, int(8*nx*ny*nz, c_size_t)) /= 0) stop 50
+   if (hipMalloc(d_b, int(8*nx*ny*nz, c_size_t)) /= 0) stop 50
+   st = hipMemcpy(d_ho, c_loc(hl), int(8*nx*ny*nz, c_size_t), 1_c_int)
+   st = hipMemcpy(d_hn, c_loc(hn), int(8*nx*ny*nz, c_size_t), 1_c_int)
+   st = hipMemcpy(d_c, c_loc(cr), int(8*nx*ny*nz, c_size_t), 1_c_int)
+   st = hipMemcpy(d_b, c_loc(bud), int(8*nx*ny*nz, c_size_t), 1_c_int)
+   st = rdb_remap_scan_preconditions(nx, ny, nz, d_ho, d_hn, 1.0e-9_c_double, n_bad, w_rel, w_neg, c_null_ptr)
+   if (st /= 0 .or. n_bad /= 0 .or. w_neg /= 0.0_c_double) stop 51
+   st = rdb_remap_tracer_field(RDB_REMAP_PPM, nx, ny, nz, d_ho, d_hn, d_c, d_c, 0_c_int, 0_c_int, d_b, &
+                               c_null_ptr, c_null_ptr)
+   if (st /= 0) then
+      print '(a)', trim(colphys_last_error())
+      stop 52
+   end if
+   if (hipDeviceSynchronize() /= 0) stop 53
+   st = hipMemcpy(c_loc(cr), d_c, int(8*nx*ny*nz, c_size_t), 2_c_int)
+   st = hipMemcpy(c_loc(bud), d_b, int(8*nx*ny*nz, c_size_t), 2_c_int)
+   do j = 1, ny
+      do i = 1, nx
+         c0 = sum(hl(i, j, :)*T(i, j, :))
+         c1 = sum(hn(i, j, :)*cr(i, j, :))
+         if (abs(c1 - c0) > 1.0e-12_c_double*abs(c0)) stop 54
+         if (abs(sum(bud(i, j, :)) - (c1 - c0)) > 1.0e-12_c_double*abs(c0)) stop 55
+      end do
+   end do
+   if (all(cr == T)) stop 56
+   st = hipFree(d_ho)
+   st = hipFree(d_hn)
+   st = hipFree(d_c)
+   st = hipFree(d_b)
    print '(a, es24.16, a, es24.16)', "fortran-abi-gpu: ok heat=", heat, " sum(T)=", sum(T1)
 
 contains

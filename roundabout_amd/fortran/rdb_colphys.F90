@@ -29,6 +29,9 @@ module rdb_colphys
    integer(c_int), parameter, public :: RDB_COLPHYS_GUARD_STRICT = 0
    integer(c_int), parameter, public :: RDB_COLPHYS_GUARD_SURFACE = 1
    integer(c_int), parameter, public :: RDB_COLPHYS_GUARD_RANGE = 2
+   !> Remap methods (rdb_constants.F90:208-218)
+   integer(c_int), parameter, public :: RDB_REMAP_PCM = 1, RDB_REMAP_PLM = 2, RDB_REMAP_PPM = 3, &
+                                        RDB_REMAP_PPM_H4 = 4, RDB_REMAP_PQM = 5
 
    !> Mirror of rdb_colphys_config (field order is the ABI).
    type, bind(c), public :: rdb_colphys_config_t
@@ -73,6 +76,7 @@ module rdb_colphys
    public :: rdb_colphys_ensemble_create, rdb_colphys_member_count
    public :: rdb_colphys_member_set_field, rdb_colphys_member_get_field
    public :: colphys_member_set, colphys_member_get
+   public :: rdb_remap_tracer_field, rdb_remap_face_velocity, rdb_remap_columns, rdb_remap_scan_preconditions
    public :: rdb_colphys_mlp_config_default, rdb_colphys_mlp_collect, rdb_colphys_mlp_train
    public :: rdb_colphys_mlp_save, rdb_colphys_mlp_load, rdb_colphys_set_guard
    public :: rdb_colphys_config_default, rdb_colphys_working_precision
@@ -290,6 +294,52 @@ module rdb_colphys
          import :: c_int, c_double, c_ptr
          type(c_ptr), value :: handle
          real(c_double), intent(out) :: t
+      end function
+      !> ocean_remap_tracer_field (rdb_ocean_remap.F90:236) on DEVICE arrays: pass
+      !> c_loc of device-resident data (host_data use_device, or hipMalloc'd
+      !> memory); c_out may be c_in (in place); budget / clamp_out may be
+      !> c_null_ptr; stream c_null_ptr for the null stream.  Returns after
+      !> enqueueing the kernels.
+      integer(c_int) function rdb_remap_tracer_field(method, nx, ny, nz, h_old, h_new, c_in, c_out, bnd_extrap, &
+                                                     nonunif, budget, clamp_out, stream) &
+         bind(c, name="rdb_remap_tracer_field")
+         import :: c_int, c_ptr
+         integer(c_int), value :: method, nx, ny, nz
+         type(c_ptr), value :: h_old, h_new, c_in, c_out
+         integer(c_int), value :: bnd_extrap, nonunif
+         type(c_ptr), value :: budget, clamp_out, stream
+      end function
+      !> remap_x_face_velocity (axis 0) / remap_y_face_velocity (axis 1)
+      !> (rdb_ocean_remap.F90:967-1129) on device arrays, in place.
+      integer(c_int) function rdb_remap_face_velocity(axis, nx, ny, nz, h_old, h_new, u, method, conserve_ke, &
+                                                      zlevel_faces, bnd_extrap, nonunif, stream) &
+         bind(c, name="rdb_remap_face_velocity")
+         import :: c_int, c_ptr
+         integer(c_int), value :: axis, nx, ny, nz
+         type(c_ptr), value :: h_old, h_new, u
+         integer(c_int), value :: method, conserve_ke, zlevel_faces, bnd_extrap, nonunif
+         type(c_ptr), value :: stream
+      end function
+      !> remap_column (rdb_remap_column.F90:328) on ncol device columns (ncol, nz).
+      integer(c_int) function rdb_remap_columns(method, ncol, nz, dz_old, dz_new, q_old, q_new, bnd_extrap, &
+                                                nonunif, stream) bind(c, name="rdb_remap_columns")
+         import :: c_int, c_ptr
+         integer(c_int), value :: method, ncol, nz
+         type(c_ptr), value :: dz_old, dz_new, q_old, q_new
+         integer(c_int), value :: bnd_extrap, nonunif
+         type(c_ptr), value :: stream
+      end function
+      !> ocean_remap_scan_preconditions (rdb_ocean_remap.F90:1348) on device arrays.
+      integer(c_int) function rdb_remap_scan_preconditions(nx, ny, nz, h_old, h_new, rel_tol, n_bad, worst_rel, &
+                                                           worst_neg, stream) &
+         bind(c, name="rdb_remap_scan_preconditions")
+         import :: c_int, c_double, c_ptr
+         integer(c_int), value :: nx, ny, nz
+         type(c_ptr), value :: h_old, h_new
+         real(c_double), value :: rel_tol
+         integer(c_int), intent(out) :: n_bad
+         real(c_double), intent(out) :: worst_rel, worst_neg
+         type(c_ptr), value :: stream
       end function
       integer(c_int) function rdb_colphys_last_error(idx, buf, cap) bind(c, name="rdb_colphys_last_error")
          import :: c_int, c_char

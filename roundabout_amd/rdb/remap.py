@@ -5,18 +5,18 @@ Mirrors the reference's remap interfaces:
 
 * :func:`ocean_remap_tracer_field` -- ``ocean_remap_tracer_field``
   (src/ALE/rdb_ocean_remap.F90:236-319), the concentration-form tracer remap
-  with its budget slot and the ``c_new_out`` / ``clamp_out`` hooks;
+  with its budget slot and the ``clamp_out`` hook;
 * :func:`ocean_apply_ale_remap_faces` -- ``ocean_apply_ale_remap_faces``
   (:915-965): both face-velocity passes;
 * :func:`remap_column` -- ``remap_column`` (src/ALE/rdb_remap_column.F90:
   328-375) over a batch of columns;
 * :func:`ocean_remap_scan_preconditions` -- (:1348-1405).
 
-Arrays are torch tensors on the GPU in the natural layout: a field of the
-reference shape ``(nx, ny, nz)`` (Fortran order, i fastest, k = 1 at the bed)
-is the C-contiguous tensor of shape ``(nz, ny, nx)``; :func:`to_device` and
-:func:`to_host` convert from and to Fortran-ordered numpy arrays of the
-reference shape.  Everything runs on torch's current stream.
+Device memory is :class:`DeviceArray`: a buffer of the HIP runtime the
+library itself runs on, holding a Fortran-ordered float64 array of the
+reference shape (``(nx, ny, nz)`` for a field, i fastest, k = 1 at the bed),
+which is the natural layout the kernels read.  Work is enqueued on the null
+stream; :func:`synchronize` waits for it.
 """
 
 from __future__ import annotations
@@ -30,6 +30,120 @@ from . import _ffi
 REMAP_PCM, REMAP_PLM, REMAP_PPM, REMAP_PPM_H4, REMAP_PQM = 1, 2, 3, 4, 5
 METHODS = {"pcm": REMAP_PCM, "plm": REMAP_PLM, "ppm": REMAP_PPM, "ppm_h4": REMAP_PPM_H4, "pqm": REMAP_PQM}
 OCEAN_REMAP_PRECOND_RTOL = 1.0e-9  # rdb_ocean_remap.F90:49
+_H2D, _D2H, _D2D = 1, 2, 3
+
+_HIP = None
+
+
+def _hip() -> ctypes.CDLL:
+    """The HIP runtime the library was loaded against (by soname, after the
+    library, so both use the same one)."""
+    global _HIP
+    if _HIP is None:
+        _ffi.load()
+        h = ctypes.CDLL("libamdhip64.so.7")
+        vp, sz = ctypes.c_void_p, ctypes.c_size_t
+        h.hipMalloc.argtypes = [ctypes.POINTER(vp), sz]
+        h.hipFree.argtypes = [vp]
+        h.hipMemcpy.argtypes = [vp, vp, sz, ctypes.c_int]
+        h.hipMemset.argtypes = [vp, ctypes.c_int, sz]
+        h.hipDeviceSynchronize.argtypes = []
+        h.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+        h.hipEventRecord.argtypes = [vp, vp]
+        h.hipEventSynchronize.argtypes = [vp]
+        h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+        h.hipEventDestroy.argtypes = [vp]
+        for fn in ("hipMalloc", "hipFree", "hipMemcpy", "hipMemset", "hipDeviceSynchronize", "hipEventCreate",
+                   "hipEventRecord", "hipEventSynchronize", "hipEventElapsedTime", "hipEventDestroy"):
+            getattr(h, fn).restype = ctypes.c_int
+        _HIP = h
+    return _HIP
+
+
+def _chk(rc: int, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError(f"{what}: HIP error {rc}")
+
+
+def synchronize() -> None:
+    _chk(_hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class DeviceArray:
+    """A float64 array of a reference shape in device memory, Fortran
+    order."""
+
+    def __init__(self, shape):
+        self.shape = tuple(int(n) for n in shape)
+        self.nbytes = 8 * int(np.prod(self.shape))
+        p = ctypes.c_void_p()
+        _chk(_hip().hipMalloc(ctypes.byref(p), max(self.nbytes, 8)), "hipMalloc")
+        self._p = p
+
+    @property
+    def ptr(self) -> ctypes.c_void_p:
+        return self._p
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.asfortranarray(np.asarray(a, dtype=np.float64))
+        d = cls(a.shape)
+        d.upload(a)
+        return d
+
+    @classmethod
+    def zeros(cls, shape) -> "DeviceArray":
+        d = cls(shape)
+        _chk(_hip().hipMemset(d._p, 0, d.nbytes), "hipMemset")
+        return d
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.asfortranarray(np.asarray(a, dtype=np.float64))
+        if a.shape != self.shape:
+            raise ValueError(f"shape {a.shape} != {self.shape}")
+        _chk(_hip().hipMemcpy(self._p, a.ctypes.data, self.nbytes, _H2D), "hipMemcpy H2D")
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64, order="F")
+        _chk(_hip().hipMemcpy(out.ctypes.data, self._p, self.nbytes, _D2H), "hipMemcpy D2H")
+        return out
+
+    def copy(self) -> "DeviceArray":
+        d = DeviceArray(self.shape)
+        _chk(_hip().hipMemcpy(d._p, self._p, self.nbytes, _D2D), "hipMemcpy D2D")
+        return d
+
+    def free(self) -> None:
+        if self._p is not None and self._p.value:
+            _hip().hipFree(self._p)
+        self._p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:  # interpreter shutdown
+            pass
+
+
+class Timer:
+    """Kernel time on the null stream from HIP events."""
+
+    def __init__(self):
+        h = _hip()
+        self._a, self._b = ctypes.c_void_p(), ctypes.c_void_p()
+        _chk(h.hipEventCreate(ctypes.byref(self._a)), "hipEventCreate")
+        _chk(h.hipEventCreate(ctypes.byref(self._b)), "hipEventCreate")
+
+    def start(self) -> None:
+        _chk(_hip().hipEventRecord(self._a, None), "hipEventRecord")
+
+    def stop_ms(self) -> float:
+        h = _hip()
+        _chk(h.hipEventRecord(self._b, None), "hipEventRecord")
+        _chk(h.hipEventSynchronize(self._b), "hipEventSynchronize")
+        ms = ctypes.c_float()
+        _chk(h.hipEventElapsedTime(ctypes.byref(ms), self._a, self._b), "hipEventElapsedTime")
+        return float(ms.value)
 
 
 def parse_remap_method(name: str) -> int:
@@ -41,114 +155,73 @@ def parse_remap_method(name: str) -> int:
     return METHODS[key]
 
 
-def _torch():
-    import torch
-    return torch
-
-
-def to_device(a: np.ndarray):
-    """A Fortran-ordered float64 array of the reference shape as a device
-    tensor in the natural layout (shape reversed)."""
-    torch = _torch()
-    a = np.asarray(a, dtype=np.float64)
-    flat = np.ascontiguousarray(a.ravel(order="F"))
-    return torch.from_numpy(flat).to("cuda").view(*reversed(a.shape))
-
-
-def to_host(t) -> np.ndarray:
-    """The inverse of :func:`to_device`."""
-    a = t.detach().to("cpu").contiguous().numpy()
-    return np.asfortranarray(a.reshape(-1).reshape(tuple(reversed(t.shape)), order="F"))
-
-
-def _check(t, shape, name):
-    torch = _torch()
-    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
-        raise TypeError(f"{name}: a contiguous float64 CUDA tensor is required")
-    if tuple(t.shape) != tuple(shape):
-        raise ValueError(f"{name}: shape {tuple(t.shape)} != {tuple(shape)}")
-    return ctypes.c_void_p(t.data_ptr())
-
-
-def _stream():
-    return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
-
-
 def _method(method) -> int:
     return parse_remap_method(method) if isinstance(method, str) else int(method)
 
 
+def _p(d: DeviceArray | None, shape, name):
+    if d is None:
+        return None
+    if not isinstance(d, DeviceArray):
+        raise TypeError(f"{name}: a DeviceArray is required")
+    if d.shape != tuple(shape):
+        raise ValueError(f"{name}: shape {d.shape} != {tuple(shape)}")
+    return d.ptr
+
+
 def ocean_remap_tracer_field(h_old, h_new, c, method=REMAP_PPM, *, bnd_extrap=False, nonunif=False, budget=None,
-                             c_new_out=None, clamp_out=None):
-    """Remap the concentration ``c`` (nz, ny, nx) from ``h_old`` to ``h_new``
-    in place; ``budget`` accumulates h_new*c_new - h_old*c_old,
-    ``c_new_out`` / ``clamp_out`` receive the bounded concentration and the
-    content the bounding moved."""
+                             clamp_out=None, c_out=None):
+    """Remap the concentration ``c`` (nx, ny, nz) from ``h_old`` to ``h_new``:
+    in place, as the reference, or into ``c_out``; ``budget`` accumulates
+    h_new*c_new - h_old*c_old, ``clamp_out`` receives the content the
+    bounding moved."""
     lib = _ffi.load()
-    nz, ny, nx = c.shape
-    shp = (nz, ny, nx)
-    p = [_check(h_old, shp, "h_old"), _check(h_new, shp, "h_new"), _check(c, shp, "c")]
-    pb = _check(budget, shp, "budget") if budget is not None else None
-    pc = _check(clamp_out, shp, "clamp_out") if clamp_out is not None else None
-    _ffi.check(lib.rdb_remap_tracer_field(_method(method), nx, ny, nz, p[0], p[1], p[2], p[2], int(bnd_extrap),
-                                          int(nonunif), pb, pc, _stream()), lib)
-    if c_new_out is not None:
-        c_new_out.copy_(c)
-    return c
-
-
-def remap_tracer_field_into(h_old, h_new, c_in, c_out, method=REMAP_PPM, *, bnd_extrap=False, nonunif=False,
-                            budget=None, clamp_out=None):
-    """:func:`ocean_remap_tracer_field` out of place: ``c_in`` is left as it
-    is and ``c_out`` receives the remapped field (no copy-back)."""
-    lib = _ffi.load()
-    nz, ny, nx = c_in.shape
-    shp = (nz, ny, nx)
-    p = [_check(h_old, shp, "h_old"), _check(h_new, shp, "h_new"), _check(c_in, shp, "c_in"),
-         _check(c_out, shp, "c_out")]
-    pb = _check(budget, shp, "budget") if budget is not None else None
-    pc = _check(clamp_out, shp, "clamp_out") if clamp_out is not None else None
-    _ffi.check(lib.rdb_remap_tracer_field(_method(method), nx, ny, nz, *p, int(bnd_extrap), int(nonunif), pb, pc,
-                                          _stream()), lib)
-    return c_out
+    nx, ny, nz = c.shape
+    shp = (nx, ny, nz)
+    out = c if c_out is None else c_out
+    _ffi.check(lib.rdb_remap_tracer_field(_method(method), nx, ny, nz, _p(h_old, shp, "h_old"),
+                                          _p(h_new, shp, "h_new"), _p(c, shp, "c"), _p(out, shp, "c_out"),
+                                          int(bnd_extrap), int(nonunif), _p(budget, shp, "budget"),
+                                          _p(clamp_out, shp, "clamp_out"), None), lib)
+    return out
 
 
 def ocean_apply_ale_remap_faces(h_old, h_new, u_face_x, v_face_y, method=REMAP_PPM, *, conserve_ke=False,
                                 zlevel_faces=False, bnd_extrap=False, nonunif=False):
-    """Remap u_face_x (nz, ny, nx+1) and v_face_y (nz, ny+1, nx) in place."""
+    """Remap u_face_x (nx+1, ny, nz) and v_face_y (nx, ny+1, nz) in place."""
     lib = _ffi.load()
-    nz, ny, nx = h_old.shape
-    ph = [_check(h_old, (nz, ny, nx), "h_old"), _check(h_new, (nz, ny, nx), "h_new")]
-    for axis, u, shp in ((0, u_face_x, (nz, ny, nx + 1)), (1, v_face_y, (nz, ny + 1, nx))):
+    nx, ny, nz = h_old.shape
+    ph = [_p(h_old, (nx, ny, nz), "h_old"), _p(h_new, (nx, ny, nz), "h_new")]
+    for axis, u, shp in ((0, u_face_x, (nx + 1, ny, nz)), (1, v_face_y, (nx, ny + 1, nz))):
         if u is None:
             continue
-        _ffi.check(lib.rdb_remap_face_velocity(axis, nx, ny, nz, ph[0], ph[1], _check(u, shp, "face velocity"),
+        _ffi.check(lib.rdb_remap_face_velocity(axis, nx, ny, nz, ph[0], ph[1], _p(u, shp, "face velocity"),
                                                _method(method), int(conserve_ke), int(zlevel_faces),
-                                               int(bnd_extrap), int(nonunif), _stream()), lib)
+                                               int(bnd_extrap), int(nonunif), None), lib)
     return u_face_x, v_face_y
 
 
 def remap_column(dz_old, dz_new, q_old, method=REMAP_PPM, *, bnd_extrap=False, nonunif=False, q_new=None):
-    """remap_column over a batch: arrays (nz, ncol), column index fastest."""
-    torch = _torch()
+    """remap_column over a batch of columns: arrays (ncol, nz), column index
+    fastest; q_new may be q_old."""
     lib = _ffi.load()
-    nz, ncol = q_old.shape
-    shp = (nz, ncol)
+    ncol, nz = q_old.shape
     if q_new is None:
-        q_new = torch.empty_like(q_old)
-    p = [_check(dz_old, shp, "dz_old"), _check(dz_new, shp, "dz_new"), _check(q_old, shp, "q_old"),
-         _check(q_new, shp, "q_new")]
-    _ffi.check(lib.rdb_remap_columns(_method(method), ncol, nz, *p, int(bnd_extrap), int(nonunif), _stream()), lib)
+        q_new = DeviceArray((ncol, nz))
+    shp = (ncol, nz)
+    _ffi.check(lib.rdb_remap_columns(_method(method), ncol, nz, _p(dz_old, shp, "dz_old"), _p(dz_new, shp, "dz_new"),
+                                     _p(q_old, shp, "q_old"), _p(q_new, shp, "q_new"), int(bnd_extrap),
+                                     int(nonunif), None), lib)
     return q_new
 
 
 def ocean_remap_scan_preconditions(h_old, h_new, rel_tol=OCEAN_REMAP_PRECOND_RTOL):
     """(n_bad, worst_rel, worst_neg) over every column."""
     lib = _ffi.load()
-    nz, ny, nx = h_old.shape
+    nx, ny, nz = h_old.shape
     n_bad, worst_rel, worst_neg = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
-    _ffi.check(lib.rdb_remap_scan_preconditions(nx, ny, nz, _check(h_old, (nz, ny, nx), "h_old"),
-                                                _check(h_new, (nz, ny, nx), "h_new"), float(rel_tol),
+    _ffi.check(lib.rdb_remap_scan_preconditions(nx, ny, nz, _p(h_old, (nx, ny, nz), "h_old"),
+                                                _p(h_new, (nx, ny, nz), "h_new"), float(rel_tol),
                                                 ctypes.byref(n_bad), ctypes.byref(worst_rel), ctypes.byref(worst_neg),
-                                                _stream()), lib)
+                                                None), lib)
     return n_bad.value, worst_rel.value, worst_neg.value

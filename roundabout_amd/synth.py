@@ -244,4 +244,33 @@ def thin(seed: int = 1, nx: int = 1440, ny: int = 1080, nz: int = 75) -> Ocean:
     return tile(seed, nx, ny, nz, thin_frac=0.05)
 
 
+def ale_pair(o: Ocean, seed: int = 1, eta_amp: float = 1.7, jitter: float = 0.05) -> tuple[np.ndarray, np.ndarray]:
+    """A source/target thickness pair for one ALE remap on the grid of `o`.
+
+    h_old: the Lagrangian thicknesses after a step -- every live layer
+    perturbed by up to +-jitter/2 and the column rescaled to carry a
+    free-surface anomaly eta (a smooth field of amplitude eta_amp metres).
+    h_new: the z* target of the same column total -- the rest thicknesses
+    stretched by (H + eta) / H; the few ulp of mismatch the arithmetic
+    leaves go into the top live target layer.  Vanished layers (bed fillers,
+    land) keep their thickness in both.  Fortran-ordered (nx, ny, nz)."""
+    rng = np.random.default_rng(seed)
+    h0 = np.asarray(o.h)
+    live = h0 > 1.5e-4
+    H = h0.sum(axis=2)
+    H_live = np.where(live, h0, 0.0).sum(axis=2)
+    eta = (eta_amp * np.sin(np.linspace(0.0, 3.0, o.nx))[:, None]
+           * np.cos(np.linspace(0.0, 2.0, o.ny))[None, :])
+    eta = np.where(H_live > 0.0, eta, 0.0)
+    ho = h0 * (1.0 + jitter * (rng.random(h0.shape) - 0.5))
+    ho = np.where(live, ho, h0)
+    ho_live = np.where(live, ho, 0.0).sum(axis=2)
+    ho = np.where(live, ho * ((H_live + eta) / np.maximum(ho_live, 1e-300))[:, :, None], h0)
+    hn = np.where(live, h0 * ((H + eta) / np.maximum(H, 1e-300))[:, :, None], h0)
+    top = np.argmax(np.where(live, np.arange(o.nz)[None, None, :], -1), axis=2)
+    ii, jj = np.meshgrid(np.arange(o.nx), np.arange(o.ny), indexing="ij")
+    hn[ii, jj, top] += ho.sum(axis=2) - hn.sum(axis=2)
+    return np.asfortranarray(ho), np.asfortranarray(hn)
+
+
 WORKLOADS = {"small": small, "tile": tile, "thin": thin}

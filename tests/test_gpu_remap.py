@@ -24,12 +24,15 @@ from roundabout_amd import synth
 from roundabout_amd.rdb import remap as R
 
 pytestmark = pytest.mark.gpu
+# device arrays come from the HIP runtime the library runs on, not from torch:
+# torch's bundled HIP runtime finds no GPU in a process where the library has
+# already initialised the system one (as the earlier GPU tests do)
 METHODS = (1, 2, 3, 4, 5)
 
 
 def _columns(ncol, nz, seed):
-    """(h_old, h_new, q) column batches, (nz, ncol), with the awkward cases
-    mixed in."""
+    """(h_old, h_new, q) column batches, (nz, ncol) (so column-fastest in
+    memory), with the awkward cases mixed in."""
     rng = np.random.default_rng(seed)
     ho = 0.5 + 30.0 * rng.random((nz, ncol))
     hn = 0.5 + 30.0 * rng.random((nz, ncol))
@@ -65,21 +68,21 @@ def _columns(ncol, nz, seed):
 
 @pytest.mark.parametrize("nz", [1, 2, 3, 4, 5, 7, 20, 75])
 def test_remap_columns_bitwise(orc, nz):
-    import torch
     ncol = 257
     ho, hn, q = _columns(ncol, nz, 100 + nz)
-    d_ho, d_hn, d_q = (torch.from_numpy(a).cuda() for a in (ho, hn, q))
+    # the device arrays are (ncol, nz) Fortran-ordered: the same bytes
+    d_ho, d_hn, d_q = (R.DeviceArray.from_host(np.asfortranarray(a.T)) for a in (ho, hn, q))
     for method in METHODS:
         for be in (0, 1):
             for nu in (0, 1):
                 print(f"[case] remap_columns nz={nz} method={method} be={be} nu={nu}", flush=True)
-                got = R.remap_column(d_ho, d_hn, d_q, method, bnd_extrap=be, nonunif=nu).cpu().numpy()
+                got = R.remap_column(d_ho, d_hn, d_q, method, bnd_extrap=be, nonunif=nu).to_host().T
                 want = np.stack([orc.remap_column(method, ho[:, c], hn[:, c], q[:, c], be, nu)
                                  for c in range(ncol)], axis=1)
                 bad = ~((got == want) | (np.isnan(got) & np.isnan(want)))
                 assert not bad.any(), (method, be, nu, int(bad.sum()), np.argwhere(bad)[:3].tolist())
     # in place
-    got = R.remap_column(d_ho, d_hn, d_q, 3, q_new=d_q).cpu().numpy()
+    got = R.remap_column(d_ho, d_hn, d_q, 3, q_new=d_q).to_host().T
     want = np.stack([orc.remap_column(3, ho[:, c], hn[:, c], q[:, c]) for c in range(ncol)], axis=1)
     assert np.array_equal(got, want)
 
@@ -118,35 +121,11 @@ def _field(nx, ny, nz, seed):
     return (np.asfortranarray(ho), np.asfortranarray(hn), np.asfortranarray(c))
 
 
-def _zstar_pair(ocean, seed):
-    """An ALE pair on a tile: h_old, the Lagrangian thicknesses after a
-    step (the live layers perturbed by up to 5 %, the column carrying a
-    free-surface anomaly eta), and h_new, the z* target of the same column
-    total (the rest thicknesses stretched by (H + eta) / H); the few ulp of
-    mismatch go into the top live target layer."""
-    rng = np.random.default_rng(seed)
-    h0 = np.asarray(ocean.h)
-    live = h0 > 1.5e-4
-    H = h0.sum(axis=2)
-    eta = 1.7 * np.sin(np.linspace(0, 3, ocean.nx))[:, None] * np.cos(np.linspace(0, 2, ocean.ny))[None, :]
-    ho = np.where(live, h0 * (1.0 + 0.05 * (rng.random(h0.shape) - 0.5)), h0)
-    hl = np.where(live, ho, 0.0).sum(axis=2)
-    scale = (H + eta - (H - np.where(live, h0, 0.0).sum(axis=2))) / np.maximum(hl, 1e-300)
-    ho = np.where(live, ho * scale[:, :, None], ho)
-    hn = h0 * ((H + eta) / np.maximum(H, 1e-300))[:, :, None]
-    hn = np.where(live, hn, h0)
-    top = np.argmax(np.where(live, np.arange(ocean.nz)[None, None, :], -1), axis=2)
-    ii, jj = np.meshgrid(np.arange(ocean.nx), np.arange(ocean.ny), indexing="ij")
-    hn[ii, jj, top] += ho.sum(axis=2) - hn.sum(axis=2)
-    return np.asfortranarray(ho), np.asfortranarray(hn)
-
-
 @pytest.mark.parametrize("nz", [1, 3, 6, 20])
 def test_remap_tracer_field_bitwise(orc, nz):
-    import torch
     nx, ny = 37, 23
     ho, hn, c = _field(nx, ny, nz, 7 + nz)
-    d_ho, d_hn = R.to_device(ho), R.to_device(hn)
+    d_ho, d_hn = R.DeviceArray.from_host(ho), R.DeviceArray.from_host(hn)
     for method in METHODS:
         for be in (0, 1):
             for nu in (0, 1):
@@ -154,39 +133,39 @@ def test_remap_tracer_field_bitwise(orc, nz):
                 bud0 = np.asfortranarray(np.random.default_rng(1).normal(size=(nx, ny, nz)))
                 want_b = bud0.copy(order="F")
                 want_c, want_cl = orc.remap_tracer_field(method, ho, hn, c, be, nu, budget=want_b)
-                d_c, d_b = R.to_device(c), R.to_device(bud0)
-                d_cl = torch.full_like(d_c, np.nan)
+                d_c, d_b = R.DeviceArray.from_host(c), R.DeviceArray.from_host(bud0)
+                d_cl = R.DeviceArray.from_host(np.full((nx, ny, nz), np.nan, order="F"))
                 if (method + be + nu) % 2:  # in place, as the reference
                     R.ocean_remap_tracer_field(d_ho, d_hn, d_c, method, bnd_extrap=be, nonunif=nu, budget=d_b,
                                                clamp_out=d_cl)
-                    got_c = R.to_host(d_c)
+                    got_c = d_c.to_host()
                 else:
-                    d_out = torch.empty_like(d_c)
-                    R.remap_tracer_field_into(d_ho, d_hn, d_c, d_out, method, bnd_extrap=be, nonunif=nu, budget=d_b,
-                                              clamp_out=d_cl)
-                    assert np.array_equal(R.to_host(d_c), c)  # the input is left alone
-                    got_c = R.to_host(d_out)
-                for name, g, w in (("c", got_c, want_c), ("clamp", R.to_host(d_cl), want_cl),
-                                   ("budget", R.to_host(d_b), want_b)):
+                    d_out = R.DeviceArray((nx, ny, nz))
+                    R.ocean_remap_tracer_field(d_ho, d_hn, d_c, method, bnd_extrap=be, nonunif=nu, budget=d_b,
+                                               clamp_out=d_cl, c_out=d_out)
+                    assert np.array_equal(d_c.to_host(), c)  # the input is left alone
+                    got_c = d_out.to_host()
+                for name, g, w in (("c", got_c, want_c), ("clamp", d_cl.to_host(), want_cl),
+                                   ("budget", d_b.to_host(), want_b)):
                     assert np.array_equal(g, w), (name, method, be, nu, int(np.sum(g != w)))
 
 
 def test_remap_tracer_field_tile_crop_zstar(orc):
     """A z-level 160 x 128 x 75 crop of the tile (partial bottom cells,
     vanished layers below the bathymetry), T and S remapped onto the z*
-    grid of a 1.7 m free-surface rise, PPM default, bitwise."""
+    grid of a free-surface anomaly, PPM default, bitwise."""
     print("[case] remap tile crop 160x128x75 z* PPM, T and S", flush=True)
     ocean = synth.tile(2, nx=160, ny=128)
-    ho, hn = _zstar_pair(ocean, seed=2)
-    d_ho, d_hn = R.to_device(ho), R.to_device(hn)
+    ho, hn = synth.ale_pair(ocean, seed=2)
+    d_ho, d_hn = R.DeviceArray.from_host(ho), R.DeviceArray.from_host(hn)
     for name, fld in (("T", ocean.T), ("S", ocean.S)):
         bud = np.zeros_like(ho, order="F")
         want, want_cl = orc.remap_tracer_field(3, ho, hn, fld, budget=bud)
-        d_c, d_b = R.to_device(fld), R.to_device(np.zeros_like(ho, order="F"))
+        d_c, d_b = R.DeviceArray.from_host(fld), R.DeviceArray.zeros(ho.shape)
         R.ocean_remap_tracer_field(d_ho, d_hn, d_c, 3, budget=d_b)
-        got = R.to_host(d_c)
+        got = d_c.to_host()
         assert np.array_equal(got, want), (name, int(np.sum(got != want)))
-        assert np.array_equal(R.to_host(d_b), bud)
+        assert np.array_equal(d_b.to_host(), bud)
         # per column the content is conserved up to what the bounding moved
         s_old = (ho * fld).sum(axis=2)
         s_new = (hn * got).sum(axis=2) - want_cl.sum(axis=2)
@@ -200,26 +179,32 @@ def test_remap_faces_bitwise(orc, zlevel):
     rng = np.random.default_rng(3)
     u = np.asfortranarray(rng.normal(0, 0.2, (nx + 1, ny, nz)))
     v = np.asfortranarray(rng.normal(0, 0.2, (nx, ny + 1, nz)))
-    d_ho, d_hn = R.to_device(ho), R.to_device(hn)
+    d_ho, d_hn = R.DeviceArray.from_host(ho), R.DeviceArray.from_host(hn)
     for method in METHODS:
         for ke in (0, 1):
             for be, nu in ((0, 0), (1, 1)):
                 print(f"[case] faces zlevel={zlevel} method={method} ke={ke} be={be} nu={nu}", flush=True)
-                d_u, d_v = R.to_device(u), R.to_device(v)
+                d_u, d_v = R.DeviceArray.from_host(u), R.DeviceArray.from_host(v)
                 R.ocean_apply_ale_remap_faces(d_ho, d_hn, d_u, d_v, method, conserve_ke=ke, zlevel_faces=zlevel,
                                               bnd_extrap=be, nonunif=nu)
                 for axis, dev, arr in ((0, d_u, u), (1, d_v, v)):
                     want = orc.remap_face_velocity(axis, ho, hn, arr, method, ke, zlevel, be, nu)
-                    got = R.to_host(dev)
+                    got = dev.to_host()
                     assert np.array_equal(got, want), (axis, method, ke, int(np.sum(got != want)))
 
 
-def test_remap_scan_preconditions(orc):
+def test_remap_scan_preconditions_and_errors(orc, gpu_lib):
     nx, ny, nz = 50, 40, 30
     ho, hn, _ = _field(nx, ny, nz, 5)
     hn = hn.copy(order="F")
     hn[3, 4, 5] = -2.0
     hn[10, 11, 2] += 1e-3
     want = orc.remap_scan_preconditions(ho, hn, 1e-9)
-    got = R.ocean_remap_scan_preconditions(R.to_device(ho), R.to_device(hn), 1e-9)
+    got = R.ocean_remap_scan_preconditions(R.DeviceArray.from_host(ho), R.DeviceArray.from_host(hn), 1e-9)
     assert got == want and got[0] >= 2 and got[2] == -2.0
+    # refusals before any work: bad extents, NULL arrays, a bad face axis
+    from roundabout_amd.rdb import _ffi
+    assert gpu_lib.rdb_remap_tracer_field(3, 0, 4, 4, None, None, None, None, 0, 0, None, None, None) == \
+        _ffi.ERR_BAD_SHAPE
+    assert gpu_lib.rdb_remap_face_velocity(2, 4, 4, 4, None, None, None, 3, 0, 0, 0, 0, None) == _ffi.ERR_BAD_SHAPE
+    assert "axis" in _ffi.last_error(gpu_lib)
