@@ -100,16 +100,85 @@ typedef struct rdb_colphys_config {
   double kv_max;        /* huge */
   double kd_max;        /* huge */
   int vmix_guard;       /* 0 */
-  /* contributors that are no-ops at the defaults; non-default values are
-   * refused at create (RDB_COLPHYS_ERR_CONFIG_VALIDATE) */
-  int conv_enable;      /* 0 */
-  double kv_ml_invz2;   /* 0 */
-  int ddiff_enable;     /* 0 */
+  /* contributors that are no-ops at the defaults (their knobs follow
+   * below); kd_smooth_iterations > 0 needs a horizontal halo and is refused
+   * at create (RDB_COLPHYS_ERR_CONFIG_VALIDATE) */
+  int conv_enable;      /* 0: convective adjustment (rdb_ocean_vmix.F90:1750-1799) */
+  double kv_ml_invz2;   /* 0: kv += K (hmix/z)^2 in the top hmix_fixed (:1716-1748) */
+  int ddiff_enable;     /* 0: double diffusion in the heat/salt split (:1936-2028) */
   int kd_smooth_iterations; /* 0 */
   /* vdiff (rdb_ocean_vdiff.F90 face_thick :1928-1962) */
   int use_harmonic;     /* 0: arithmetic face thickness */
   int precision;        /* RDB_COLPHYS_PREC_* */
+  /* ---- contributors off at the defaults; when on, the step runs the
+   * staged kernels (ocean_vmix_t, rdb_ocean_vmix.F90:305-544) ---- */
+  double hmix_fixed;            /* 20 m: depth scale of kv_ml_invz2 */
+  double conv_kd;               /* 1 m^2/s: convective adjustment diffusivity */
+  double conv_prandtl;          /* 1 */
+  double conv_n2_thresh;        /* 0 s^-2 */
+  double ddiff_strat_param_max; /* 2.55: R_rho cutoff of salt fingering */
+  double ddiff_kappa_s;         /* 1e-4 m^2/s */
+  double ddiff_exp1;            /* 1 */
+  double ddiff_exp2;            /* 3 */
+  double ddiff_param1;          /* 0.909 */
+  double ddiff_param2;          /* 4.6 */
+  double ddiff_param3;          /* -0.54 */
+  double ddiff_mol_diff;        /* 1.5e-6 m^2/s */
+  int ddiff_use_k90;            /* 0 */
+  /* ---- EPBL (ocean_epbl_t, rdb_ocean_epbl.F90:148-330): replaces the
+   * KPP overlay when on (use_kpp is then ignored, as the reference's
+   * configure_ocean_epbl turns it off) ---- */
+  int epbl_enable;              /* 0 */
+  int epbl_mstar_scheme;        /* RDB_EPBL_MSTAR_OM4 */
+  int epbl_vstar_scheme;        /* RDB_EPBL_VSTAR_CUBE_ROOT */
+  int epbl_combine;             /* RDB_EPBL_COMBINE_ADD */
+  int epbl_mld_iteration;       /* 1 */
+  int epbl_mld_max_its;         /* 20 */
+  int epbl_mld_bisection;       /* 0 */
+  int epbl_mld_use_prev_guess;  /* 0 */
+  int epbl_use_lt;              /* 0: Langmuir enhancement (LF17) */
+  int epbl_lt_scheme;           /* RDB_EPBL_LT_RESCALE */
+  int epbl_tke_diags;           /* 0: keep the column TKE budget terms */
+  double epbl_mstar;            /* 1.2 (constant scheme) */
+  double epbl_mstar_cap;        /* -1: off */
+  double epbl_mstar_coef1;      /* 0.3 */
+  double epbl_c_ek;             /* 0.085 */
+  double epbl_mstar_conv_adj;   /* 0 */
+  double epbl_rh18_cn1, epbl_rh18_cn2, epbl_rh18_cn3; /* 0.275, 8, -5 */
+  double epbl_rh18_cs1, epbl_rh18_cs2;                /* 0.2, 0.4 */
+  double epbl_nstar;            /* 0.2 */
+  double epbl_tke_decay;        /* 2.5 */
+  double epbl_wstar_ustar_coef; /* 1 */
+  double epbl_vstar_scale_fac;  /* 1 */
+  double epbl_vstar_surf_fac;   /* 1.2 */
+  double epbl_von_karman;       /* 0.41 */
+  double epbl_ekman_scale_coef; /* 1 */
+  double epbl_min_mix_len;      /* 0 m */
+  double epbl_mixlen_exponent;  /* 2 */
+  double epbl_translay_scale;   /* 0.1 */
+  double epbl_mld_tol;          /* 1 m */
+  double epbl_omega;            /* 7.2921e-5 1/s */
+  double epbl_omega_frac;       /* 0 */
+  double epbl_ustar_min;        /* 1e-8 m/s */
+  double epbl_prandtl;          /* 1 */
+  double epbl_lt_enhance_coef;  /* 0.447 */
+  double epbl_lt_enhance_exp;   /* -1.33 */
+  double epbl_lt_max_enhance;   /* 5 */
+  double epbl_la_frac_hbl;      /* 0.04 */
+  double epbl_lt_lac1, epbl_lt_lac2, epbl_lt_lac3, epbl_lt_lac4, epbl_lt_lac5; /* -0.87, 0, 0, 0.95, 0.95 */
 } rdb_colphys_config;
+
+/* EPBL scheme tags (rdb_ocean_epbl.F90:68-104). */
+#define RDB_EPBL_MSTAR_CONSTANT 1
+#define RDB_EPBL_MSTAR_OM4 2
+#define RDB_EPBL_MSTAR_RH18 3
+#define RDB_EPBL_VSTAR_CUBE_ROOT 1
+#define RDB_EPBL_VSTAR_RH18 2
+#define RDB_EPBL_COMBINE_ADD 1
+#define RDB_EPBL_COMBINE_MAX 2
+#define RDB_EPBL_COMBINE_MAX_SHEAR 3 /* without kappa-shear: exactly ADD */
+#define RDB_EPBL_LT_RESCALE 1
+#define RDB_EPBL_LT_ADDITIVE 2
 
 typedef struct rdb_colphys rdb_colphys;
 
@@ -177,13 +246,19 @@ int rdb_colphys_destroy(rdb_colphys **handle);
  *       use_kpp, pp81_nu0, pp81_nu_bg (also kv_bg), pp81_kappa_bg (also
  *       kt_bg and ks_bg), pp81_alpha, shear2_floor, kpp_ri_crit,
  *       kpp_cs_nonlocal, kpp_c_vt2, kv_max, kd_max, vmix_guard,
- *       harmonic_visc, kv_ml_invz2, kd_smooth_iterations; switches of
+ *       harmonic_visc, kv_ml_invz2, hmix_fixed, kd_smooth_iterations;
+ *       switches of
  *       branches this engine does not run (use_closure = .false.,
  *       interior_closure = 'none', direct_stress, bkgnd_profile,
  *       bkgnd_henyey, bkgnd_prandtl != -1, buoyancy_coeffs = 'eos') return
  *       RDB_COLPHYS_ERR_CONFIG_VALIDATE; the driver's time-step and
  *       advection knobs are accepted and have no effect;
  *   &ocean_eos_nml   eos ('wright' only), c_p, p_ref;
+ *   &ocean_conv_nml  (:226-250) enable, kd_conv, prandtl_conv, n2_thresh;
+ *   &ocean_ddiff_nml (:252-292) enable, strat_param_max, kappa_ddiff_s,
+ *       ddiff_exp1, ddiff_exp2, param1..param3, mol_diff, use_k90;
+ *   &ocean_epbl_nml  (:294-393) every key of the group (mstar_scheme,
+ *       vel_scale_scheme, combine and lt_scheme by name);
  *   &ocean_ic_nml    rho_0;
  *   &ocean_vdiff_nml (rdb_colphys_momentum_config_from_string) the momentum
  *       switches implicit_stress, implicit_drag, implicit_top_drag,
@@ -517,6 +592,19 @@ int rdb_colphys_member_get_field(rdb_colphys *h, int member, const char *name, d
  * be checked against the reference's golden bit table
  * (tests/test_safe_math_golden.inc:557-606). */
 int rdb_colphys_cbrt(const double *x, double *y, int n);
+
+/* The device exp / log / pow / erfc of the EPBL and double-diffusion paths
+ * on n host arguments (fn: RDB_SM_EXP, RDB_SM_LOG, RDB_SM_POW with y the
+ * exponents, RDB_SM_ERFC; y may be NULL otherwise): the position-independent
+ * rdb_sm_exp / rdb_sm_log / rdb_sm_pow / rdb_sm_erfc
+ * (src/shared_module_utilities/rdb_safe_math_core.inc, _pow.inc, _erfc.inc),
+ * so that they can be checked against the reference's golden bit tables
+ * (tests/test_safe_math_golden.inc: G_EXP, G_LOG, G_POW, G_POWBIG, G_ERFC). */
+#define RDB_SM_EXP 0
+#define RDB_SM_LOG 1
+#define RDB_SM_POW 2
+#define RDB_SM_ERFC 3
+int rdb_colphys_safe_math(int fn, const double *x, const double *y, double *out, int n);
 
 /* ---- conservative vertical (ALE) remap, on device arrays --------------- */
 

@@ -510,25 +510,41 @@ void orc_nonlocal(int nx, int ny, int nz, double dt, const double *gamma_t,
   }
 }
 
-/* One column step: EOS -> vmix_build_coefficients (PP81, KPP, split,
- * assemble) -> surface flux -> vdiff_apply_tracers -> non-local, the order
- * of apply_deferred_thermo (rdb_ocean_dyn.F90:3836-3856) with rho refreshed
- * first. */
+/* One column step: EOS -> vmix_build_coefficients -> surface flux ->
+ * vdiff_apply_tracers -> non-local, the order of apply_deferred_thermo
+ * (rdb_ocean_dyn.F90:3836-3856) with rho refreshed first.  The coefficient
+ * build follows vmix_build_coefficients (rdb_ocean_dyn.F90:1473-1634): PP81,
+ * then the KPP overlay or the EPBL solve and merge (EPBL turns KPP off,
+ * rdb_ocean_setup_vmix.F90:552-719), KV_ML_INVZ2, convective adjustment
+ * against the live boundary-layer depth of whichever surface scheme ran,
+ * the heat/salt split (double diffusion when on), the assembly gate. */
 int orc_column_step(const rdb_colphys_config *cfg, orc_state *st, double dt) {
   int nx = st->nx, ny = st->ny, nz = st->nz;
+  int epbl = cfg->epbl_enable != 0;
+  int kpp = cfg->use_kpp && !epbl;
   orc_eos_wright(nx, ny, nz, st->h, st->S, st->T, st->rho, cfg->rho0, cfg->p_ref);
   orc_pp81(cfg, nx, ny, nz, st->u, st->v, st->h, st->rho, st->kv, st->kt);
-  if (cfg->use_kpp)
+  if (kpp)
     orc_kpp_overlay(cfg, nx, ny, nz, st->u, st->v, st->h, st->rho, st->stress_mag,
                     st->stress_shelf, st->Q_heat, st->Q_salt, st->bl_depth, st->kv,
                     st->kt, st->gamma_t, st->gamma_s, st->b0, st->kbl);
-  orc_split(nx, ny, nz, st->kt, st->ks);
+  if (epbl)
+    orc_epbl_compute_merge(cfg, nx, ny, nz, dt, st->h, st->T, st->S, st->stress_mag, st->stress_shelf,
+                           st->Q_heat, st->Q_salt, st->wet, st->coriolis, st->kd_epbl, st->mld, st->b0_epbl,
+                           st->la, cfg->epbl_tke_diags ? st->tke : NULL, st->kv, st->kt);
+  orc_vmix_invz2(cfg, nx, ny, nz, st->h, st->kv);
+  if (cfg->conv_enable)
+    orc_vmix_convection(cfg, nx, ny, nz, st->h, st->rho, epbl ? st->mld : st->bl_depth, st->kv, st->kt);
+  if (cfg->ddiff_enable)
+    orc_vmix_split_ddiff(cfg, nx, ny, nz, st->h, st->T, st->S, st->kt, st->ks);
+  else
+    orc_split(nx, ny, nz, st->kt, st->ks);
   int status = orc_assemble(cfg, nx, ny, nz, st->kv, st->kt, st->ks);
   orc_surface_flux(cfg, nx, ny, nz, dt, st->T, st->S, st->h, st->wet, st->Q_heat,
                    st->Q_salt, st->k_top);
   orc_vdiff_tracers_budget(nx, ny, nz, cfg->use_harmonic, dt, st->kt, st->ks, st->h, st->T,
                            st->S, st->n_passive, st->passive, st->heat_budget, st->salt_budget);
-  if (cfg->use_kpp) orc_nonlocal(nx, ny, nz, dt, st->gamma_t, st->gamma_s, st->h, st->T, st->S);
+  if (kpp) orc_nonlocal(nx, ny, nz, dt, st->gamma_t, st->gamma_s, st->h, st->T, st->S);
   return status;
 }
 

@@ -82,6 +82,10 @@ typedef struct orc_state {
   double *a, *b, *c, *rhs;
   /* heat/salt_budget_vdiff slots (NULL: not kept) */
   double *heat_budget, *salt_budget;
+  /* EPBL (epbl_oracle.c; read only when cfg->epbl_enable): |f| at centres
+   * [nx*ny] (NULL: 0), the persistent MLD, kd_int [nz+1], La, B0 and the six
+   * TKE budget planes [6*nx*ny] (NULL unless cfg->epbl_tke_diags) */
+  double *coriolis, *mld, *kd_epbl, *la, *b0_epbl, *tke;
 } orc_state;
 
 /* One column step in the order of apply_deferred_thermo
@@ -128,6 +132,38 @@ void orc_remap_face_velocity(int axis, int nx, int ny, int nz, const double *h_o
                              int nonunif);
 void orc_remap_scan_preconditions(int nx, int ny, int nz, const double *h_old, const double *h_new, double rel_tol,
                                   int *n_bad, double *worst_rel, double *worst_neg);
+
+/* epbl_oracle.c: the contributors that are off at the defaults (EPBL,
+ * KV_ML_INVZ2, convective adjustment, double diffusion) and the
+ * position-independent math they use. */
+double orc_sm_exp_core(double x, double c);
+double orc_sm_exp(double x);
+double orc_sm_log(double x);
+void orc_sm_log_dd(double x, double *lh, double *ll);
+double orc_sm_pow(double x, double y);
+double orc_sm_erfc(double x);
+void orc_specvol_derivs(double T, double S, double p, double *dsv_dt, double *dsv_ds);
+double orc_epbl_find_mstar(const rdb_colphys_config *c, double b0, double ustar, double bld, double absf);
+double orc_epbl_mixlen_shape(double z_depth, double mld_guess, double translay, double expo, int shaped);
+void orc_epbl_lf17_wave_state(double ustar_w, double rho_ocn, double *u10, double *ustokes, double *kphil);
+double orc_epbl_lf17_la(double ustar_w, double zsl, double ustokes, double kphil);
+double orc_epbl_lt_enhance(const rdb_colphys_config *c, double la, double b0, double ustar, double bld, double absf,
+                           double mstar);
+void orc_epbl_column(const rdb_colphys_config *c, int nz, double dt, const double *h, const double *T,
+                     const double *S, double tau_tot, double Q_heat, double Q_salt, double wet, double f_centre,
+                     int eos_linear, double *kd_int, double *mld, double *b0_out, double *la_out, double *tke);
+void orc_epbl_compute_merge(const rdb_colphys_config *c, int nx, int ny, int nz, double dt, const double *h,
+                            const double *T, const double *S, const double *stress_mag, const double *stress_shelf,
+                            const double *Q_heat, const double *Q_salt, const double *wet, const double *f_centre,
+                            double *kd_int, double *mld, double *b0, double *la, double *tke, double *kv,
+                            double *kt);
+void orc_vmix_invz2(const rdb_colphys_config *c, int nx, int ny, int nz, const double *h, double *kv);
+void orc_vmix_convection(const rdb_colphys_config *c, int nx, int ny, int nz, const double *h, const double *rho,
+                         const double *bl_depth, double *kv, double *kt);
+void orc_ddiff_iface(const rdb_colphys_config *c, double hu, double hl, double Tu, double Tl, double Su, double Sl,
+                     double *kd_t, double *kd_s);
+void orc_vmix_split_ddiff(const rdb_colphys_config *c, int nx, int ny, int nz, const double *h, const double *T,
+                          const double *S, double *kt, double *ks);
 
 #ifdef __cplusplus
 }

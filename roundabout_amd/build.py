@@ -21,7 +21,7 @@ LIB_DIR = PKG / "lib"
 LIB = LIB_DIR / "librdb_colphys.so"
 
 SOURCES = ["colphys_kernels.hip", "colphys_api.hip", "mlp_kernels.hip", "mlp_net.hip", "mlp_api.hip", "surrogate.hip",
-           "momentum.hip", "boundary.hip", "remap.hip"]
+           "momentum.hip", "boundary.hip", "remap.hip", "contrib.hip"]
 ARCH = os.environ.get("RDB_COLPHYS_ARCH", "gfx950")
 
 

@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <cstddef>
 #include <string>
 #include <vector>
 
@@ -295,6 +296,92 @@ bool group_is(const std::string& g, const char* base) {
   return g == base || g == std::string(base) + "_nml";
 }
 
+// Keys of the contributor groups: (name, kind, field).  kind 'b' logical,
+// 'r' real, 'i' integer.
+struct KeySpec {
+  const char* key;
+  char kind;
+  size_t off;
+};
+#define RK(k, f) {k, 'r', offsetof(rdb_colphys_config, f)}
+#define BK(k, f) {k, 'b', offsetof(rdb_colphys_config, f)}
+#define IK(k, f) {k, 'i', offsetof(rdb_colphys_config, f)}
+const KeySpec kConv[] = {BK("enable", conv_enable), RK("kd_conv", conv_kd), RK("prandtl_conv", conv_prandtl),
+                         RK("n2_thresh", conv_n2_thresh)};
+const KeySpec kDdiff[] = {BK("enable", ddiff_enable),          RK("strat_param_max", ddiff_strat_param_max),
+                          RK("kappa_ddiff_s", ddiff_kappa_s),  RK("ddiff_exp1", ddiff_exp1),
+                          RK("ddiff_exp2", ddiff_exp2),        RK("param1", ddiff_param1),
+                          RK("param2", ddiff_param2),          RK("param3", ddiff_param3),
+                          RK("mol_diff", ddiff_mol_diff),      BK("use_k90", ddiff_use_k90)};
+const KeySpec kEpbl[] = {
+    BK("enable", epbl_enable),
+    RK("mstar", epbl_mstar),
+    RK("mstar_cap", epbl_mstar_cap),
+    RK("mstar_coef1", epbl_mstar_coef1),
+    RK("c_ek", epbl_c_ek),
+    RK("mstar_conv_adj", epbl_mstar_conv_adj),
+    RK("rh18_cn1", epbl_rh18_cn1),
+    RK("rh18_cn2", epbl_rh18_cn2),
+    RK("rh18_cn3", epbl_rh18_cn3),
+    RK("rh18_cs1", epbl_rh18_cs1),
+    RK("rh18_cs2", epbl_rh18_cs2),
+    RK("nstar", epbl_nstar),
+    RK("tke_decay", epbl_tke_decay),
+    RK("wstar_ustar_coef", epbl_wstar_ustar_coef),
+    RK("vstar_scale_fac", epbl_vstar_scale_fac),
+    RK("vstar_surf_fac", epbl_vstar_surf_fac),
+    RK("von_karman", epbl_von_karman),
+    RK("ekman_scale_coef", epbl_ekman_scale_coef),
+    RK("min_mix_len", epbl_min_mix_len),
+    RK("mixlen_exponent", epbl_mixlen_exponent),
+    RK("translay_scale", epbl_translay_scale),
+    BK("mld_iteration", epbl_mld_iteration),
+    RK("mld_tol", epbl_mld_tol),
+    IK("mld_max_its", epbl_mld_max_its),
+    BK("mld_bisection", epbl_mld_bisection),
+    BK("mld_use_prev_guess", epbl_mld_use_prev_guess),
+    RK("omega", epbl_omega),
+    RK("omega_frac", epbl_omega_frac),
+    RK("prandtl", epbl_prandtl),
+    BK("tke_diags", epbl_tke_diags),
+    BK("use_lt", epbl_use_lt),
+    RK("lt_enhance_coef", epbl_lt_enhance_coef),
+    RK("lt_enhance_exp", epbl_lt_enhance_exp),
+    RK("lt_max_enhance", epbl_lt_max_enhance),
+    RK("la_frac_hbl", epbl_la_frac_hbl),
+    RK("lt_lac1", epbl_lt_lac1),
+    RK("lt_lac2", epbl_lt_lac2),
+    RK("lt_lac3", epbl_lt_lac3),
+    RK("lt_lac4", epbl_lt_lac4),
+    RK("lt_lac5", epbl_lt_lac5),
+};
+#undef RK
+#undef BK
+#undef IK
+
+int contrib_key(const NmlEntry& e, rdb_colphys_config* c, const char* group, const KeySpec* tab, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    if (e.key != tab[i].key) continue;
+    char* base = reinterpret_cast<char*>(c) + tab[i].off;
+    int st = 0;
+    if (tab[i].kind == 'r') {
+      double x = 0.0;
+      if ((st = as_real(e, &x))) return st;
+      *reinterpret_cast<double*>(base) = x;
+    } else if (tab[i].kind == 'b') {
+      int b = 0;
+      if ((st = as_bool(e, &b))) return st;
+      *reinterpret_cast<int*>(base) = b;
+    } else {
+      int v = 0;
+      if ((st = as_int(e, &v))) return st;
+      *reinterpret_cast<int*>(base) = v;
+    }
+    return 0;
+  }
+  return err(RDB_COLPHYS_ERR_CONFIG_PARSE, "%s: unknown key '%s'", group, e.key.c_str());
+}
+
 // &ocean_vmix_nml (src/core/config/rdb_config_register_vmix.F90:738-870),
 // &ocean_eos_nml (:594-650: c_p, p_ref, eos) and &ocean_ic_nml rho_0
 // (src/core/config/rdb_config_register_core.F90:806) into a config that
@@ -380,7 +467,10 @@ int config_from_entries(const std::vector<NmlEntry>& ents, rdb_colphys_config* c
         if (as_str(e) != "constant")
           return err(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "ocean_vmix_nml: buoyancy_coeffs = '%s' (only constant)",
                      as_str(e).c_str());
-      } else if (k == "hmix_stress" || k == "hmix_fixed" || k == "bkgnd_kd_sfc" || k == "bkgnd_kd_deep" ||
+      } else if (k == "hmix_fixed") {
+        if ((st = as_real(e, &x))) return st;
+        c->hmix_fixed = x;
+      } else if (k == "hmix_stress" || k == "bkgnd_kd_sfc" || k == "bkgnd_kd_deep" ||
                  k == "bkgnd_z0" || k == "bkgnd_delta" || k == "bkgnd_kd_min" || k == "bkgnd_henyey_n0_2omega" ||
                  k == "bkgnd_henyey_max_lat" || k == "dt_therm_ratio" || k == "diabatic_first" ||
                  k == "dt_tracer_advect_ratio" || k == "tracer_advection_scheme" || k == "tracer_recon") {
@@ -404,6 +494,40 @@ int config_from_entries(const std::vector<NmlEntry>& ents, rdb_colphys_config* c
       if (k == "rho_0") {
         if ((st = as_real(e, &x))) return st;
         c->rho0 = x;
+      }
+    } else if (group_is(e.group, "ocean_conv")) {
+      // &ocean_conv_nml (rdb_config_register_vmix.F90:226-250)
+      if ((st = contrib_key(e, c, "ocean_conv_nml", kConv, sizeof kConv / sizeof kConv[0]))) return st;
+    } else if (group_is(e.group, "ocean_ddiff")) {
+      // &ocean_ddiff_nml (:252-292)
+      if ((st = contrib_key(e, c, "ocean_ddiff_nml", kDdiff, sizeof kDdiff / sizeof kDdiff[0]))) return st;
+    } else if (group_is(e.group, "ocean_epbl")) {
+      // &ocean_epbl_nml (:294-393)
+      if (k == "mstar_scheme" || k == "vel_scale_scheme" || k == "combine" || k == "lt_scheme") {
+        const std::string v = as_str(e);
+        int tag = -1;
+        if (k == "mstar_scheme") {
+          tag = v == "constant" ? RDB_EPBL_MSTAR_CONSTANT
+                : v == "om4"    ? RDB_EPBL_MSTAR_OM4
+                : (v == "rh18" || v == "reichl_h18") ? RDB_EPBL_MSTAR_RH18 : -1;
+          c->epbl_mstar_scheme = tag;
+        } else if (k == "vel_scale_scheme") {
+          tag = (v == "cube_root" || v == "cube_root_tke") ? RDB_EPBL_VSTAR_CUBE_ROOT
+                : (v == "rh18" || v == "reichl_h18")      ? RDB_EPBL_VSTAR_RH18 : -1;
+          c->epbl_vstar_scheme = tag;
+        } else if (k == "combine") {
+          tag = (v == "add" || v == "additive") ? RDB_EPBL_COMBINE_ADD
+                : v == "max"       ? RDB_EPBL_COMBINE_MAX
+                : v == "max_shear" ? RDB_EPBL_COMBINE_MAX_SHEAR : -1;
+          c->epbl_combine = tag;
+        } else {
+          tag = v == "rescale" ? RDB_EPBL_LT_RESCALE : (v == "additive" || v == "add") ? RDB_EPBL_LT_ADDITIVE : -1;
+          c->epbl_lt_scheme = tag;
+        }
+        if (tag < 0)
+          return err(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "ocean_epbl_nml: unknown %s '%s'", k.c_str(), v.c_str());
+      } else if ((st = contrib_key(e, c, "ocean_epbl_nml", kEpbl, sizeof kEpbl / sizeof kEpbl[0]))) {
+        return st;
       }
     }
   }

@@ -93,7 +93,114 @@ Params make_params(const rdb_colphys_config& c) {
   p.has_salt = c.has_salt;
   p.use_harmonic = c.use_harmonic;
   p.guard = c.vmix_guard;
+  p.ddiff = c.ddiff_enable ? 1 : 0;
+  p.ddiff_k90 = c.ddiff_use_k90 ? 1 : 0;
+  p.dd_rmax = c.ddiff_strat_param_max;
+  p.dd_kappa_s = c.ddiff_kappa_s;
+  p.dd_exp1 = c.ddiff_exp1;
+  p.dd_exp2 = c.ddiff_exp2;
+  p.dd_p1 = c.ddiff_param1;
+  p.dd_p2 = c.ddiff_param2;
+  p.dd_p3 = c.ddiff_param3;
+  p.dd_mol = c.ddiff_mol_diff;
   return p;
+}
+
+colphys::EpblParams make_epbl_params(const rdb_colphys_config& c) {
+  colphys::EpblParams e{};
+  e.mstar_scheme = c.epbl_mstar_scheme;
+  e.vstar_scheme = c.epbl_vstar_scheme;
+  e.mld_iteration = c.epbl_mld_iteration ? 1 : 0;
+  e.mld_max_its = c.epbl_mld_max_its;
+  e.mld_bisection = c.epbl_mld_bisection ? 1 : 0;
+  e.mld_use_prev_guess = c.epbl_mld_use_prev_guess ? 1 : 0;
+  e.use_lt = c.epbl_use_lt ? 1 : 0;
+  e.lt_scheme = c.epbl_lt_scheme;
+  e.tke_diags = c.epbl_tke_diags ? 1 : 0;
+  e.mstar_const = c.epbl_mstar;
+  e.mstar_cap = c.epbl_mstar_cap;
+  e.mstar_coef1 = c.epbl_mstar_coef1;
+  e.c_ek = c.epbl_c_ek;
+  e.mstar_conv_adj = c.epbl_mstar_conv_adj;
+  e.cn1 = c.epbl_rh18_cn1;
+  e.cn2 = c.epbl_rh18_cn2;
+  e.cn3 = c.epbl_rh18_cn3;
+  e.cs1 = c.epbl_rh18_cs1;
+  e.cs2 = c.epbl_rh18_cs2;
+  e.nstar = c.epbl_nstar;
+  e.tke_decay = c.epbl_tke_decay;
+  e.wstar_ustar_coef = c.epbl_wstar_ustar_coef;
+  e.vstar_scale_fac = c.epbl_vstar_scale_fac;
+  e.vstar_surf_fac = c.epbl_vstar_surf_fac;
+  e.von_karman = c.epbl_von_karman;
+  e.ekman_scale_coef = c.epbl_ekman_scale_coef;
+  e.min_mix_len = c.epbl_min_mix_len;
+  e.mixlen_exponent = c.epbl_mixlen_exponent;
+  e.translay = c.epbl_translay_scale;
+  e.mld_tol = c.epbl_mld_tol;
+  e.omega = c.epbl_omega;
+  e.omega_frac = c.epbl_omega_frac;
+  e.ustar_min = c.epbl_ustar_min;
+  e.lt_coef = c.epbl_lt_enhance_coef;
+  e.lt_exp = c.epbl_lt_enhance_exp;
+  e.lt_max = c.epbl_lt_max_enhance;
+  e.la_frac_hbl = c.epbl_la_frac_hbl;
+  e.lac1 = c.epbl_lt_lac1;
+  e.lac2 = c.epbl_lt_lac2;
+  e.lac3 = c.epbl_lt_lac3;
+  e.lac4 = c.epbl_lt_lac4;
+  e.lac5 = c.epbl_lt_lac5;
+  return e;
+}
+
+// Whether a config turns on a contributor that the fused step does not hold.
+bool has_extras(const rdb_colphys_config& c) {
+  return c.epbl_enable || c.conv_enable || c.ddiff_enable || (c.kv_ml_invz2 > 0.0 && c.hmix_fixed > 0.0);
+}
+
+// Configure-time checks of the contributors off at the defaults
+// (configure_ocean_epbl rdb_ocean_setup_vmix.F90:552-719,
+// configure_ocean_conv :1200-1260, configure_ocean_ddiff :1262-1330).
+int validate_extras(const rdb_colphys_config& c) {
+  if (c.conv_enable) {
+    if (c.conv_kd < 0.0) return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "ocean_conv_nml: kd_conv must be >= 0");
+    if (c.conv_prandtl <= 0.0)
+      return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "ocean_conv_nml: prandtl_conv must be > 0");
+  }
+  if (c.ddiff_enable) {
+    if (c.ddiff_strat_param_max <= 1.0)
+      return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE,
+                        "ocean_ddiff_nml: strat_param_max must be > 1 (the fingering form divides by "
+                        "strat_param_max - 1)");
+    if (c.ddiff_kappa_s < 0.0 || c.ddiff_mol_diff < 0.0)
+      return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "ocean_ddiff_nml: kappa_ddiff_s and mol_diff must be >= 0");
+  }
+  if (c.epbl_enable) {
+    if (c.epbl_mstar_scheme < RDB_EPBL_MSTAR_CONSTANT || c.epbl_mstar_scheme > RDB_EPBL_MSTAR_RH18)
+      return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "ocean_epbl_nml: unknown mstar_scheme %d (constant/om4/rh18)",
+                        c.epbl_mstar_scheme);
+    if (c.epbl_vstar_scheme != RDB_EPBL_VSTAR_CUBE_ROOT && c.epbl_vstar_scheme != RDB_EPBL_VSTAR_RH18)
+      return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "ocean_epbl_nml: unknown vel_scale_scheme %d (cube_root/rh18)",
+                        c.epbl_vstar_scheme);
+    if (c.epbl_combine < RDB_EPBL_COMBINE_ADD || c.epbl_combine > RDB_EPBL_COMBINE_MAX_SHEAR)
+      return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "ocean_epbl_nml: unknown combine %d (add/max/max_shear)",
+                        c.epbl_combine);
+    if (c.epbl_use_lt && c.epbl_lt_scheme != RDB_EPBL_LT_RESCALE && c.epbl_lt_scheme != RDB_EPBL_LT_ADDITIVE)
+      return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "ocean_epbl_nml: unknown lt_scheme %d (rescale/additive)",
+                        c.epbl_lt_scheme);
+    if (c.epbl_mld_iteration && (c.epbl_translay_scale < 0.0 || c.epbl_translay_scale >= 1.0))
+      return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE,
+                        "ocean_epbl_nml: translay_scale must be in [0,1) when mld_iteration is on");
+    if (c.epbl_combine == RDB_EPBL_COMBINE_MAX_SHEAR && c.use_kpp)
+      return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE,
+                        "ocean_epbl_nml: combine='max_shear' is refused with use_kpp=.true.; set use_kpp=.false.");
+    if (c.epbl_mld_iteration && c.epbl_mld_max_its < 1)
+      return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "ocean_epbl_nml: mld_max_its must be >= 1");
+  }
+  if (has_extras(c) && c.precision != RDB_COLPHYS_PREC_FP64)
+    return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE,
+                      "EPBL, KV_ML_INVZ2, convection and double diffusion run in FP64 only (precision 0)");
+  return RDB_COLPHYS_OK;
 }
 
 bool lookup_field(const char* name, int n_passive, FieldSpec& spec, int& tracer) {
@@ -127,6 +234,17 @@ bool lookup_field(const char* name, int n_passive, FieldSpec& spec, int& tracer)
       {"wet_t", {F_COL, false, true}},
       {"heat_budget_vdiff", {F_LAYER, true, true}},
       {"salt_budget_vdiff", {F_LAYER, true, true}},
+      // EPBL (ocean_epbl_t, rdb_ocean_epbl.F90:283-312), kept when it is on
+      {"mld", {F_COL, true, true}},
+      {"kd_epbl", {F_IFACE, false, true}},
+      {"la", {F_COL, false, true}},
+      {"b0_epbl", {F_COL, false, true}},
+      {"tke_wind", {F_COL, false, true}},
+      {"tke_conv", {F_COL, false, true}},
+      {"tke_forcing", {F_COL, false, true}},
+      {"tke_mixing", {F_COL, false, true}},
+      {"tke_mech_decay", {F_COL, false, true}},
+      {"tke_conv_decay", {F_COL, false, true}},
   };
   for (const E& e : table)
     if (std::strcmp(e.n, name) == 0) {
@@ -168,6 +286,15 @@ double* layer_ptr(rdb_colphys* h, const char* name, int tracer) {
   if (!std::strcmp(name, "wet_t")) return s.wet;
   if (!std::strcmp(name, "heat_budget_vdiff")) return s.budT;
   if (!std::strcmp(name, "salt_budget_vdiff")) return s.budS;
+  const colphys::EpblView& e = h->ev;
+  if (!std::strcmp(name, "mld")) return e.mld;
+  if (!std::strcmp(name, "kd_epbl")) return e.kd;
+  if (!std::strcmp(name, "la")) return e.la;
+  if (!std::strcmp(name, "b0_epbl")) return e.b0;
+  static const char* const tke_names[6] = {"tke_wind",   "tke_conv", "tke_forcing",
+                                           "tke_mixing", "tke_mech_decay", "tke_conv_decay"};
+  for (int q = 0; q < 6; ++q)
+    if (!std::strcmp(name, tke_names[q])) return e.tke ? e.tke + (size_t)q * (size_t)s.P : nullptr;
   return nullptr;
 }
 
@@ -315,12 +442,20 @@ int launch_step_all(rdb_colphys* h, double dt, bool surr) {
   return launch_check("k_column_step_ens");
 }
 
-// Staged sequence (one kernel per reference routine) over [p0, p1).
-int staged_coefficients(rdb_colphys* h, int p0, int p1) {
+// Staged sequence (one kernel per reference routine) over [p0, p1), in the
+// order of vmix_build_coefficients (rdb_ocean_dyn.F90:1473-1634): PP81, the
+// KPP overlay or the EPBL solve, the EPBL merge + KV_ML_INVZ2 + convective
+// adjustment, the heat/salt split (double diffusion) and the assembly gate.
+int staged_coefficients(rdb_colphys* h, int p0, int p1, double dt) {
   if (p1 <= p0) return RDB_COLPHYS_OK;
   dim3 g = grid1(p1 - p0, h->block), b(h->block);
   hipLaunchKernelGGL(colphys::k_pp81, g, b, 0, h->stream, h->s, h->pr, p0, p1);
   if (h->cfg.use_kpp) hipLaunchKernelGGL(colphys::k_kpp, g, b, 0, h->stream, h->s, h->pr, p0, p1);
+  if (h->cfg.epbl_enable)
+    hipLaunchKernelGGL(colphys::k_epbl, grid1(p1 - p0, 64), dim3(64), 0, h->stream, h->s, h->ep, h->ev,
+                       h->d_coriolis, h->cfg.rho0, h->cfg.cp, dt, p0, p1);
+  if (h->extras && (h->cpar.epbl || h->cpar.conv || (h->cpar.kv_ml_invz2 > 0.0 && h->cpar.hmix > 0.0)))
+    hipLaunchKernelGGL(colphys::k_contrib, g, b, 0, h->stream, h->s, h->cpar, h->ev, p0, p1);
   dim3 g2(g.x, (unsigned)(h->nz + 1));
   hipLaunchKernelGGL(colphys::k_split_assemble, g2, b, 0, h->stream, h->s, h->pr, p0, p1, h->d_guard);
   return launch_check("coefficients");
@@ -436,6 +571,66 @@ int rdb_colphys_config_default(rdb_colphys_config* c) {
   c->kv_max = 1.7976931348623157e308;
   c->kd_max = 1.7976931348623157e308;
   c->precision = RDB_COLPHYS_PREC_FP64;
+  /* contributors off at the defaults: rdb_config.F90:1768-1772, 2036-2100,
+   * 2644-2677 */
+  c->hmix_fixed = 20.0;
+  c->conv_kd = 1.0;
+  c->conv_prandtl = 1.0;
+  c->conv_n2_thresh = 0.0;
+  c->ddiff_strat_param_max = 2.55;
+  c->ddiff_kappa_s = 1.0e-4;
+  c->ddiff_exp1 = 1.0;
+  c->ddiff_exp2 = 3.0;
+  c->ddiff_param1 = 0.909;
+  c->ddiff_param2 = 4.6;
+  c->ddiff_param3 = -0.54;
+  c->ddiff_mol_diff = 1.5e-6;
+  c->ddiff_use_k90 = 0;
+  c->epbl_enable = 0;
+  c->epbl_mstar_scheme = RDB_EPBL_MSTAR_OM4;
+  c->epbl_vstar_scheme = RDB_EPBL_VSTAR_CUBE_ROOT;
+  c->epbl_combine = RDB_EPBL_COMBINE_ADD;
+  c->epbl_mld_iteration = 1;
+  c->epbl_mld_max_its = 20;
+  c->epbl_mld_bisection = 0;
+  c->epbl_mld_use_prev_guess = 0;
+  c->epbl_use_lt = 0;
+  c->epbl_lt_scheme = RDB_EPBL_LT_RESCALE;
+  c->epbl_tke_diags = 0;
+  c->epbl_mstar = 1.2;
+  c->epbl_mstar_cap = -1.0;
+  c->epbl_mstar_coef1 = 0.3;
+  c->epbl_c_ek = 0.085;
+  c->epbl_mstar_conv_adj = 0.0;
+  c->epbl_rh18_cn1 = 0.275;
+  c->epbl_rh18_cn2 = 8.0;
+  c->epbl_rh18_cn3 = -5.0;
+  c->epbl_rh18_cs1 = 0.2;
+  c->epbl_rh18_cs2 = 0.4;
+  c->epbl_nstar = 0.2;
+  c->epbl_tke_decay = 2.5;
+  c->epbl_wstar_ustar_coef = 1.0;
+  c->epbl_vstar_scale_fac = 1.0;
+  c->epbl_vstar_surf_fac = 1.2;
+  c->epbl_von_karman = 0.41;
+  c->epbl_ekman_scale_coef = 1.0;
+  c->epbl_min_mix_len = 0.0;
+  c->epbl_mixlen_exponent = 2.0;
+  c->epbl_translay_scale = 0.1;
+  c->epbl_mld_tol = 1.0;
+  c->epbl_omega = 7.2921e-5;
+  c->epbl_omega_frac = 0.0;
+  c->epbl_ustar_min = 1.0e-8;
+  c->epbl_prandtl = 1.0;
+  c->epbl_lt_enhance_coef = 0.447;
+  c->epbl_lt_enhance_exp = -1.33;
+  c->epbl_lt_max_enhance = 5.0;
+  c->epbl_la_frac_hbl = 0.04;
+  c->epbl_lt_lac1 = -0.87;
+  c->epbl_lt_lac2 = 0.0;
+  c->epbl_lt_lac3 = 0.0;
+  c->epbl_lt_lac4 = 0.95;
+  c->epbl_lt_lac5 = 0.95;
   return RDB_COLPHYS_OK;
 }
 
@@ -456,12 +651,7 @@ int create_plan(const rdb_colphys_config* cfg, int nx, int ny, int nz, int n_pas
                       n_passive);
   if ((long long)nx * ny > 0x7fffffffLL / 2)
     return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "create: %d x %d columns exceed the index range", nx, ny);
-  if (cfg->conv_enable)
-    return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "convective adjustment (conv_enable) is not supported by the column engine");
-  if (cfg->kv_ml_invz2 != 0.0)
-    return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "kv_ml_invz2 != 0 is not supported by the column engine");
-  if (cfg->ddiff_enable)
-    return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "double diffusion (ddiff_enable) is not supported by the column engine");
+  if (int st0 = validate_extras(*cfg)) return st0;
   if (cfg->kd_smooth_iterations != 0)
     return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "kd_smooth_iterations > 0 needs a horizontal halo; not supported");
   if (cfg->use_harmonic != 0 && cfg->use_harmonic != 1)
@@ -474,7 +664,22 @@ int create_plan(const rdb_colphys_config* cfg, int nx, int ny, int nz, int n_pas
 
   rdb_colphys* h = new rdb_colphys();
   h->cfg = *cfg;
-  h->pr = make_params(*cfg);
+  // EPBL and the KPP overlay are mutually exclusive surface schemes: EPBL
+  // turns KPP off (rdb_ocean_setup_vmix.F90:697-703)
+  if (cfg->epbl_enable) h->cfg.use_kpp = 0;
+  h->pr = make_params(h->cfg);
+  h->extras = has_extras(h->cfg);
+  h->ep = make_epbl_params(h->cfg);
+  h->cpar.epbl = h->cfg.epbl_enable ? 1 : 0;
+  h->cpar.conv = h->cfg.conv_enable ? 1 : 0;
+  h->cpar.combine = h->cfg.epbl_combine;
+  h->cpar.prandtl = h->cfg.epbl_prandtl;
+  h->cpar.kv_ml_invz2 = h->cfg.kv_ml_invz2;
+  h->cpar.hmix = h->cfg.hmix_fixed;
+  h->cpar.conv_kd = h->cfg.conv_kd;
+  h->cpar.conv_prandtl = h->cfg.conv_prandtl;
+  h->cpar.n2_thresh = h->cfg.conv_n2_thresh;
+  h->cpar.rho0 = h->cfg.rho0;
   h->nx = nx;
   h->ny = ny;
   h->nz = nz;
@@ -589,6 +794,13 @@ int create_plan(const rdb_colphys_config* cfg, int nx, int ny, int nz, int n_pas
   A(&h->d_coriolis, 8 * P);
   A(&h->d_guard, 64);
   A(&h->d_acc, 64);
+  if (h->cfg.epbl_enable) {
+    A(&h->ev.kd, 8 * I);
+    A(&h->ev.mld, 8 * P);
+    A(&h->ev.b0, 8 * P);
+    A(&h->ev.la, 8 * P);
+    if (h->cfg.epbl_tke_diags) A(&h->ev.tke, 8 * 6 * P);
+  }
   h->stage_elems = (size_t)(nx + 1) * (size_t)(ny + 1) * (size_t)(nz + 1);
   A(&h->d_stage, 8 * h->stage_elems);
   if (st) {
@@ -659,6 +871,9 @@ int rdb_colphys_ensemble_create(const rdb_colphys_config* cfg, int n_members, co
     return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "ensemble_create: bad member count %d or NULL extents", n_members);
   if (n_passive > 4)
     return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "ensemble_create: at most 4 passive tracers");
+  if (cfg && has_extras(*cfg))
+    return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE,
+                      "ensemble_create: EPBL, KV_ML_INVZ2, convection and double diffusion run on single-grid handles");
   // the parent: a 1 x 1 x 1 all-land grid that owns the stream, the graph,
   // the counters and the learned closure; its members hold the columns
   const double land = 0.0;
@@ -781,7 +996,7 @@ int rdb_colphys_set_field(rdb_colphys* h, const char* name, const double* data, 
   }
   if (!spec.settable) return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "set_field: '%s' is read-only", name);
   if (tracer < 0 && (spec.kind == F_LAYER || spec.kind == F_IFACE || spec.kind == F_COL) && !layer_ptr(h, name, tracer))
-    return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "set_field: '%s' is not kept (rdb_colphys_set_budgets)", name);
+    return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "set_field: '%s' is not kept by this handle (budget slots off, EPBL off or its TKE diagnostics off)", name);
   const int nx = h->nx, ny = h->ny, nz = h->nz;
   int e1 = nx, e2 = ny, e3 = 1;
   switch (spec.kind) {
@@ -935,7 +1150,7 @@ int rdb_colphys_get_field(rdb_colphys* h, const char* name, double* out, int n1,
   }
   const int ncol = h->n_all, P = h->P, B = 256;
   if (spec.kind != F_COL_INT && !layer_ptr(h, name, tracer))
-    return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "get_field: '%s' is not kept (rdb_colphys_set_budgets)", name);
+    return push_error(RDB_COLPHYS_ERR_NOT_FOUND, "get_field: '%s' is not kept by this handle (budget slots off, EPBL off or its TKE diagnostics off)", name);
   if (spec.kind == F_COL_INT) {
     hipLaunchKernelGGL(colphys::k_unpack_int, grid1(ncol, B), dim3(B), 0, h->stream, h->s.kbl, h->d_stage, h->d_col,
                        ncol);
@@ -963,13 +1178,12 @@ int rdb_colphys_eos(rdb_colphys* h) {
 }
 
 int rdb_colphys_coefficients(rdb_colphys* h, double dt) {
-  (void)dt;
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "coefficients: bad handle");
   if (!h->members.empty())
     return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "coefficients: an ensemble handle is reached per member (rdb_colphys_member_*) "
                       "and stepped with rdb_colphys_step");
   ++h->state_ver;
-  int st = staged_coefficients(h, 0, h->n_all);
+  int st = staged_coefficients(h, 0, h->n_all, dt);
   if (st) return st;
   h->static_dirty = true;  // the staged pass rewrote every interface
   if (h->cfg.vmix_guard) {
@@ -1029,13 +1243,30 @@ int rdb_colphys_step(rdb_colphys* h, double dt, int n_steps) {
     // coefficients, n steps captured in one hipGraph and replayed
     st = colphys::surr_run(h, dt, n_steps);
     if (st) return st;
-  } else if (h->n_passive > 4) {
-    // beyond the fused kernel's tracer slots: the staged sequence, per step
+  } else if (h->n_passive > 4 || h->extras) {
+    // beyond the fused kernel's tracer slots, or a contributor it does not
+    // hold: the staged sequence, per step (with a contributor on, over every
+    // column: KV_ML_INVZ2 also reaches dry columns)
+    const int pc = h->extras ? h->n_all : h->n_wet;
+    if (h->timing) {
+      st = ensure_events(h, 2);
+      if (st) return st;
+      HIP_TRY(hipEventRecord(h->events[0], h->stream));
+    }
     for (int i = 0; i < n_steps; ++i) {
-      if ((st = rdb_colphys_eos(h)) || (st = staged_coefficients(h, 0, h->n_wet)) ||
+      if ((st = rdb_colphys_eos(h)) || (st = staged_coefficients(h, 0, pc, dt)) ||
           (st = rdb_colphys_surface_flux(h, dt)) || (st = rdb_colphys_diffuse_tracers(h, dt)) ||
           (st = rdb_colphys_nonlocal(h, dt)))
         return st;
+    }
+    if (h->timing) {
+      HIP_TRY(hipEventRecord(h->events[1], h->stream));
+      HIP_TRY(hipEventSynchronize(h->events[1]));
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, h->events[0], h->events[1]));
+      h->last_ms = ms / n_steps;
+      h->sum_ms += ms;
+      h->timed += n_steps;
     }
   } else {
     if (!h->graph_exec || h->graph_dt != dt || h->graph_n != n_steps || h->graph_block != h->block ||
@@ -1147,7 +1378,9 @@ extern "C" {
 
 int rdb_colphys_autotune(rdb_colphys* h, double dt) {
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "autotune: bad handle");
-  if (h->block_pinned) return RDB_COLPHYS_OK;
+  // the staged path (a contributor the fused step does not hold) has no
+  // launch shape to tune
+  if (h->block_pinned || h->extras) return RDB_COLPHYS_OK;
   int st = refresh_static(h, dt);
   if (st) return st;
   // back up the evolving state (T, S, passives, bl_depth) of every grid
@@ -1246,6 +1479,9 @@ int rdb_colphys_set_precision(rdb_colphys* h, int mode) {
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "set_precision: bad handle");
   if (mode != RDB_COLPHYS_PREC_FP64 && mode != RDB_COLPHYS_PREC_SPLIT)
     return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "set_precision: unknown mode %d", mode);
+  if (h->extras && mode != RDB_COLPHYS_PREC_FP64)
+    return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE,
+                      "set_precision: EPBL, KV_ML_INVZ2, convection and double diffusion run in FP64 only");
   if (mode != h->precision) h->static_dirty = true;
   h->precision = mode;
   for (rdb_colphys* m : h->members) {
@@ -1262,6 +1498,10 @@ int rdb_colphys_set_closure(rdb_colphys* h, int mode) {
     return RDB_COLPHYS_OK;
   }
   if (mode == RDB_COLPHYS_CLOSURE_SURROGATE) {
+    if (h->extras)
+      return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE,
+                        "set_closure: the surrogate replaces PP81 + KPP; it is not defined with EPBL, KV_ML_INVZ2, "
+                        "convection or double diffusion on");
     if (!colphys::surr_ready(h))
       return push_error(RDB_COLPHYS_ERR_NOT_INITIALISED,
                         "set_closure: no surrogate network is trained or loaded into this handle");
@@ -1364,6 +1604,10 @@ int rdb_colphys_mlp_config_default(rdb_colphys_mlp_config* c) {
 int rdb_colphys_mlp_collect(rdb_colphys* h, double dt, int n_steps, int stride, long long* n_samples) {
   if (!valid(h)) return push_error(RDB_COLPHYS_ERR_BAD_HANDLE, "mlp_collect: bad handle");
   if (n_steps < 0) return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "mlp_collect: negative step count");
+  if (h->extras)
+    return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE,
+                      "mlp_collect: the surrogate learns PP81 + KPP; it is not defined with EPBL, KV_ML_INVZ2, "
+                      "convection or double diffusion on");
   return colphys::surr_collect(h, dt, n_steps, stride, n_samples);
 }
 
@@ -1416,6 +1660,28 @@ int rdb_colphys_vdiff_momentum(rdb_colphys* h, double dt, const rdb_colphys_mome
   if (!cfg) return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "vdiff_momentum: NULL config");
   if (!(dt > 0.0) || !std::isfinite(dt)) return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "vdiff_momentum: dt = %g", dt);
   return colphys::mom_solve(h, dt, cfg);
+}
+
+int rdb_colphys_safe_math(int fn, const double* x, const double* y, double* out, int n) {
+  if (fn < RDB_SM_EXP || fn > RDB_SM_ERFC) return push_error(RDB_COLPHYS_ERR_CONFIG_VALIDATE, "safe_math: unknown fn %d", fn);
+  if (n < 0 || n > (1 << 26) || (n > 0 && (!x || !out || (fn == RDB_SM_POW && !y))))
+    return push_error(RDB_COLPHYS_ERR_BAD_SHAPE, "safe_math: n = %d outside [0, 2^26] or NULL arrays", n);
+  if (n == 0) return RDB_COLPHYS_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return push_error(RDB_COLPHYS_ERR_SETUP, "safe_math: no HIP device available");
+  double* d = nullptr;
+  HIP_TRY(hipMalloc((void**)&d, 24 * (size_t)n));
+  hipError_t e = hipMemcpy(d, x, 8 * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess && fn == RDB_SM_POW) e = hipMemcpy(d + n, y, 8 * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(colphys::k_safe_math_eval, grid1(n, 256), dim3(256), 0, 0, fn, d, d + n, d + 2 * (size_t)n, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, d + 2 * (size_t)n, 8 * (size_t)n, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return push_error(RDB_COLPHYS_ERR_SETUP, "safe_math: %s", hipGetErrorString(e));
+  return RDB_COLPHYS_OK;
 }
 
 int rdb_colphys_cbrt(const double* x, double* y, int n) {

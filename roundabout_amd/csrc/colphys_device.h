@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "safe_math.h"
+
 namespace colphys {
 
 constexpr double GRAVITY = 9.80665;     // rdb_constants.F90:37
@@ -26,6 +28,9 @@ struct Params {
   double ri_crit, c_vt2, cs_nonlocal, alpha_T, beta_S, p_ref, cp;
   double kv_bg, kt_bg, ks_bg, kv_max, kd_max;
   int use_kpp, has_heat, has_salt, use_harmonic, guard;
+  // double diffusion inside the heat/salt split (off: ks := kt)
+  int ddiff, ddiff_k90;
+  double dd_rmax, dd_kappa_s, dd_exp1, dd_exp2, dd_p1, dd_p2, dd_p3, dd_mol;
 };
 
 template <typename R>
@@ -160,6 +165,41 @@ __device__ __forceinline__ double face_thick(double ha, double hb, int use_harmo
 __device__ __forceinline__ double conc_add_source(double c, double h, double src) {
   if (h > H_VANISHED) return c + src / (h + H_DIV_EPS);
   return c;
+}
+
+// One interface of vmix_split_ddiff_impl (rdb_ocean_vmix.F90:1936-2028),
+// scalar alpha_T / beta_S: salt fingering (Large et al. 1994) and diffusive
+// convection (Marmorino-Caldwell 1976, or Kelley 1990).  u is the layer
+// above the interface, l the one below.
+__device__ __forceinline__ void ddiff_iface(const Params& pr, double hu, double hl, double Tu, double Tl, double Su,
+                                            double Sl, double& kd_t, double& kd_s) {
+  kd_t = 0.0;
+  kd_s = 0.0;
+  if (!(hu > H_VANISHED && hl > H_VANISHED)) return;
+  const double adT = pr.alpha_T * (Tu - Tl);
+  const double bdS = pr.beta_S * (Su - Sl);
+  if (adT >= bdS && bdS > 0.0) {
+    const double rrho = adT / bdS;
+    if (rrho < pr.dd_rmax) {
+      const double fing0 = (rrho - 1.0) / (pr.dd_rmax - 1.0);
+      const double fing = (pr.dd_exp1 == 1.0) ? fing0 : sm::pow(fing0, pr.dd_exp1);
+      const double dd = sm::pow(1.0 - fing, pr.dd_exp2);
+      kd_s = pr.dd_kappa_s * dd;
+    }
+    kd_t = 0.7 * kd_s;
+  } else if (adT >= bdS && adT < 0.0) {
+    const double rrho = adT / bdS;
+    double dd;
+    if (pr.ddiff_k90)
+      dd = pr.dd_mol * 8.7 * sm::pow(rrho, 1.1);
+    else
+      dd = pr.dd_mol * pr.dd_p1 * sm::exp(pr.dd_p2 * sm::exp(pr.dd_p3 * (1.0 / rrho - 1.0)));
+    kd_t = dd;
+    if (rrho < 0.5)
+      kd_s = 0.15 * rrho * dd;
+    else
+      kd_s = (1.85 * rrho - 0.85) * dd;
+  }
 }
 
 // nonlocal_face_flux (rdb_ocean_vmix.F90:1694-1714).

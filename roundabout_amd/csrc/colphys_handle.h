@@ -119,6 +119,12 @@ struct rdb_colphys {
   long long ens_wet = 0;                   // ensemble: wet columns over all members
   long long row_off = 0;                   // member: first row in the ensemble's network batch
   int lev_off = 0;                         // member: network levels below this grid's bed
+  // contributors off at the defaults (EPBL, KV_ML_INVZ2, convection, double
+  // diffusion): when any is on, every step runs the staged kernels
+  bool extras = false;
+  colphys::EpblParams ep{};
+  colphys::EpblView ev{};                  // allocated when EPBL is on
+  colphys::ContribParams cpar{};
 };
 
 namespace colphys {

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "colphys_device.h"
+#include "../../include/rdb_colphys.h"
 
 namespace colphys {
 
@@ -87,6 +88,31 @@ template <typename R, int NTR, bool TWO_FACTOR, bool SURR>
 __global__ void k_column_step_ens(const StateView* views, const SurrView* svs, const int* n_wet, Params pr, double dt,
                                   const int2* work, int n_work, unsigned int* guard_count, int M);
 __global__ void k_totals(StateView s, int p0, int p1, unsigned long long* acc);
+
+// EPBL knobs (ocean_epbl_t, rdb_ocean_epbl.F90:148-330), copied from the
+// config at create.
+struct EpblParams {
+  int mstar_scheme, vstar_scheme, mld_iteration, mld_max_its, mld_bisection, mld_use_prev_guess, use_lt, lt_scheme;
+  int tke_diags, pad;
+  double mstar_const, mstar_cap, mstar_coef1, c_ek, mstar_conv_adj, cn1, cn2, cn3, cs1, cs2;
+  double nstar, tke_decay, wstar_ustar_coef, vstar_scale_fac, vstar_surf_fac, von_karman, ekman_scale_coef;
+  double min_mix_len, mixlen_exponent, translay, mld_tol, omega, omega_frac, ustar_min;
+  double lt_coef, lt_exp, lt_max, la_frac_hbl, lac1, lac2, lac3, lac4, lac5;
+};
+// EPBL state, packed: kd [nz+1][P], tke [6][P] (wind, conv, forcing,
+// mixing, mech_decay, conv_decay), the rest [P].
+struct EpblView {
+  double *kd, *mld, *b0, *la, *tke;
+};
+// The per-interface contributors after the surface scheme.
+struct ContribParams {
+  int epbl, conv, combine, pad;
+  double prandtl, kv_ml_invz2, hmix, conv_kd, conv_prandtl, n2_thresh, rho0;
+};
+__global__ void k_epbl(StateView s, EpblParams c, EpblView e, const double* fcor, double rho0, double cp, double dt,
+                       int p0, int p1);
+__global__ void k_contrib(StateView s, ContribParams c, EpblView e, int p0, int p1);
+__global__ void k_safe_math_eval(int fn, const double* x, const double* y, double* out, int n);
 // Allow every fused-step kernel lds_bytes of dynamic LDS (needed above 64 KiB).
 hipError_t allow_full_lds(int lds_bytes);
 

@@ -230,8 +230,15 @@ __global__ void k_split_assemble(StateView s, Params pr, int p0, int p1, unsigne
   double kt = LV(s.kt, k);
   double kv = LV(s.kv, k);
   double ks = kt;
+  if (pr.ddiff) {
+    double kd_t = 0.0, kd_s = 0.0;
+    if (k >= 1 && k < s.nz)
+      ddiff_iface(pr, LV(s.h, k), LV(s.h, k - 1), LV(s.T, k), LV(s.T, k - 1), LV(s.S, k), LV(s.S, k - 1), kd_t, kd_s);
+    ks = kt + kd_s;
+    kt = kt + kd_t;
+  }
   if (pr.guard) {
-    if (isnan(kv) || isnan(kt) || kv < 0.0 || kt < 0.0) atomicAdd(guard_count, 1u);
+    if (isnan(kv) || isnan(kt) || isnan(ks) || kv < 0.0 || kt < 0.0 || ks < 0.0) atomicAdd(guard_count, 1u);
   }
   if (k >= 1 && k < s.nz) {
     kv = fmn(fmx(kv, pr.kv_bg), pr.kv_max);

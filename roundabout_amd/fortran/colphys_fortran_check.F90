@@ -95,11 +95,15 @@ program colphys_fortran_check
    qs = 0.0_c_double
 
    if (trim(mode) == "cpu") then
-      cfg%conv_enable = 1
+      ! a configure-time check of a contributor reaches Fortran (the EPBL
+      ! fields of the bind(c) mirror line up with the C struct)
+      cfg%epbl_enable = 1
+      cfg%epbl_translay_scale = 1.0_c_double
       st = rdb_colphys_create(cfg, nx, ny, nz, 0, wet, c_null_ptr, c_null_ptr, h)
       if (st /= RDB_COLPHYS_ERR_CONFIG_VALIDATE .or. c_associated(h)) stop 5
-      if (index(colphys_last_error(), "conv_enable") == 0) stop 6
-      cfg%conv_enable = 0
+      if (index(colphys_last_error(), "translay_scale") == 0) stop 6
+      cfg%epbl_enable = 0
+      cfg%epbl_translay_scale = 0.1_c_double
       st = rdb_colphys_create(cfg, nx, ny, nz, 0, wet, c_null_ptr, c_null_ptr, h)
       if (st /= RDB_COLPHYS_ERR_SETUP .or. c_associated(h)) stop 7
       print '(a)', "fortran-abi-cpu: ok"

@@ -48,6 +48,23 @@ module rdb_colphys
       real(c_double) :: kv_ml_invz2
       integer(c_int) :: ddiff_enable, kd_smooth_iterations
       integer(c_int) :: use_harmonic, precision
+      ! contributors off at the defaults (KV_ML_INVZ2 depth, convection,
+      ! double diffusion, EPBL)
+      real(c_double) :: hmix_fixed, conv_kd, conv_prandtl, conv_n2_thresh
+      real(c_double) :: ddiff_strat_param_max, ddiff_kappa_s, ddiff_exp1, ddiff_exp2
+      real(c_double) :: ddiff_param1, ddiff_param2, ddiff_param3, ddiff_mol_diff
+      integer(c_int) :: ddiff_use_k90
+      integer(c_int) :: epbl_enable, epbl_mstar_scheme, epbl_vstar_scheme, epbl_combine
+      integer(c_int) :: epbl_mld_iteration, epbl_mld_max_its, epbl_mld_bisection, epbl_mld_use_prev_guess
+      integer(c_int) :: epbl_use_lt, epbl_lt_scheme, epbl_tke_diags
+      real(c_double) :: epbl_mstar, epbl_mstar_cap, epbl_mstar_coef1, epbl_c_ek, epbl_mstar_conv_adj
+      real(c_double) :: epbl_rh18_cn1, epbl_rh18_cn2, epbl_rh18_cn3, epbl_rh18_cs1, epbl_rh18_cs2
+      real(c_double) :: epbl_nstar, epbl_tke_decay, epbl_wstar_ustar_coef, epbl_vstar_scale_fac
+      real(c_double) :: epbl_vstar_surf_fac, epbl_von_karman, epbl_ekman_scale_coef, epbl_min_mix_len
+      real(c_double) :: epbl_mixlen_exponent, epbl_translay_scale, epbl_mld_tol, epbl_omega
+      real(c_double) :: epbl_omega_frac, epbl_ustar_min, epbl_prandtl
+      real(c_double) :: epbl_lt_enhance_coef, epbl_lt_enhance_exp, epbl_lt_max_enhance, epbl_la_frac_hbl
+      real(c_double) :: epbl_lt_lac1, epbl_lt_lac2, epbl_lt_lac3, epbl_lt_lac4, epbl_lt_lac5
    end type rdb_colphys_config_t
 
    !> Mirror of rdb_colphys_mlp_config (learned closure training).

@@ -86,7 +86,38 @@ class Config(ctypes.Structure):
         ("kd_smooth_iterations", ctypes.c_int),
         ("use_harmonic", ctypes.c_int),
         ("precision", ctypes.c_int),
-    ]
+        ("hmix_fixed", ctypes.c_double),
+        ("conv_kd", ctypes.c_double),
+        ("conv_prandtl", ctypes.c_double),
+        ("conv_n2_thresh", ctypes.c_double),
+        ("ddiff_strat_param_max", ctypes.c_double),
+        ("ddiff_kappa_s", ctypes.c_double),
+        ("ddiff_exp1", ctypes.c_double),
+        ("ddiff_exp2", ctypes.c_double),
+        ("ddiff_param1", ctypes.c_double),
+        ("ddiff_param2", ctypes.c_double),
+        ("ddiff_param3", ctypes.c_double),
+        ("ddiff_mol_diff", ctypes.c_double),
+        ("ddiff_use_k90", ctypes.c_int),
+        ("epbl_enable", ctypes.c_int),
+        ("epbl_mstar_scheme", ctypes.c_int),
+        ("epbl_vstar_scheme", ctypes.c_int),
+        ("epbl_combine", ctypes.c_int),
+        ("epbl_mld_iteration", ctypes.c_int),
+        ("epbl_mld_max_its", ctypes.c_int),
+        ("epbl_mld_bisection", ctypes.c_int),
+        ("epbl_mld_use_prev_guess", ctypes.c_int),
+        ("epbl_use_lt", ctypes.c_int),
+        ("epbl_lt_scheme", ctypes.c_int),
+        ("epbl_tke_diags", ctypes.c_int),
+    ] + [(n, ctypes.c_double) for n in (
+        "epbl_mstar", "epbl_mstar_cap", "epbl_mstar_coef1", "epbl_c_ek", "epbl_mstar_conv_adj",
+        "epbl_rh18_cn1", "epbl_rh18_cn2", "epbl_rh18_cn3", "epbl_rh18_cs1", "epbl_rh18_cs2",
+        "epbl_nstar", "epbl_tke_decay", "epbl_wstar_ustar_coef", "epbl_vstar_scale_fac", "epbl_vstar_surf_fac",
+        "epbl_von_karman", "epbl_ekman_scale_coef", "epbl_min_mix_len", "epbl_mixlen_exponent",
+        "epbl_translay_scale", "epbl_mld_tol", "epbl_omega", "epbl_omega_frac", "epbl_ustar_min", "epbl_prandtl",
+        "epbl_lt_enhance_coef", "epbl_lt_enhance_exp", "epbl_lt_max_enhance", "epbl_la_frac_hbl",
+        "epbl_lt_lac1", "epbl_lt_lac2", "epbl_lt_lac3", "epbl_lt_lac4", "epbl_lt_lac5")]
 
 
 class MlpConfig(ctypes.Structure):
@@ -161,6 +192,13 @@ class Counters(ctypes.Structure):
     ]
 
 
+# EPBL scheme tags (rdb_ocean_epbl.F90:68-104)
+EPBL_MSTAR_CONSTANT, EPBL_MSTAR_OM4, EPBL_MSTAR_RH18 = 1, 2, 3
+EPBL_VSTAR_CUBE_ROOT, EPBL_VSTAR_RH18 = 1, 2
+EPBL_COMBINE_ADD, EPBL_COMBINE_MAX, EPBL_COMBINE_MAX_SHEAR = 1, 2, 3
+EPBL_LT_RESCALE, EPBL_LT_ADDITIVE = 1, 2
+
+
 def default_config() -> Config:
     """Reference defaults (src/core/rdb_config.F90:1743-1893), filled in Python
     so that hosts without the HIP library can still build a config."""
@@ -193,6 +231,64 @@ def default_config() -> Config:
     c.kd_smooth_iterations = 0
     c.use_harmonic = 0
     c.precision = PREC_FP64
+    c.hmix_fixed = 20.0
+    c.conv_kd = 1.0
+    c.conv_prandtl = 1.0
+    c.conv_n2_thresh = 0.0
+    c.ddiff_strat_param_max = 2.55
+    c.ddiff_kappa_s = 1.0e-4
+    c.ddiff_exp1 = 1.0
+    c.ddiff_exp2 = 3.0
+    c.ddiff_param1 = 0.909
+    c.ddiff_param2 = 4.6
+    c.ddiff_param3 = -0.54
+    c.ddiff_mol_diff = 1.5e-6
+    c.ddiff_use_k90 = 0
+    c.epbl_enable = 0
+    c.epbl_mstar_scheme = EPBL_MSTAR_OM4
+    c.epbl_vstar_scheme = EPBL_VSTAR_CUBE_ROOT
+    c.epbl_combine = EPBL_COMBINE_ADD
+    c.epbl_mld_iteration = 1
+    c.epbl_mld_max_its = 20
+    c.epbl_mld_bisection = 0
+    c.epbl_mld_use_prev_guess = 0
+    c.epbl_use_lt = 0
+    c.epbl_lt_scheme = EPBL_LT_RESCALE
+    c.epbl_tke_diags = 0
+    c.epbl_mstar = 1.2
+    c.epbl_mstar_cap = -1.0
+    c.epbl_mstar_coef1 = 0.3
+    c.epbl_c_ek = 0.085
+    c.epbl_mstar_conv_adj = 0.0
+    c.epbl_rh18_cn1 = 0.275
+    c.epbl_rh18_cn2 = 8.0
+    c.epbl_rh18_cn3 = -5.0
+    c.epbl_rh18_cs1 = 0.2
+    c.epbl_rh18_cs2 = 0.4
+    c.epbl_nstar = 0.2
+    c.epbl_tke_decay = 2.5
+    c.epbl_wstar_ustar_coef = 1.0
+    c.epbl_vstar_scale_fac = 1.0
+    c.epbl_vstar_surf_fac = 1.2
+    c.epbl_von_karman = 0.41
+    c.epbl_ekman_scale_coef = 1.0
+    c.epbl_min_mix_len = 0.0
+    c.epbl_mixlen_exponent = 2.0
+    c.epbl_translay_scale = 0.1
+    c.epbl_mld_tol = 1.0
+    c.epbl_omega = 7.2921e-5
+    c.epbl_omega_frac = 0.0
+    c.epbl_ustar_min = 1.0e-8
+    c.epbl_prandtl = 1.0
+    c.epbl_lt_enhance_coef = 0.447
+    c.epbl_lt_enhance_exp = -1.33
+    c.epbl_lt_max_enhance = 5.0
+    c.epbl_la_frac_hbl = 0.04
+    c.epbl_lt_lac1 = -0.87
+    c.epbl_lt_lac2 = 0.0
+    c.epbl_lt_lac3 = 0.0
+    c.epbl_lt_lac4 = 0.95
+    c.epbl_lt_lac5 = 0.95
     return c
 
 
@@ -253,6 +349,7 @@ PROTOTYPES = [
      [_ip, ctypes.c_int, _fp, ctypes.c_int, _fp, _fp, _fp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
       ctypes.c_double, ctypes.c_double, _fp]),
     ("rdb_colphys_cbrt", ctypes.c_int, [_dp, _dp, ctypes.c_int]),
+    ("rdb_colphys_safe_math", ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
     ("rdb_colphys_momentum_config_default", ctypes.c_int, [ctypes.POINTER(MomentumConfig)]),
     ("rdb_colphys_momentum_config_from_string", ctypes.c_int,
      [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(MomentumConfig)]),

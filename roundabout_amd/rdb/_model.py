@@ -19,8 +19,11 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
 LAYER_FIELDS = ("h_layer", "temperature", "salinity", "rho_layer", "heat_budget_vdiff", "salt_budget_vdiff")
-IFACE_FIELDS = ("kv", "kt", "ks", "gamma_t", "gamma_s")
-COLUMN_FIELDS = ("stress_shelf", "Q_heat", "Q_salt", "bl_depth", "stress_mag", "b0", "kbl", "coriolis", "wet_t")
+IFACE_FIELDS = ("kv", "kt", "ks", "gamma_t", "gamma_s", "kd_epbl")
+COLUMN_FIELDS = ("stress_shelf", "Q_heat", "Q_salt", "bl_depth", "stress_mag", "b0", "kbl", "coriolis", "wet_t",
+                 # EPBL state (rdb_ocean_epbl.F90:283-312), kept when EPBL is on
+                 "mld", "la", "b0_epbl", "tke_wind", "tke_conv", "tke_forcing", "tke_mixing", "tke_mech_decay",
+                 "tke_conv_decay")
 # face-located inputs of the momentum solve's folds (rdb_colphys_vdiff_momentum)
 U_FACE_2D = ("lambda_bot_u", "lambda_top_u", "cover_u", "kv_bbl_u", "bbl_thick_u", "k_top_u", "k_bot_u")
 V_FACE_2D = ("lambda_bot_v", "lambda_top_v", "cover_v", "kv_bbl_v", "bbl_thick_v", "k_top_v", "k_bot_v")

@@ -52,19 +52,53 @@ def test_error_paths_without_device(gpu_lib):
     assert gpu_lib.rdb_colphys_destroy(None) == 0
     assert gpu_lib.rdb_colphys_step(None, 1.0, 1) == _ffi.ERR_BAD_HANDLE
     assert "bad handle" in _ffi.last_error(gpu_lib)
-    # refused off-branch knobs (the reference's contributors that are no-ops at defaults)
-    cfg = _ffi.default_config()
-    cfg.conv_enable = 1
+    # refused knobs: a horizontal smoother, and the configure-time checks of
+    # the contributors (rdb_ocean_setup_vmix.F90:552-719, 1200-1330)
     wet = np.ones((2, 2))
-    st = gpu_lib.rdb_colphys_create(ctypes.byref(cfg), 2, 2, 3, 0, wet.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                    None, None, ctypes.byref(h))
-    assert st == _ffi.ERR_CONFIG_VALIDATE and h.value is None
-    assert "conv_enable" in _ffi.last_error(gpu_lib)
+    for knob, val, msg in (("kd_smooth_iterations", 1, "kd_smooth_iterations"),
+                           ("ddiff_strat_param_max", 1.0, "strat_param_max must be > 1"),
+                           ("conv_prandtl", 0.0, "prandtl_conv must be > 0"),
+                           ("epbl_translay_scale", 1.0, "translay_scale must be in [0,1)"),
+                           ("epbl_combine", 3, "max_shear")):
+        cfg = _ffi.default_config()
+        setattr(cfg, knob, val)
+        cfg.ddiff_enable = int(knob.startswith("ddiff"))
+        cfg.conv_enable = int(knob.startswith("conv"))
+        cfg.epbl_enable = int(knob.startswith("epbl"))
+        st = gpu_lib.rdb_colphys_create(ctypes.byref(cfg), 2, 2, 3, 0,
+                                        wet.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), None, None,
+                                        ctypes.byref(h))
+        assert st == _ffi.ERR_CONFIG_VALIDATE and h.value is None, knob
+        assert msg in _ffi.last_error(gpu_lib), knob
     # snprintf semantics of the ring
     buf = ctypes.create_string_buffer(4)
     n = gpu_lib.rdb_colphys_last_error(0, buf, 4)
     assert n > 3 and len(buf.value) == 3
     assert gpu_lib.rdb_colphys_last_error(10_000, buf, 4) == 0
+
+
+def test_config_mirror_matches_the_c_layout(tmp_path):
+    # sizeof and every field offset of rdb_colphys_config, from gcc, against
+    # the ctypes mirror (the Fortran bind(c) type lists the same fields)
+    import subprocess
+    from roundabout_amd.rdb import _ffi
+    names = [n for n, _ in _ffi.Config._fields_]
+    src = tmp_path / "layout.c"
+    body = "\n".join(f'  printf("%s %zu\\n", "{n}", offsetof(rdb_colphys_config, {n}));' for n in names)
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "rdb_colphys.h"\nint main(void) {\n'
+                   '  printf("sizeof %zu\\n", sizeof(rdb_colphys_config));\n' + body + "\n  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                        text=True).stdout.splitlines())
+    assert int(out["sizeof"]) == ctypes.sizeof(_ffi.Config)
+    for n in names:
+        assert int(out[n]) == getattr(_ffi.Config, n).offset, n
+    ftext = (ROOT / "roundabout_amd" / "fortran" / "rdb_colphys.F90").read_text().lower()
+    block = ftext[ftext.index("type, bind(c), public :: rdb_colphys_config_t"):ftext.index("end type rdb_colphys_config_t")]
+    fnames = [w.strip() for line in block.splitlines()[1:] if "::" in line and not line.strip().startswith("!")
+              for w in line.split("::", 1)[1].split(",")]
+    assert fnames == [n.lower() for n in names]
 
 
 def test_missing_library_fails_loudly(tmp_path):
