@@ -247,6 +247,70 @@ def remap_leg(ocean, seed: int, reps: int = 5, cpu: bool = True) -> dict:
     return out
 
 
+CONTRIB_KNOBS = dict(epbl_enable=1, epbl_tke_diags=1, conv_enable=1, ddiff_enable=1, kv_ml_invz2=1.0e-2)
+
+
+def contrib_leg(ocean, steps: int = 6, cpu: bool = True) -> dict:
+    """The contributors off at the reference defaults, all on, on the same
+    tile: EPBL (OM4 mstar, cube-root v*, MLD iteration, additive merge, TKE
+    ledger kept) in place of KPP, KV_ML_INVZ2, convective adjustment and
+    double diffusion.  The step runs the staged kernels; ms per step from
+    the handle's HIP events around `steps` steps after one untimed step, and
+    the coefficient build (PP81 + EPBL + merge/invz2/convection + split with
+    double diffusion + assemble) timed alone the same way."""
+    from roundabout_amd.rdb import ColumnPhysics, default_config
+
+    cfg = default_config()
+    for k, v in CONTRIB_KNOBS.items():
+        setattr(cfg, k, v)
+    nx, ny, nz = ocean.nx, ocean.ny, ocean.nz
+    with ColumnPhysics.from_ocean(ocean, config=cfg) as cp:
+        cp.step(DT, 1)
+        cp.synchronize()
+        cp.set_timing(True)
+        cp.step(DT, steps)
+        stats = cp.stats()
+        ms_step = stats["sum_kernel_ms"] / max(stats["timed_launches"], 1)
+        cp.set_timing(False)
+        cp.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            cp.coefficients(DT)
+        cp.synchronize()
+        ms_coef = (time.perf_counter() - t0) * 1e3 / steps
+        mld = cp.get_field("mld")
+        wet = ocean.wet > 0.5
+        n_wet = int(wet.sum())
+        out = {
+            "workload": f"{nx}x{ny}x{nz} tile, EPBL (OM4, cube-root v*, MLD iteration, add) + KV_ML_INVZ2 "
+                        f"(1e-2 m^2/s, 20 m) + convective adjustment + double diffusion, dt={DT:.0f} s",
+            "ms_per_step": ms_step,
+            "column_steps_per_s": n_wet / (ms_step * 1e-3),
+            "coefficients_ms": ms_coef,
+            "mld_m": {"p5": float(np.percentile(mld[wet], 5)), "median": float(np.median(mld[wet])),
+                      "p95": float(np.percentile(mld[wet], 95))},
+            "kernels": "k_eos, k_epbl, k_coeff_tail (PP81 + merge + KV_ML_INVZ2 + convection + split with "
+                       "double diffusion + assemble), k_surface_flux, k_vdiff (staged, one lane per column)",
+        }
+    if cpu:
+        from oracle import oracle as O
+        from roundabout_amd import synth
+        sub = synth.crop(ocean, 0, min(240, nx), 0, min(180, ny))
+        O.set_threads(int(os.environ.get("OMP_NUM_THREADS", host_cpu()["cpus_allowed"])))
+        st = O.OracleState(sub, cfg)
+        st.step(DT, 1)
+        n, t1 = 0, time.perf_counter()
+        while time.perf_counter() - t1 < 5.0:
+            st.step(DT, 1)
+            n += 1
+        dt = (time.perf_counter() - t1) / n
+        out["cpu_baseline"] = {"value": sub.n_wet / dt, "unit": "column-steps/s", "cores": O.num_threads(),
+                               "kind": "port",
+                               "sample": f"oracle column step with the same contributors on the {sub.nx}x{sub.ny}x"
+                                         f"{nz} corner crop, {n} steps in {dt * n:.1f} s"}
+    return out
+
+
 def live_rms(ocean, a, b) -> float:
     live = (ocean.h > 1.5e-4) & (ocean.wet > 0.5)[:, :, None]
     d = (a - b)[live]
@@ -534,6 +598,9 @@ def main() -> int:
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-closure leg")
     ap.add_argument("--no-remap", action="store_true", help="skip the ALE remap leg")
     ap.add_argument("--remap-only", action="store_true", help="run only the ALE remap leg (for profiling)")
+    ap.add_argument("--no-contrib", action="store_true", help="skip the EPBL/convection/double-diffusion leg")
+    ap.add_argument("--contrib-only", action="store_true",
+                    help="run only the EPBL/convection/double-diffusion leg (for profiling)")
     ap.add_argument("--ensemble", action="store_true",
                     help="the 16-member batched ensemble (about 10.5 M columns) instead of the single tile")
     ap.add_argument("--guard", default="surface", choices=["strict", "surface", "range"])
@@ -563,6 +630,9 @@ def main() -> int:
     ocean = synth.WORKLOADS[args.workload](args.seed)
     if args.remap_only:
         print(json.dumps({"remap": remap_leg(ocean, args.seed, cpu=not args.no_cpu)}), flush=True)
+        return 0
+    if args.contrib_only:
+        print(json.dumps({"contrib": contrib_leg(ocean, args.steps, cpu=not args.no_cpu)}), flush=True)
         return 0
     log(f"[bench] generated {args.workload} {ocean.nx}x{ocean.ny}x{ocean.nz}, wet {ocean.n_wet} "
         f"in {time.perf_counter() - t0:.1f} s")
@@ -646,6 +716,9 @@ def main() -> int:
     if not args.no_remap:
         log("[bench] ALE remap leg ...")
         out["remap"] = remap_leg(ocean, args.seed, cpu=not args.no_cpu)
+    if not args.no_contrib and args.precision == "fp64":
+        log("[bench] EPBL / convection / double-diffusion leg ...")
+        out["contrib"] = contrib_leg(ocean, cpu=not args.no_cpu)
     if not args.no_surrogate and args.precision == "fp64":
         log("[bench] learned closure leg ...")
         out["surrogate"] = surrogate_leg(ocean, cfg, args, value)

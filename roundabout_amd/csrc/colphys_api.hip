@@ -449,13 +449,25 @@ int launch_step_all(rdb_colphys* h, double dt, bool surr) {
 int staged_coefficients(rdb_colphys* h, int p0, int p1, double dt) {
   if (p1 <= p0) return RDB_COLPHYS_OK;
   dim3 g = grid1(p1 - p0, h->block), b(h->block);
+  if (h->extras) {
+    // EPBL reads only h, T and S: it runs first, and PP81 folds into the
+    // one-pass tail (merge, KV_ML_INVZ2, convection, split, assemble); the
+    // KPP overlay needs PP81's values first, so PP81 then stays separate
+    // (over every column: a dry column still books its B0)
+    if (h->cfg.epbl_enable)
+      hipLaunchKernelGGL(colphys::k_epbl, grid1(h->n_all - p0, 64), dim3(64), 0, h->stream, h->s, h->ep, h->ev,
+                         h->d_coriolis, h->cfg.rho0, h->cfg.cp, dt, p0, h->n_all);
+    const int fold_pp81 = h->cfg.use_kpp ? 0 : 1;
+    if (!fold_pp81) {
+      hipLaunchKernelGGL(colphys::k_pp81, g, b, 0, h->stream, h->s, h->pr, p0, p1);
+      hipLaunchKernelGGL(colphys::k_kpp, g, b, 0, h->stream, h->s, h->pr, p0, p1);
+    }
+    hipLaunchKernelGGL(colphys::k_coeff_tail, g, b, 0, h->stream, h->s, h->pr, h->cpar, h->ev, fold_pp81, p0, p1,
+                       h->d_guard);
+    return launch_check("coefficients");
+  }
   hipLaunchKernelGGL(colphys::k_pp81, g, b, 0, h->stream, h->s, h->pr, p0, p1);
   if (h->cfg.use_kpp) hipLaunchKernelGGL(colphys::k_kpp, g, b, 0, h->stream, h->s, h->pr, p0, p1);
-  if (h->cfg.epbl_enable)
-    hipLaunchKernelGGL(colphys::k_epbl, grid1(p1 - p0, 64), dim3(64), 0, h->stream, h->s, h->ep, h->ev,
-                       h->d_coriolis, h->cfg.rho0, h->cfg.cp, dt, p0, p1);
-  if (h->extras && (h->cpar.epbl || h->cpar.conv || (h->cpar.kv_ml_invz2 > 0.0 && h->cpar.hmix > 0.0)))
-    hipLaunchKernelGGL(colphys::k_contrib, g, b, 0, h->stream, h->s, h->cpar, h->ev, p0, p1);
   dim3 g2(g.x, (unsigned)(h->nz + 1));
   hipLaunchKernelGGL(colphys::k_split_assemble, g2, b, 0, h->stream, h->s, h->pr, p0, p1, h->d_guard);
   return launch_check("coefficients");
@@ -1247,7 +1259,11 @@ int rdb_colphys_step(rdb_colphys* h, double dt, int n_steps) {
     // beyond the fused kernel's tracer slots, or a contributor it does not
     // hold: the staged sequence, per step (with a contributor on, over every
     // column: KV_ML_INVZ2 also reaches dry columns)
-    const int pc = h->extras ? h->n_all : h->n_wet;
+    // Dry columns hold no live layer (the land contract), so their tracer
+    // solve is the identity and is skipped; their coefficients change only
+    // through KV_ML_INVZ2.
+    const bool invz2 = h->cpar.kv_ml_invz2 > 0.0 && h->cpar.hmix > 0.0;
+    const int pc = (h->extras && invz2) ? h->n_all : h->n_wet;
     if (h->timing) {
       st = ensure_events(h, 2);
       if (st) return st;
@@ -1255,9 +1271,14 @@ int rdb_colphys_step(rdb_colphys* h, double dt, int n_steps) {
     }
     for (int i = 0; i < n_steps; ++i) {
       if ((st = rdb_colphys_eos(h)) || (st = staged_coefficients(h, 0, pc, dt)) ||
-          (st = rdb_colphys_surface_flux(h, dt)) || (st = rdb_colphys_diffuse_tracers(h, dt)) ||
-          (st = rdb_colphys_nonlocal(h, dt)))
+          (st = rdb_colphys_surface_flux(h, dt)))
         return st;
+      if (h->n_wet > 0) {
+        hipLaunchKernelGGL(colphys::k_vdiff, grid1(h->n_wet, h->block), dim3(h->block), 0, h->stream, h->s, h->pr,
+                           dt, 0, h->n_wet);
+        if ((st = launch_check("k_vdiff"))) return st;
+      }
+      if ((st = rdb_colphys_nonlocal(h, dt))) return st;
     }
     if (h->timing) {
       HIP_TRY(hipEventRecord(h->events[1], h->stream));

@@ -111,7 +111,8 @@ struct ContribParams {
 };
 __global__ void k_epbl(StateView s, EpblParams c, EpblView e, const double* fcor, double rho0, double cp, double dt,
                        int p0, int p1);
-__global__ void k_contrib(StateView s, ContribParams c, EpblView e, int p0, int p1);
+__global__ void k_coeff_tail(StateView s, Params pr, ContribParams c, EpblView e, int do_pp81, int p0, int p1,
+                             unsigned int* guard_count);
 __global__ void k_safe_math_eval(int fn, const double* x, const double* y, double* out, int n);
 // Allow every fused-step kernel lds_bytes of dynamic LDS (needed above 64 KiB).
 hipError_t allow_full_lds(int lds_bytes);

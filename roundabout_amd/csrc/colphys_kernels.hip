@@ -272,16 +272,28 @@ __global__ void k_surface_flux(StateView s, Params pr, double dt, int p0, int p1
   }
 }
 
-// A12 factorise one diffusivity field into (a, b', c') scratch.
-__device__ void factorize_column(const StateView& s, const double* __restrict__ K, double dt, int use_harmonic,
-                                 int p) {
+// A11-A13 for a group of G tracers that share one diffusivity field K: the
+// rows of build_factorize_tracer_matrix (rdb_ocean_vdiff.F90:1288-1333), the
+// pivots and the forward sweep of apply_factored_tracer (:1351-1414) in one
+// pass up the column, the back substitution in one pass down.  The
+// pivot-scaled super-diagonal c' goes to the fc scratch and the forward
+// residuals stay in place in the tracers, so every level is read and
+// written once per pass; the arithmetic is the reference's, operation for
+// operation (piv = b - a c'_{k-1}, c' = c / piv, r = (x - a r_{k-1}) / piv,
+// x = r - c' x_{k+1}).  With a budget slot, budget += h*c_new - h*c_old.
+template <int G>
+__device__ __forceinline__ void solve_group(const StateView& s, const double* __restrict__ K, double dt,
+                                            int use_harmonic, int p, double* const (&xs)[G],
+                                            double* const (&bud)[G], double* const (&pold)[G]) {
   const int P = s.P, nz = s.nz;
-  double h_km1 = 0.0, h_k = LV(s.h, 0);
+  double r[G];
+  double h_km1 = 0.0, h_k = LV(s.h, 0), K_k = LV(K, 0);
   double cprev = 0.0;
   for (int k = 0; k < nz; ++k) {
-    double h_kp1 = (k + 1 < nz) ? LV(s.h, k + 1) : 0.0;
+    const double h_kp1 = (k + 1 < nz) ? LV(s.h, k + 1) : 0.0;
+    const double K_kp1 = LV(K, k + 1);
     double a, b, c;
-    tracer_row(k, nz, dt, use_harmonic, h_km1, h_k, h_kp1, LV(K, k), LV(K, k + 1), a, b, c);
+    tracer_row(k, nz, dt, use_harmonic, h_km1, h_k, h_kp1, K_k, K_kp1, a, b, c);
     double piv;
     if (k == 0) {
       cprev = c / b;
@@ -290,45 +302,83 @@ __device__ void factorize_column(const StateView& s, const double* __restrict__ 
       piv = b - a * cprev;
       cprev = c / piv;
     }
-    LV(s.fa, k) = a;
-    LV(s.fb, k) = piv;
     LV(s.fc, k) = cprev;
+#pragma unroll
+    for (int t = 0; t < G; ++t) {
+      const double x = LV(xs[t], k);
+      if (bud[t]) LV(pold[t], k) = h_k * x;
+      r[t] = (k == 0) ? x / piv : (x - a * r[t]) / piv;
+      LV(xs[t], k) = r[t];
+    }
     h_km1 = h_k;
     h_k = h_kp1;
+    K_k = K_kp1;
   }
-}
-
-// A13 forward sweep + back substitution of one tracer against stored pivots;
-// with a budget slot, budget += h*c_new - h*c_old per cell
-// (apply_factored_tracer, rdb_ocean_vdiff.F90:1351-1414).
-__device__ void apply_column(const StateView& s, double* __restrict__ x, int p, double* budget = nullptr,
-                             double* pold = nullptr) {
-  const int P = s.P, nz = s.nz;
-  if (budget)
-    for (int k = 0; k < nz; ++k) LV(pold, k) = LV(s.h, k) * LV(x, k);
-  double r = LV(x, 0) / LV(s.fb, 0);
-  LV(x, 0) = r;
-  for (int k = 1; k < nz; ++k) {
-    r = (LV(x, k) - LV(s.fa, k) * r) / LV(s.fb, k);
-    LV(x, k) = r;
-  }
+  const double h_top = LV(s.h, nz - 1);
+#pragma unroll
+  for (int t = 0; t < G; ++t)
+    if (bud[t]) LV(bud[t], nz - 1) = LV(bud[t], nz - 1) + (h_top * r[t] - LV(pold[t], nz - 1));
   for (int k = nz - 2; k >= 0; --k) {
-    r = LV(x, k) - LV(s.fc, k) * r;
-    LV(x, k) = r;
+    const double cp = LV(s.fc, k);
+    const double hk = LV(s.h, k);
+#pragma unroll
+    for (int t = 0; t < G; ++t) {
+      r[t] = LV(xs[t], k) - cp * r[t];
+      LV(xs[t], k) = r[t];
+      if (bud[t]) LV(bud[t], k) = LV(bud[t], k) + (hk * r[t] - LV(pold[t], k));
+    }
   }
-  if (budget)
-    for (int k = 0; k < nz; ++k) LV(budget, k) = LV(budget, k) + (LV(s.h, k) * LV(x, k) - LV(pold, k));
 }
 
-// A11 two-pass dispatch: kt -> T; ks -> S and passives (rdb_ocean_vdiff.F90:1107-1226).
+// A11 two-pass dispatch: kt -> T; ks -> S and passives (rdb_ocean_vdiff.F90:
+// 1107-1226), the ks tracers in groups of up to four.
 __global__ void k_vdiff(StateView s, Params pr, double dt, int p0, int p1) {
   int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= p1) return;
-  factorize_column(s, s.kt, dt, pr.use_harmonic, p);
-  apply_column(s, s.T, p, s.budT, s.pold);
-  factorize_column(s, s.ks, dt, pr.use_harmonic, p);
-  apply_column(s, s.S, p, s.budS, s.pold);
-  for (int t = 0; t < s.n_passive; ++t) apply_column(s, s.tr + (size_t)t * (size_t)s.nz * (size_t)s.P, p);
+  const size_t L = (size_t)s.nz * (size_t)s.P;
+  {
+    double* const xs[1] = {s.T};
+    double* const bud[1] = {s.budT};
+    double* const pold[1] = {s.pold};
+    solve_group<1>(s, s.kt, dt, pr.use_harmonic, p, xs, bud, pold);
+  }
+  // S first, then the passives, four tracers a pass
+  const int n = 1 + s.n_passive;
+  for (int g0 = 0; g0 < n; g0 += 4) {
+    auto tracer = [&](int t) { return t == 0 ? s.S : s.tr + (size_t)(t - 1) * L; };
+    double* const b0 = g0 == 0 ? s.budS : nullptr;
+    double* const q0 = s.pold ? s.pold + L : nullptr;
+    switch (n - g0 >= 4 ? 4 : n - g0) {
+      case 1: {
+        double* const xs[1] = {tracer(g0)};
+        double* const bud[1] = {b0};
+        double* const pold[1] = {q0};
+        solve_group<1>(s, s.ks, dt, pr.use_harmonic, p, xs, bud, pold);
+        break;
+      }
+      case 2: {
+        double* const xs[2] = {tracer(g0), tracer(g0 + 1)};
+        double* const bud[2] = {b0, nullptr};
+        double* const pold[2] = {q0, nullptr};
+        solve_group<2>(s, s.ks, dt, pr.use_harmonic, p, xs, bud, pold);
+        break;
+      }
+      case 3: {
+        double* const xs[3] = {tracer(g0), tracer(g0 + 1), tracer(g0 + 2)};
+        double* const bud[3] = {b0, nullptr, nullptr};
+        double* const pold[3] = {q0, nullptr, nullptr};
+        solve_group<3>(s, s.ks, dt, pr.use_harmonic, p, xs, bud, pold);
+        break;
+      }
+      default: {
+        double* const xs[4] = {tracer(g0), tracer(g0 + 1), tracer(g0 + 2), tracer(g0 + 3)};
+        double* const bud[4] = {b0, nullptr, nullptr, nullptr};
+        double* const pold[4] = {q0, nullptr, nullptr, nullptr};
+        solve_group<4>(s, s.ks, dt, pr.use_harmonic, p, xs, bud, pold);
+        break;
+      }
+    }
+  }
 }
 
 // A14 non-local tendencies on T (gamma_t) and S (gamma_s)

@@ -6,11 +6,11 @@
 //              2018), epbl_column_kernel (rdb_ocean_epbl.F90:931-1659): MLD
 //              root-find around a downward interface sweep with the
 //              closed-form energy solve and the embedded forward elimination;
-//   k_contrib  the EPBL merge into kv/kt (:1661-1688), KV_ML_INVZ2
-//              (rdb_ocean_vmix.F90:1716-1748) and convective adjustment
-//              (:1768-1799) in one top-down pass;
-//   ddiff_iface (called from k_split_assemble) double diffusion inside the
-//              heat/salt split (:1936-2028).
+//   k_coeff_tail  the rest of the coefficient build in one top-down pass:
+//              PP81 (when EPBL runs), the EPBL merge into kv/kt (:1661-1688),
+//              KV_ML_INVZ2 (rdb_ocean_vmix.F90:1716-1748), convective
+//              adjustment (:1768-1799), the heat/salt split with double
+//              diffusion (ddiff_iface, :1936-2028) and vmix_assemble.
 //
 // The EPBL sweep needs, per level, the pre-mixing T/S and the pressure-
 // weighted PE and column-height sensitivities.  The reference fills six
@@ -18,6 +18,8 @@
 // each iteration recomputes them on the fly while it walks down (the same
 // pressure stack, summed in the same order, so the same bits): every
 // iteration streams h, T and S only, and no work array is written.
+#include <type_traits>
+
 #include "colphys_kernels.h"
 #include "safe_math.h"
 
@@ -50,7 +52,7 @@ __device__ __forceinline__ void specvol_derivs(double T, double S, double p, dou
 __device__ __forceinline__ double cube(double x) { return (x * x) * x; }
 
 // epbl_find_mstar (rdb_ocean_epbl.F90:630-688).
-__device__ __noinline__ double find_mstar(const EpblParams& c, double b0, double ustar, double bld, double absf) {
+__device__ __forceinline__ double find_mstar(const EpblParams& c, double b0, double ustar, double bld, double absf) {
   double mstar;
   const double absf_floor = fmx(absf, 1.0e-20);
   if (c.mstar_scheme == RDB_EPBL_MSTAR_OM4) {
@@ -206,7 +208,7 @@ __device__ __forceinline__ LevelPrep level_prep(double hk, double T, double S, d
 
 // The EPBL solve for the packed columns [p0, p1): kd_int, MLD, B0, La and
 // the TKE ledger of each (free surface, no penetrating shortwave).
-__global__ void __launch_bounds__(256) k_epbl(StateView s, EpblParams c, EpblView e, const double* __restrict__ fcor,
+__global__ void __launch_bounds__(64, 2) k_epbl(StateView s, EpblParams c, EpblView e, const double* __restrict__ fcor,
                                                double rho0, double cp, double dt, int p0, int p1) {
   const int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= p1) return;
@@ -255,7 +257,14 @@ __global__ void __launch_bounds__(256) k_epbl(StateView s, EpblParams c, EpblVie
     const int n_its = c.mld_iteration ? c.mld_max_its : 1;
     for (int k = k_sfc + 1; k <= nz; ++k) LV(e.kd, k) = 0.0;
     LV(e.kd, 0) = 0.0;
-    for (int obl_it = 1; obl_it <= n_its; ++obl_it) {
+    // One sweep of the column for the guess mld_guess.  A guess that is not
+    // the last one only needs the MLD, which stops growing once the sweep
+    // disconnects from the surface: those sweeps (FULL = false) stop there
+    // and write nothing.  The last guess is swept in full (kd_int and the
+    // TKE ledger); both kinds run the same arithmetic down to the
+    // disconnect, so the MLD they find is the same to the bit.
+    auto sweep = [&](auto full_tag, double mld_guess) -> double {
+      constexpr bool FULL = decltype(full_tag)::value;
       d_wind = d_conv = d_forcing = d_mixing = d_mdecay = d_cdecay = 0.0;
       double mstar_val = find_mstar(c, b0, ustar, mld_guess, absf);
       if (c.use_lt) {
@@ -294,9 +303,13 @@ __global__ void __launch_bounds__(256) k_epbl(StateView s, EpblParams c, EpblVie
         h_ka = hA + H_NEGLECT;
         const double h_kb = hB + H_NEGLECT;
         bool sfc_disconnect = false;
-        const double exp_kh = sm::exp(-h_ka * idecay);
-        d_mdecay = d_mdecay + (1.0 - exp_kh) * mech_tke;
-        mech_tke = mech_tke * exp_kh;
+        // with no mechanical TKE left the decay changes nothing:
+        // (1 - e) * 0 and 0 * e are 0 for the finite e in (0, 1]
+        if (mech_tke != 0.0) {
+          const double exp_kh = sm::exp(-h_ka * idecay);
+          d_mdecay = d_mdecay + (1.0 - exp_kh) * mech_tke;
+          mech_tke = mech_tke * exp_kh;
+        }
         double nstar_fc = nstar;
         if (conv_perel > 0.0 && absf > 0.0) {
           const double ah = absf * htot;
@@ -365,7 +378,8 @@ __global__ void __launch_bounds__(256) k_epbl(StateView s, EpblParams c, EpblVie
             conv_perel = r_reduc * conv_perel;
           }
         }
-        LV(e.kd, ki) = kd_val;
+        if (!FULL && sfc_disconnect) return mld_output;  // the MLD is final
+        if (FULL) LV(e.kd, ki) = kd_val;
         const double kddt_cur = kd_val * dt_h;
         const double b1 = 1.0 / (hp_a + kddt_cur);
         const double c1 = kddt_cur * b1;
@@ -399,10 +413,19 @@ __global__ void __launch_bounds__(256) k_epbl(StateView s, EpblParams c, EpblVie
       }
       d_mdecay = d_mdecay + mech_tke;
       d_cdecay = d_cdecay + nstar * conv_perel;
-      mld_found = mld_output;
-      if (!c.mld_iteration) break;
-      if (fabs(mld_found - mld_guess) < c.mld_tol) break;
-      if (obl_it == n_its) break;
+      return mld_output;
+    };
+    for (int obl_it = 1; obl_it <= n_its; ++obl_it) {
+      // a guess known to be the last is swept in full at once
+      if (!c.mld_iteration || obl_it == n_its) {
+        mld_found = sweep(std::true_type{}, mld_guess);
+        break;
+      }
+      mld_found = sweep(std::false_type{}, mld_guess);
+      if (fabs(mld_found - mld_guess) < c.mld_tol) {
+        sweep(std::true_type{}, mld_guess);
+        break;
+      }
       if (mld_found > mld_guess) {
         min_mld = mld_guess;
         dmld_min = mld_found - mld_guess;
@@ -434,11 +457,15 @@ __global__ void __launch_bounds__(256) k_epbl(StateView s, EpblParams c, EpblVie
   }
 }
 
-// The EPBL merge, KV_ML_INVZ2 and convective adjustment for the packed
-// columns [p0, p1), interfaces nz-1 .. 1 from the top.  The three are
-// per-interface updates applied in the reference's order (merge, invz2,
-// convection), so one pass gives the result of the three sweeps.
-__global__ void k_contrib(StateView s, ContribParams c, EpblView e, int p0, int p1) {
+// The coefficient build after the surface scheme, one pass per column from
+// the top: PP81 at each interface (when the surface scheme did not need it
+// first, i.e. with EPBL), the EPBL merge, KV_ML_INVZ2, convective
+// adjustment, then the heat/salt split with double diffusion, the guard and
+// the floors/ceilings of vmix_assemble (rdb_ocean_vmix.F90:2265-2385).  Each
+// is a per-interface update applied in the reference's order, so one pass
+// gives the bits of the separate sweeps; kv, kt and ks are written once.
+__global__ void k_coeff_tail(StateView s, Params pr, ContribParams c, EpblView e, int do_pp81, int p0, int p1,
+                             unsigned int* guard_count) {
   const int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= p1) return;
   const int P = s.P, nz = s.nz;
@@ -447,10 +474,45 @@ __global__ void k_contrib(StateView s, ContribParams c, EpblView e, int p0, int 
   const double hmix_sq = c.hmix * c.hmix;
   const double dz_min_safe = 0.5 * c.hmix / (double)(nz > 1 ? nz : 1);
   bool inv_done = !invz2;
+  unsigned int bad = 0;
+  auto finish = [&](int k, double kv, double kt, double kd_t, double kd_s) {
+    double ks = kt;
+    if (pr.ddiff) {
+      ks = kt + kd_s;
+      kt = kt + kd_t;
+    }
+    if (pr.guard && (isnan(kv) || isnan(kt) || isnan(ks) || kv < 0.0 || kt < 0.0 || ks < 0.0)) ++bad;
+    if (k >= 1 && k < nz) {
+      kv = fmn(fmx(kv, pr.kv_bg), pr.kv_max);
+      kt = fmn(fmx(kt, pr.kt_bg), pr.kd_max);
+      ks = fmn(fmx(ks, pr.ks_bg), pr.kd_max);
+    }
+    LV(s.kv, k) = kv;
+    LV(s.kt, k) = kt;
+    LV(s.ks, k) = ks;
+  };
+  if (do_pp81) {
+    finish(nz, 0.0, 0.0, 0.0, 0.0);
+  } else {
+    finish(nz, LV(s.kv, nz), LV(s.kt, nz), 0.0, 0.0);
+  }
   double z = 0.0;
-  double h_up = LV(s.h, nz - 1), rho_up = c.conv ? LV(s.rho, nz - 1) : 0.0;
+  double h_up = LV(s.h, nz - 1), rho_up = LV(s.rho, nz - 1);
+  double ub_up = do_pp81 ? LV(s.ub, nz - 1) : 0.0, vb_up = do_pp81 ? LV(s.vb, nz - 1) : 0.0;
+  double T_up = LV(s.T, nz - 1), S_up = LV(s.S, nz - 1);
   for (int k = nz - 1; k >= 1; --k) {
-    double kv = LV(s.kv, k), kt = LV(s.kt, k);
+    const double h_dn = LV(s.h, k - 1), rho_dn = LV(s.rho, k - 1);
+    const double T_dn = LV(s.T, k - 1), S_dn = LV(s.S, k - 1);
+    double ub_dn = 0.0, vb_dn = 0.0;
+    double kv, kt;
+    if (do_pp81) {
+      ub_dn = LV(s.ub, k - 1);
+      vb_dn = LV(s.vb, k - 1);
+      pp81_iface<double>(pr, ub_dn, vb_dn, ub_up, vb_up, h_dn, h_up, rho_up - rho_dn, kv, kt);
+    } else {
+      kv = LV(s.kv, k);
+      kt = LV(s.kt, k);
+    }
     if (c.epbl) {
       const double kd = LV(e.kd, k);
       if (c.combine == RDB_EPBL_COMBINE_MAX) {
@@ -462,7 +524,6 @@ __global__ void k_contrib(StateView s, ContribParams c, EpblView e, int p0, int 
       }
     }
     z = z + h_up;
-    const double h_dn = LV(s.h, k - 1);
     if (!inv_done) {
       if (z >= c.hmix) {
         inv_done = true;
@@ -471,9 +532,7 @@ __global__ void k_contrib(StateView s, ContribParams c, EpblView e, int p0, int 
         kv = kv + c.kv_ml_invz2 * hmix_sq / (m * m);
       }
     }
-    double rho_dn = 0.0;
     if (c.conv) {
-      rho_dn = LV(s.rho, k - 1);
       const double dz_face = 0.5 * (h_dn + h_up);
       if (dz_face > H_VANISHED && !(z < bld)) {
         const double n2 = -(GRAVITY * (rho_up - rho_dn) / (c.rho0 * dz_face));
@@ -483,11 +542,21 @@ __global__ void k_contrib(StateView s, ContribParams c, EpblView e, int p0, int 
         }
       }
     }
-    LV(s.kv, k) = kv;
-    LV(s.kt, k) = kt;
+    double kd_t = 0.0, kd_s = 0.0;
+    if (pr.ddiff) ddiff_iface(pr, h_up, h_dn, T_up, T_dn, S_up, S_dn, kd_t, kd_s);
+    finish(k, kv, kt, kd_t, kd_s);
     h_up = h_dn;
     rho_up = rho_dn;
+    ub_up = ub_dn;
+    vb_up = vb_dn;
+    T_up = T_dn;
+    S_up = S_dn;
   }
+  if (do_pp81)
+    finish(0, 0.0, 0.0, 0.0, 0.0);
+  else
+    finish(0, LV(s.kv, 0), LV(s.kt, 0), 0.0, 0.0);
+  if (bad) atomicAdd(guard_count, bad);
 }
 
 // The device safe math over n arguments (fn: 0 exp, 1 log, 2 pow, 3 erfc),

@@ -67,7 +67,68 @@ def test_config_from_namelist(gpu_lib):
     assert st == 0 and bytes(c0) == bytes(d)
 
 
+CONTRIB_NML = """
+&ocean_vmix_nml use_kpp = .false., kv_ml_invz2 = 1.0e-2, hmix_fixed = 30.0 /
+&ocean_conv_nml enable = .true., kd_conv = 0.5, prandtl_conv = 2.0, n2_thresh = 1.0e-7 /
+&ocean_ddiff_nml enable = T, strat_param_max = 2.0, kappa_ddiff_s = 2.0e-4, ddiff_exp1 = 1.5,
+   ddiff_exp2 = 2.0, param1 = 0.8, param2 = 4.0, param3 = -0.5, mol_diff = 1.4e-6, use_k90 = .true. /
+&ocean_epbl_nml enable = .true., mstar_scheme = 'rh18', vel_scale_scheme = "rh18", combine = 'max',
+   mstar = 1.1, mstar_cap = 2.0, mstar_coef1 = 0.25, c_ek = 0.08, mstar_conv_adj = 0.3,
+   rh18_cn1 = 0.3, rh18_cn2 = 7.0, rh18_cn3 = -4.0, rh18_cs1 = 0.25, rh18_cs2 = 0.35,
+   nstar = 0.15, tke_decay = 2.0, wstar_ustar_coef = 0.9, vstar_scale_fac = 1.1, vstar_surf_fac = 1.3,
+   von_karman = 0.4, ekman_scale_coef = 0.9, min_mix_len = 0.5, mixlen_exponent = 1.5,
+   translay_scale = 0.2, mld_iteration = .true., mld_tol = 0.5, mld_max_its = 12, mld_bisection = .true.,
+   mld_use_prev_guess = .true., omega = 7.3e-5, omega_frac = 0.1, prandtl = 0.8, tke_diags = .true.,
+   use_lt = .true., lt_scheme = 'additive', lt_enhance_coef = 0.4, lt_enhance_exp = -1.2,
+   lt_max_enhance = 4.0, la_frac_hbl = 0.05, lt_lac1 = -0.8, lt_lac2 = 0.1, lt_lac3 = 0.2, lt_lac4 = 0.9,
+   lt_lac5 = 0.85 /
+"""
+
+
+def test_contributor_groups(gpu_lib):
+    # &ocean_conv_nml / &ocean_ddiff_nml / &ocean_epbl_nml
+    # (rdb_config_register_vmix.F90:226-393) and the vmix invz2 knobs
+    st, c = _cfg(gpu_lib, CONTRIB_NML)
+    assert st == 0, _ffi.last_error(gpu_lib)
+    assert (c.kv_ml_invz2, c.hmix_fixed) == (1.0e-2, 30.0)
+    assert (c.conv_enable, c.conv_kd, c.conv_prandtl, c.conv_n2_thresh) == (1, 0.5, 2.0, 1.0e-7)
+    assert (c.ddiff_enable, c.ddiff_strat_param_max, c.ddiff_kappa_s, c.ddiff_exp1, c.ddiff_exp2) == \
+        (1, 2.0, 2.0e-4, 1.5, 2.0)
+    assert (c.ddiff_param1, c.ddiff_param2, c.ddiff_param3, c.ddiff_mol_diff, c.ddiff_use_k90) == \
+        (0.8, 4.0, -0.5, 1.4e-6, 1)
+    assert (c.epbl_enable, c.epbl_mstar_scheme, c.epbl_vstar_scheme, c.epbl_combine) == \
+        (1, _ffi.EPBL_MSTAR_RH18, _ffi.EPBL_VSTAR_RH18, _ffi.EPBL_COMBINE_MAX)
+    want = dict(epbl_mstar=1.1, epbl_mstar_cap=2.0, epbl_mstar_coef1=0.25, epbl_c_ek=0.08, epbl_mstar_conv_adj=0.3,
+                epbl_rh18_cn1=0.3, epbl_rh18_cn2=7.0, epbl_rh18_cn3=-4.0, epbl_rh18_cs1=0.25, epbl_rh18_cs2=0.35,
+                epbl_nstar=0.15, epbl_tke_decay=2.0, epbl_wstar_ustar_coef=0.9, epbl_vstar_scale_fac=1.1,
+                epbl_vstar_surf_fac=1.3, epbl_von_karman=0.4, epbl_ekman_scale_coef=0.9, epbl_min_mix_len=0.5,
+                epbl_mixlen_exponent=1.5, epbl_translay_scale=0.2, epbl_mld_iteration=1, epbl_mld_tol=0.5,
+                epbl_mld_max_its=12, epbl_mld_bisection=1, epbl_mld_use_prev_guess=1, epbl_omega=7.3e-5,
+                epbl_omega_frac=0.1, epbl_prandtl=0.8, epbl_tke_diags=1, epbl_use_lt=1,
+                epbl_lt_scheme=_ffi.EPBL_LT_ADDITIVE, epbl_lt_enhance_coef=0.4, epbl_lt_enhance_exp=-1.2,
+                epbl_lt_max_enhance=4.0, epbl_la_frac_hbl=0.05, epbl_lt_lac1=-0.8, epbl_lt_lac2=0.1,
+                epbl_lt_lac3=0.2, epbl_lt_lac4=0.9, epbl_lt_lac5=0.85)
+    for k, v in want.items():
+        assert getattr(c, k) == v, k
+    d = _ffi.default_config()
+    assert c.epbl_ustar_min == d.epbl_ustar_min  # not a namelist knob (configure keeps the default)
+    # the parsers' aliases (rdb_ocean_epbl.F90:570-624)
+    for text, field, tag in (("mstar_scheme = 'OM4'", "epbl_mstar_scheme", _ffi.EPBL_MSTAR_OM4),
+                             ("mstar_scheme = 'reichl_h18'", "epbl_mstar_scheme", _ffi.EPBL_MSTAR_RH18),
+                             ("vel_scale_scheme = 'cube_root_tke'", "epbl_vstar_scheme", _ffi.EPBL_VSTAR_CUBE_ROOT),
+                             ("combine = 'additive'", "epbl_combine", _ffi.EPBL_COMBINE_ADD),
+                             ("combine = 'max_shear'", "epbl_combine", _ffi.EPBL_COMBINE_MAX_SHEAR),
+                             ("lt_scheme = 'add'", "epbl_lt_scheme", _ffi.EPBL_LT_ADDITIVE)):
+        st, c = _cfg(gpu_lib, f"&ocean_epbl_nml {text} /")
+        assert st == 0 and getattr(c, field) == tag, text
+
+
 @pytest.mark.parametrize("text,code", [
+    ("&ocean_epbl_nml mstar_scheme = 'kpp' /", _ffi.ERR_CONFIG_VALIDATE),
+    ("&ocean_epbl_nml combine = 'min' /", _ffi.ERR_CONFIG_VALIDATE),
+    ("&ocean_epbl_nml no_such_knob = 1 /", _ffi.ERR_CONFIG_PARSE),
+    ("&ocean_conv_nml kd_conv = x /", _ffi.ERR_CONFIG_PARSE),
+    ("&ocean_ddiff_nml use_k90 = 3 /", _ffi.ERR_CONFIG_PARSE),
     ("&ocean_vmix_nml use_closure = .false. /", _ffi.ERR_CONFIG_VALIDATE),
     ("&ocean_vmix_nml interior_closure = 'none' /", _ffi.ERR_CONFIG_VALIDATE),
     ("&ocean_vmix_nml bkgnd_profile = .true. /", _ffi.ERR_CONFIG_VALIDATE),
