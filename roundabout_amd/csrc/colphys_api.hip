@@ -1229,7 +1229,7 @@ int rdb_colphys_diffuse_tracers(rdb_colphys* h, double dt) {
                       "and stepped with rdb_colphys_step");
   ++h->state_ver;
   hipLaunchKernelGGL(colphys::k_vdiff, grid1(h->n_all, h->block), dim3(h->block), 0, h->stream, h->s, h->pr, dt, 0,
-                     h->n_all);
+                     h->n_all, 0);
   return launch_check("k_vdiff");
 }
 
@@ -1275,7 +1275,7 @@ int rdb_colphys_step(rdb_colphys* h, double dt, int n_steps) {
         return st;
       if (h->n_wet > 0) {
         hipLaunchKernelGGL(colphys::k_vdiff, grid1(h->n_wet, h->block), dim3(h->block), 0, h->stream, h->s, h->pr,
-                           dt, 0, h->n_wet);
+                           dt, 0, h->n_wet, 1);
         if ((st = launch_check("k_vdiff"))) return st;
       }
       if ((st = rdb_colphys_nonlocal(h, dt))) return st;

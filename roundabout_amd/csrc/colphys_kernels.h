@@ -60,7 +60,7 @@ __global__ void k_pp81(StateView s, Params pr, int p0, int p1);
 __global__ void k_kpp(StateView s, Params pr, int p0, int p1);
 __global__ void k_split_assemble(StateView s, Params pr, int p0, int p1, unsigned int* guard_count);
 __global__ void k_surface_flux(StateView s, Params pr, double dt, int p0, int p1);
-__global__ void k_vdiff(StateView s, Params pr, double dt, int p0, int p1);
+__global__ void k_vdiff(StateView s, Params pr, double dt, int p0, int p1, int from_kb);
 __global__ void k_nonlocal(StateView s, double dt, int p0, int p1);
 // Learned-closure inputs of the fused step: the network's outputs stored
 // output-major (Yt[n * ldy + p], p the packed wet column), their

@@ -281,15 +281,22 @@ __global__ void k_surface_flux(StateView s, Params pr, double dt, int p0, int p1
 // written once per pass; the arithmetic is the reference's, operation for
 // operation (piv = b - a c'_{k-1}, c' = c / piv, r = (x - a r_{k-1}) / piv,
 // x = r - c' x_{k+1}).  With a budget slot, budget += h*c_new - h*c_old.
+// With kb > 0 the elimination starts at the first live layer: a bed filler
+// row is the identity (alpha = beta = 0: a = -0, b = 1, c = 0), so the
+// reference's elimination through the fillers leaves their values and the
+// first live row's pivot, c' and residual as they are when it starts there
+// (c'_{kb-1} = 0 and r_{kb-1} enter only through a * 0).
 template <int G>
 __device__ __forceinline__ void solve_group(const StateView& s, const double* __restrict__ K, double dt,
-                                            int use_harmonic, int p, double* const (&xs)[G],
+                                            int use_harmonic, int p, int kb, double* const (&xs)[G],
                                             double* const (&bud)[G], double* const (&pold)[G]) {
   const int P = s.P, nz = s.nz;
   double r[G];
-  double h_km1 = 0.0, h_k = LV(s.h, 0), K_k = LV(K, 0);
+#pragma unroll
+  for (int t = 0; t < G; ++t) r[t] = 0.0;
+  double h_km1 = kb > 0 ? LV(s.h, kb - 1) : 0.0, h_k = LV(s.h, kb), K_k = LV(K, kb);
   double cprev = 0.0;
-  for (int k = 0; k < nz; ++k) {
+  for (int k = kb; k < nz; ++k) {
     const double h_kp1 = (k + 1 < nz) ? LV(s.h, k + 1) : 0.0;
     const double K_kp1 = LV(K, k + 1);
     double a, b, c;
@@ -318,7 +325,7 @@ __device__ __forceinline__ void solve_group(const StateView& s, const double* __
 #pragma unroll
   for (int t = 0; t < G; ++t)
     if (bud[t]) LV(bud[t], nz - 1) = LV(bud[t], nz - 1) + (h_top * r[t] - LV(pold[t], nz - 1));
-  for (int k = nz - 2; k >= 0; --k) {
+  for (int k = nz - 2; k >= kb; --k) {
     const double cp = LV(s.fc, k);
     const double hk = LV(s.h, k);
 #pragma unroll
@@ -331,16 +338,18 @@ __device__ __forceinline__ void solve_group(const StateView& s, const double* __
 }
 
 // A11 two-pass dispatch: kt -> T; ks -> S and passives (rdb_ocean_vdiff.F90:
-// 1107-1226), the ks tracers in groups of up to four.
-__global__ void k_vdiff(StateView s, Params pr, double dt, int p0, int p1) {
+// 1107-1226), the ks tracers in groups of up to four.  from_kb: start at the
+// first live layer s.kb[p] (current only after the static refresh).
+__global__ void k_vdiff(StateView s, Params pr, double dt, int p0, int p1, int from_kb) {
   int p = p0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= p1) return;
   const size_t L = (size_t)s.nz * (size_t)s.P;
+  const int kb = from_kb ? min(s.kb[p], s.nz - 1) : 0;
   {
     double* const xs[1] = {s.T};
     double* const bud[1] = {s.budT};
     double* const pold[1] = {s.pold};
-    solve_group<1>(s, s.kt, dt, pr.use_harmonic, p, xs, bud, pold);
+    solve_group<1>(s, s.kt, dt, pr.use_harmonic, p, kb, xs, bud, pold);
   }
   // S first, then the passives, four tracers a pass
   const int n = 1 + s.n_passive;
@@ -353,28 +362,28 @@ __global__ void k_vdiff(StateView s, Params pr, double dt, int p0, int p1) {
         double* const xs[1] = {tracer(g0)};
         double* const bud[1] = {b0};
         double* const pold[1] = {q0};
-        solve_group<1>(s, s.ks, dt, pr.use_harmonic, p, xs, bud, pold);
+        solve_group<1>(s, s.ks, dt, pr.use_harmonic, p, kb, xs, bud, pold);
         break;
       }
       case 2: {
         double* const xs[2] = {tracer(g0), tracer(g0 + 1)};
         double* const bud[2] = {b0, nullptr};
         double* const pold[2] = {q0, nullptr};
-        solve_group<2>(s, s.ks, dt, pr.use_harmonic, p, xs, bud, pold);
+        solve_group<2>(s, s.ks, dt, pr.use_harmonic, p, kb, xs, bud, pold);
         break;
       }
       case 3: {
         double* const xs[3] = {tracer(g0), tracer(g0 + 1), tracer(g0 + 2)};
         double* const bud[3] = {b0, nullptr, nullptr};
         double* const pold[3] = {q0, nullptr, nullptr};
-        solve_group<3>(s, s.ks, dt, pr.use_harmonic, p, xs, bud, pold);
+        solve_group<3>(s, s.ks, dt, pr.use_harmonic, p, kb, xs, bud, pold);
         break;
       }
       default: {
         double* const xs[4] = {tracer(g0), tracer(g0 + 1), tracer(g0 + 2), tracer(g0 + 3)};
         double* const bud[4] = {b0, nullptr, nullptr, nullptr};
         double* const pold[4] = {q0, nullptr, nullptr, nullptr};
-        solve_group<4>(s, s.ks, dt, pr.use_harmonic, p, xs, bud, pold);
+        solve_group<4>(s, s.ks, dt, pr.use_harmonic, p, kb, xs, bud, pold);
         break;
       }
     }
