@@ -182,6 +182,9 @@ __global__ void __launch_bounds__(256) k_vdiff_momentum(MomView v, MomFlags fl, 
       break;
     }
   }
+  // first row of the solve that is not the identity (a top row below the
+  // first live one takes the top forcing: then every row is solved)
+  const int k_lo = (ktop < kb_live) ? 0 : kb_live;
   auto kvat = [&](size_t c, int kf) -> double { return fl.scalar_kv ? fl.K : v.kv[c + C * kf]; };
   double hv_0, hv_p, zi_0, zi_p;
   m_face_layer(fl, v.h[cl], v.h[cr], v.u[f], 0, kb, d_min, i_hbbl, zacc, zcol_l, zcol_r, hv_p, zi_p);
@@ -251,9 +254,14 @@ __global__ void __launch_bounds__(256) k_vdiff_momentum(MomView v, MomFlags fl, 
       dp = (r_k - a_k * dp_prev) / denom;
       gp = (1.0 - a_k * gp_prev) / denom;
     }
-    v.cd[f + F * k] = cp;
-    if (fl.solve) v.rd[f + F * k] = dp;
-    if (fl.remnant) v.rem[f + F * k] = gp;
+    // rows below k_lo are the identity (cp = 0, dp = u, gp = 1): nothing of
+    // them is stored, and the back substitution stops at k_lo (u - 0 * u_next
+    // leaves u as it is, the remnant is 1)
+    if (k >= k_lo) {
+      v.cd[f + F * k] = cp;
+      if (fl.solve) v.rd[f + F * k] = dp;
+      if (fl.remnant) v.rem[f + F * k] = gp;
+    }
     cp_prev = cp;
     dp_prev = dp;
     gp_prev = gp;
@@ -262,7 +270,7 @@ __global__ void __launch_bounds__(256) k_vdiff_momentum(MomView v, MomFlags fl, 
   double g_next = mmin(gp_prev, 1.0);
   if (fl.solve) v.u[f + F * (nz - 1)] = u_next;
   if (fl.remnant) v.rem[f + F * (nz - 1)] = g_next;
-  for (int k = nz - 2; k >= 0; --k) {
+  for (int k = nz - 2; k >= k_lo; --k) {
     double cpk = v.cd[f + F * k];
     if (fl.solve) {
       u_next = v.rd[f + F * k] - cpk * u_next;
@@ -273,6 +281,8 @@ __global__ void __launch_bounds__(256) k_vdiff_momentum(MomView v, MomFlags fl, 
       v.rem[f + F * k] = g_next;
     }
   }
+  if (fl.remnant)
+    for (int k = min(k_lo, nz - 1) - 1; k >= 0; --k) v.rem[f + F * k] = 1.0;
 }
 
 inline dim3 mgrid(size_t n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
