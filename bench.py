@@ -317,22 +317,30 @@ def live_rms(ocean, a, b) -> float:
     return float(np.sqrt(np.mean(d * d)))
 
 
-def train_in_chunks(cp, epochs: int, guard: int, chunk: int = 240) -> tuple[list, list]:
-    """Train `epochs` epochs in calls of at most `chunk` epochs, logging after
-    each: a call keeps the network and its Adam moments, so the calls
-    continue one training (each call draws the same held-out split and
-    restarts the epoch shuffles from the configured seed)."""
+def parse_schedule(text: str) -> list[tuple[int, float]]:
+    """'180@1e-3,60@3e-4' -> [(180, 1e-3), (60, 3e-4)]: epochs at a learning rate, in order."""
+    out = []
+    for part in text.split(","):
+        n, lr = part.split("@")
+        out.append((int(n), float(lr)))
+    if not out or any(n < 1 or lr <= 0 for n, lr in out):
+        raise SystemExit(f"bad --lr-schedule {text!r}")
+    return out
+
+
+def train_schedule(cp, schedule: list[tuple[int, float]], guard: int) -> tuple[list, list]:
+    """Train through `schedule` (epochs at a learning rate, in order), one
+    mlp_train call per stage: a call keeps the network and its Adam moments,
+    so the stages continue one training (each call draws the same held-out
+    split and restarts the epoch shuffles from the configured seed)."""
     tl, vl, done = [], [], 0
-    while done < epochs or not tl:
-        n = min(chunk, epochs - done)
-        a, b = cp.mlp_train(epochs=n, guard_mode=guard)
+    total = sum(n for n, _ in schedule)
+    for n, lr in schedule:
+        a, b = cp.mlp_train(epochs=n, guard_mode=guard, lr=lr)
         tl += a
         vl += b
         done += n
-        if epochs > chunk:
-            log(f"[bench] surrogate: {done}/{epochs} epochs, val loss {vl[-1]:.5f}")
-        if n == 0:
-            break
+        log(f"[bench] surrogate: {done}/{total} epochs (lr {lr:g}), val loss {vl[-1]:.5f}")
     return tl, vl
 
 
@@ -346,6 +354,8 @@ def surrogate_leg(ocean, cfg, args, plain_value: float) -> dict:
 
     guard = {"strict": GUARD_STRICT, "surface": GUARD_SURFACE, "range": GUARD_RANGE}[args.guard]
     K = ACCURACY_STEPS  # the error horizon is fixed, whatever --steps times for the plain path
+    schedule = parse_schedule(args.lr_schedule)
+    epochs = sum(n for n, _ in schedule)
     net_path = Path(tempfile.gettempdir()) / f"rdb_bench_closure_{os.getpid()}.bin"
     t0 = time.perf_counter()
     with ColumnPhysics.from_ocean(synth.redraw(ocean, args.seed + 1000), config=cfg) as cp:
@@ -357,11 +367,11 @@ def surrogate_leg(ocean, cfg, args, plain_value: float) -> dict:
         cp.synchronize()
         t_col = time.perf_counter() - t0
         t1 = time.perf_counter()
-        tl, vl = train_in_chunks(cp, args.epochs, guard)
+        tl, vl = train_schedule(cp, schedule, guard)
         cp.synchronize()
         t_train = time.perf_counter() - t1
         cp.mlp_save(net_path)
-    log(f"[bench] surrogate: {n} training columns in {t_col:.1f} s, {args.epochs} epochs in {t_train:.1f} s, "
+    log(f"[bench] surrogate: {n} training columns in {t_col:.1f} s, {epochs} epochs in {t_train:.1f} s, "
         f"val loss {vl[0]:.4f} -> {vl[-1]:.4f}")
     with ColumnPhysics.from_ocean(ocean, config=cfg) as cp:
         cp.step(DT, K)
@@ -392,7 +402,7 @@ def surrogate_leg(ocean, cfg, args, plain_value: float) -> dict:
     # over the training columns and a forward over the held-out ones
     fwd_per_col = st["mlp_flop_per_step"] / max(st["wet_columns"], 1)
     n_val = round(0.05 * n)
-    train_tflops = args.epochs * fwd_per_col * (3 * (n - n_val) + n_val) / t_train / 1e12
+    train_tflops = epochs * fwd_per_col * (3 * (n - n_val) + n_val) / t_train / 1e12
     return {
         "value": value,
         "unit": "column-steps/s",
@@ -412,8 +422,9 @@ def surrogate_leg(ocean, cfg, args, plain_value: float) -> dict:
                      "source": f"{args.train_steps} plain steps of each of {args.train_states} "
                      f"independent state draws (seeds {args.seed + 1000}..{args.seed + 999 + args.train_states}) "
                      f"on the same grid, every {args.train_stride}. column of each step",
-                     "epochs": args.epochs, "batch": 8192,
-                     "lr": 1e-3, "seconds": t_train, "val_loss_first": vl[0], "val_loss_last": vl[-1]},
+                     "epochs": epochs, "batch": 8192,
+                     "lr_schedule": [{"epochs": n, "lr": lr} for n, lr in schedule], "seconds": t_train,
+                     "val_loss_first": vl[0], "val_loss_last": vl[-1]},
         "mlp": {"forward_ms": mlp_ms, "flop_per_step": st["mlp_flop_per_step"], "achieved_tflops": tflops,
                 "peak_tflops": PEAK_BF16_TFLOPS, "mfma_frac": (tflops / PEAK_BF16_TFLOPS) if tflops else None,
                 "widths": "381-256-256-256-256-149"},
@@ -488,11 +499,12 @@ def ensemble_bench(args) -> dict:
         del train_oceans
         n = ens.mlp_collect(DT, args.train_steps, stride=4)
         t2 = time.perf_counter()
-        tl, vl = ens.mlp_train(epochs=args.epochs, guard_mode=guard)
+        schedule = parse_schedule(args.lr_schedule)
+        tl, vl = train_schedule(ens, schedule, guard)
         ens.synchronize()
         t_train = time.perf_counter() - t2
         ens.mlp_save(net_path)
-    log(f"[bench] ensemble surrogate: {n} training columns, {args.epochs} epochs in {t_train:.1f} s, "
+    log(f"[bench] ensemble surrogate: {n} training columns, {sum(n for n, _ in schedule)} epochs in {t_train:.1f} s, "
         f"val loss {vl[0]:.4f} -> {vl[-1]:.4f}")
     with ColumnEnsemble.from_oceans(oceans, config=cfg) as ens:
         ens.mlp_load(net_path)
@@ -552,7 +564,7 @@ def ensemble_bench(args) -> dict:
                       "meets_error_bar": bool(t_rms <= 0.01 and s_rms <= 0.002),
                       "heat_rel_residual": abs((h1 - h0) - dep) / abs(h0), "salt_rel_residual": abs(s1 - s0) / abs(s0),
                       "fallback_fraction_last_step": st_s["fallback_fraction"], "guard": args.guard,
-                      "training": {"columns": n, "epochs": args.epochs, "val_loss_first": vl[0],
+                      "training": {"columns": n, "epochs": sum(e for e, _ in schedule), "val_loss_first": vl[0],
                                    "val_loss_last": vl[-1], "seconds": t_train},
                       "mlp": {"forward_ms": st_s["mlp_forward_ms"], "flop_per_step": st_s["mlp_flop_per_step"]}},
     }
@@ -604,7 +616,8 @@ def main() -> int:
     ap.add_argument("--ensemble", action="store_true",
                     help="the 16-member batched ensemble (about 10.5 M columns) instead of the single tile")
     ap.add_argument("--guard", default="surface", choices=["strict", "surface", "range"])
-    ap.add_argument("--epochs", type=int, default=240)
+    ap.add_argument("--lr-schedule", default="180@1e-3,60@3e-4,60@1e-4,60@3e-5",
+                    help="learned-closure training: comma-separated EPOCHS@LR stages, run in order")
     ap.add_argument("--train-steps", type=int, default=2, help="plain steps recorded per training state")
     ap.add_argument("--train-states", type=int, default=6, help="independent state draws the network trains on")
     ap.add_argument("--train-stride", type=int, default=1, help="record every n-th column of each training step")
