@@ -16,16 +16,14 @@ constexpr int FP_MAXL = 8;
 constexpr int FP_MAXK = 384;
 constexpr int FP_MAXN = 256;  // widest layer output
 constexpr int FP_WAVES = 8;   // wavefronts per workgroup
-// LDS row pitches (bf16) of the activation tile's two regions: A holds
-// columns 0..255, B the first layer's input columns 256..383; rows start 4
-// banks apart
+// LDS row pitches (bf16) of the activation buffers: rows start 4 banks
+// apart.  FA_PITCH: a layer's input or output (up to 256 columns); FB_PITCH:
+// the first layer's input columns 256..383, staged into the second buffer
 constexpr int FA_PITCH = 256 + 8;
 constexpr int FB_PITCH = 128 + 8;
-// dynamic LDS of k_mlp_fused (the activation tile; region B must also hold
-// two weight slices of FP_MAXN x 32 bf16); three slices (48 KB) are static
-// LDS on top
-constexpr size_t FP_LDS_BYTES = (size_t)128 * (FA_PITCH + FB_PITCH) * 2;
-static_assert(128 * FB_PITCH >= 2 * FP_MAXN * 32, "region B holds two weight slices");
+static_assert(FB_PITCH <= FA_PITCH, "the first layer's columns 256.. fit the second buffer");
+// dynamic LDS of k_mlp_fused: two activation buffers of 128 rows
+constexpr size_t FP_LDS_BYTES = (size_t)2 * 128 * FA_PITCH * 2;
 struct FusedNet {
   const bf16* W[FP_MAXL];
   const float* b[FP_MAXL];

@@ -354,210 +354,189 @@ template __global__ void k_gemm_bf16<EPI_MSE>(GemmArgs);
 template __global__ void k_gemm_bf16<EPI_F32>(GemmArgs);
 template __global__ void k_gemm_pair<EPI_F32, EPI_GELU_BWD>(GemmArgs, GemmArgs);
 
-// Eight 16-byte LDS reads (MFMA fragments) and the wait for them, as one
-// asm statement (for the same reason as lds_read4).
-__device__ __forceinline__ void lds_read8(const bf16* const (&p)[8], bf16x8 (&x)[8]) {
-  typedef __attribute__((ext_vector_type(4))) unsigned u32x4;
-  auto lds = [](const bf16* a) {
-    return (unsigned)(uintptr_t)(const __attribute__((address_space(3))) bf16*)a;
-  };
-  u32x4 v[8];
-  asm volatile(
-      "ds_read_b128 %0, %8\n\t"
-      "ds_read_b128 %1, %9\n\t"
-      "ds_read_b128 %2, %10\n\t"
-      "ds_read_b128 %3, %11\n\t"
-      "ds_read_b128 %4, %12\n\t"
-      "ds_read_b128 %5, %13\n\t"
-      "ds_read_b128 %6, %14\n\t"
-      "ds_read_b128 %7, %15\n\t"
-      "s_waitcnt lgkmcnt(0)"
-      : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]), "=&v"(v[6]), "=&v"(v[7])
-      : "v"(lds(p[0])), "v"(lds(p[1])), "v"(lds(p[2])), "v"(lds(p[3])), "v"(lds(p[4])), "v"(lds(p[5])),
-        "v"(lds(p[6])), "v"(lds(p[7]))
-      : "memory");
-#pragma unroll
-  for (int i = 0; i < 8; ++i) x[i] = __builtin_bit_cast(bf16x8, v[i]);
-}
-
-// One layer of the fused kernel for a wave owning rows [row0, row0 + 64) of
-// the block and NTW 16-wide output tiles from column nw0:
-// acc[i][j] (rows row0 + 16 i.., columns nw0 + 16 j..) = act * W^T.
+// ---------------------------------------------------------------------------
+// Fused inference: the whole network on a block of 128 rows per workgroup.
 //
-// The layer input is the block's LDS tile: columns 0..255 in region A (row
-// pitch FA_PITCH), columns 256.. in region B (pitch FB_PITCH).  W streams
-// through LDS in k-slices of 32 (N rows x 64 B) in NB buffers, NB - 1
-// slices ahead of the MFMAs that use them, with the direct global-to-LDS
-// load (global_load_lds_dwordx4: lane l of a wave writes its 16 bytes at the
-// instruction's LDS base + 16 l).  Within a slice, the 16-byte chunk q of
-// row n sits at slot 4 n + ((q + n / 4) mod 4): the 16 rows of one MFMA
-// operand then start in 16 different bank groups.  Each k-step issues the
-// slice NB - 1 ahead, reads all fragments (one asm statement), runs the MFMAs,
-// then waits for the next slice only and meets the other waves at a bare
-// s_barrier.  NTW and NB are compile-time constants; the loop is unrolled
-// by the buffer rotation.
-// This wave's share of the k-slice [k0, k0 + 32) of W (N x K) into an LDS
-// slice buffer: N / 16 row groups of 16 rows (1 KB of LDS, one instruction
-// each), group g by wave g mod FP_WAVES; lane l loads row g * 16 + l / 4
-// into LDS slot l % 4, which holds global chunk (l % 4 - l / 16) mod 4.
-__device__ __forceinline__ void load_w_slice(const bf16* __restrict__ W, int K, int N, int wave, int lane, int k0,
-                                             bf16* dst) {
-  const int lrow = lane >> 2;
-  const int lchunk = ((lane & 3) - (lane >> 4) + 4) & 3;
-  for (int g = wave; g < N / 16; g += FP_WAVES) {
-    const bf16* src = W + (size_t)(g * 16 + lrow) * (size_t)K + k0 + lchunk * 8;
-    __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)(dst + g * 512), 16,
-                                     0, 0);
-  }
-}
+// The block's activations stay in LDS, in two buffers of 128 x FA_PITCH
+// bf16 (buf0, then buf1).  The first layer's input is staged into buf0
+// (columns 0..255) and, for columns 256.., into buf1 at row pitch FB_PITCH.
+//
+// Each layer computes out^T = W act^T: W is the MFMA's A operand (16 output
+// features x 32 k, one 16-byte global load per lane) and the activations its
+// B operand (16 rows x 32 k, one 16-byte LDS read per lane), so a result
+// tile holds, per lane, four consecutive output features of one row -- one
+// 8-byte LDS write in the epilogue.  The 16-wide output tiles of a layer go
+// to the waves round robin (wave w takes tiles w and w + 8) and every wave
+// covers all 128 rows: no weight row is used by two waves, so each wave
+// streams its own weight fragments from global memory (the 0.7 MB of
+// weights stay in L2, shared by every workgroup) through a ring of D
+// register slots, D k-steps ahead of the MFMAs, and the k-loop has no
+// barrier.  The refills of a layer's last D steps load the next layer's
+// first D steps.  Layer 0 reads buf0/buf1 and writes buf1 after a barrier
+// (its input covers part of buf1); layer l >= 1 reads one buffer and writes
+// the other, so one barrier per layer separates an epilogue's writes from
+// the next layer's reads.
+// ---------------------------------------------------------------------------
 
-// The first NB - 1 slices of a layer (issued before the previous layer's
-// epilogue, so that they arrive while it runs).
-template <int NB>
-__device__ __forceinline__ void fused_prefetch(const bf16* __restrict__ W, int K, int N, int wave, int lane,
-                                               bf16* const (&buf)[NB]) {
-#pragma unroll
-  for (int u = 0; u < NB - 1; ++u)
-    if (u < K / 32) load_w_slice(W, K, N, wave, lane, 32 * u, buf[u]);
-}
+constexpr int FP_TILES = 2;  // 16-wide output tiles per wave and layer (FP_MAXN / 16 / FP_WAVES)
+static_assert(FP_TILES * 16 * FP_WAVES == FP_MAXN, "two output tiles per wave cover the widest layer");
 
-template <int NTW, int NB>
-__device__ __forceinline__ void fused_layer(const bf16* __restrict__ W, int K, int N, int wave, int lane, int row0,
-                                            int nw0, const bf16* actA, const bf16* actB, bf16* const (&buf)[NB],
-                                            f32x4 (&acc)[4][4]) {
-  const int r = lane & 15, q = lane >> 4;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < NTW; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int groups = N / 16;
-  // this wave's loads per slice (the vmcnt a slice still in flight holds)
-  const int per_wave = wave < groups ? (groups - wave + FP_WAVES - 1) / FP_WAVES : 0;
-  const int nsteps = K / 32;
-  auto load_slice = [&](int k0, bf16* dst) { load_w_slice(W, K, N, wave, lane, k0, dst); };
-  // k-step t: slice t from `cur`; slice t + NB - 1 starts into `ahead_buf`
-  // (the buffer every wave finished reading before the barrier of step t - 1)
-  auto step = [&](int t, const bf16* cur, bf16* ahead_buf) {
-    const int k0 = 32 * t;
-    if (t + NB - 1 < nsteps) load_slice(k0 + 32 * (NB - 1), ahead_buf);
-    const bool inA = k0 < 256;
-    const int ap = inA ? FA_PITCH : FB_PITCH;
-    const bf16* arow = (inA ? actA + k0 : actB + (k0 - 256)) + (row0 + r) * ap + q * 8;
-    const bf16* pf[8];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) pf[i] = arow + 16 * i * ap;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int n = nw0 + 16 * (j < NTW ? j : NTW - 1) + r;  // (unused reads repeat the last tile)
-      pf[4 + j] = cur + (4 * n + ((q + (n >> 2)) & 3)) * 8;
-    }
-    bf16x8 x[8];
-    lds_read8(pf, x);
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < NTW; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x[i], x[4 + j], acc[i][j], 0, 0, 0);
-    __builtin_amdgcn_sched_barrier(0);
-    // slice t + 1 has landed (slices t + 2 .. t + NB - 1 may still be in
-    // flight); the barrier also frees `cur` for slice t + NB.  A bare
-    // s_barrier: the loop writes no LDS itself, the reads of `cur` are
-    // complete (their values fed the MFMAs) and the direct loads are covered
-    // by the wait above.
-    const int newer = min(NB - 2, nsteps - 2 - t);
-    wait_vm_except(newer > 0 ? newer * per_wave : 0);
-    __builtin_amdgcn_s_barrier();
-  };
-  // slices 0 .. NB - 2 were issued by fused_prefetch
-  {
-    const int newer = min(NB - 2, nsteps - 1);
-    wait_vm_except(newer > 0 ? newer * per_wave : 0);  // slice 0 has landed
-  }
+// LDS writes before the barrier are visible to the LDS reads after it; the
+// fences name LDS only, so loads from global memory stay in flight across it
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
-  for (int t = 0; t < nsteps; t += NB) {
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// This lane's weight-fragment pointers for one layer: tile j of the wave is
+// output tile wave + 8 j, W row 16 (wave + 8 j) + r, k chunk q.  A tile the
+// layer does not have reads a tile it has (loaded, never used).
+struct WPtrs {
+  const bf16* w[FP_TILES];
+};
+__device__ __forceinline__ WPtrs wptrs(const bf16* W, int K, int N, int wave, int r, int q) {
+  WPtrs p;
+  const int tiles = N / 16;
 #pragma unroll
-    for (int u = 0; u < NB; ++u)
-      if (t + u < nsteps) step(t + u, buf[u], buf[(u + NB - 1) % NB]);
+  for (int j = 0; j < FP_TILES; ++j) {
+    const int t = wave + FP_WAVES * j < tiles ? wave + FP_WAVES * j : wave % tiles;
+    p.w[j] = W + (size_t)(16 * t + r) * (size_t)K + 8 * q;
+  }
+  return p;
+}
+
+// The k-loop of one layer for a wave with NT output tiles: acc[j][i] (output
+// features of tile j, rows 16 i .. 16 i + 15) = W_tile act^T.  The ring
+// holds the weight fragments of steps t .. t + D - 1 on entry (slot s % D);
+// each step refills its slot with step s + D, of this layer or (for the
+// last D steps) of the next.  The activation fragments are double-buffered:
+// step s issues the LDS reads of step s + 1 (the last step re-reads its
+// own), then its MFMAs, then the refill, and a scheduling barrier keeps that
+// order (left to itself, the compiler sank the refills to the end of the
+// unrolled block, where the next block's first MFMAs waited for them, and
+// moved the LDS reads of step s + 1 down to their use: an empty asm
+// statement with a memory clobber holds them in place).
+// HASB: the input reaches past column 256 (the first layer), whose columns
+// 256.. sit in `in1` at pitch FB_PITCH.
+template <bool HASB>
+__device__ __forceinline__ void fused_read_act(bf16x8 (&a)[8], const bf16* a0, const bf16* a1, int k0) {
+  if (!HASB || k0 < 256) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = *(const bf16x8*)(a0 + k0 + 16 * i * FA_PITCH);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = *(const bf16x8*)(a1 + (k0 - 256) + 16 * i * FB_PITCH);
   }
 }
 
-// Epilogue of one layer: GELU(acc + b) to region A of the block's LDS tile
-// (hidden layers, N <= 256), or acc + b to Y (the last layer).
-template <int NTW>
-__device__ __forceinline__ void fused_epilogue(const f32x4 (&acc)[4][4], const float (&bnv)[4], int row0, int nw0,
-                                               int r, int q, bf16* actA, bool last, float* __restrict__ Y,
-                                               long long ldy, long long m0, int out_major) {
+template <int NT, int D, bool HASB>
+__device__ __forceinline__ void fused_kloop(bf16x8 (&ring)[4][FP_TILES], const WPtrs& cur, const WPtrs& nxt,
+                                            int nsteps, const bf16* in0, const bf16* in1, int r, int q,
+                                            f32x4 (&acc)[FP_TILES][8]) {
 #pragma unroll
-  for (int j = 0; j < NTW; ++j) {
-    const int n = nw0 + 16 * j + r;
-    const float bn = bnv[j];
+  for (int j = 0; j < FP_TILES; ++j)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int mr = row0 + 16 * i + 4 * q;  // rows mr .. mr + 3 of the block
-      if (!last) {
+    for (int i = 0; i < 8; ++i) acc[j][i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bf16* a0 = in0 + r * FA_PITCH + 8 * q;
+  const bf16* a1 = in1 + r * FB_PITCH + 8 * q;
+  bf16x8 a[2][8];
+  if (NT > 0) fused_read_act<HASB>(a[0], a0, a1, 0);
+  for (int t = 0; t < nsteps; t += D) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) actA[(mr + e) * FA_PITCH + n] = f2bf(gelu(acc[i][j][e] + bn));
-      } else if (out_major) {
-        f32x4 v = acc[i][j];
-        v[0] += bn;
-        v[1] += bn;
-        v[2] += bn;
-        v[3] += bn;
-        *(f32x4*)(Y + (size_t)n * (size_t)ldy + (size_t)(m0 + mr)) = v;
+    for (int u = 0; u < D; ++u) {
+      const int s = t + u;
+      if (NT > 0) {
+        const int s1 = s + 1 < nsteps ? s + 1 : s;
+        fused_read_act<HASB>(a[(u + 1) & 1], a0, a1, 32 * s1);
+        __asm__ volatile("" ::: "memory");  // the reads stay ahead of this step's MFMAs
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+#pragma unroll
+          for (int i = 0; i < 8; ++i)
+            acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring[u][j], a[u & 1][i], acc[j][i], 0, 0, 0);
+      }
+      const int sr = s + D;
+#pragma unroll
+      for (int j = 0; j < FP_TILES; ++j)
+        ring[u][j] = *(const bf16x8*)(sr < nsteps ? cur.w[j] + 32 * sr : nxt.w[j] + 32 * (sr - nsteps));
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+template <int D, bool HASB>
+__device__ __forceinline__ void fused_kloop_n(int nt, bf16x8 (&ring)[4][FP_TILES], const WPtrs& cur, const WPtrs& nxt,
+                                              int nsteps, const bf16* in0, const bf16* in1, int r, int q,
+                                              f32x4 (&acc)[FP_TILES][8]) {
+  if (nt >= 2)
+    fused_kloop<2, D, HASB>(ring, cur, nxt, nsteps, in0, in1, r, q, acc);
+  else if (nt == 1)
+    fused_kloop<1, D, HASB>(ring, cur, nxt, nsteps, in0, in1, r, q, acc);
+  else
+    fused_kloop<0, D, HASB>(ring, cur, nxt, nsteps, in0, in1, r, q, acc);
+}
+
+__device__ __forceinline__ unsigned pack_bf16x2(float lo, float hi) {
+  return (unsigned)f2bf(lo) | ((unsigned)f2bf(hi) << 16);
+}
+
+// Hidden-layer epilogue: GELU(acc + b) in bf16 to the next layer's LDS
+// input, four features of one row per 8-byte write.
+template <int NT>
+__device__ __forceinline__ void fused_store_act(const f32x4 (&acc)[FP_TILES][8], const float* sb, int wave, int r,
+                                                int q, bf16* out) {
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int n0 = 16 * (wave + FP_WAVES * j) + 4 * q;
+    const f32x4 b = *(const f32x4*)(sb + n0);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      uint2 w;
+      w.x = pack_bf16x2(gelu(acc[j][i][0] + b[0]), gelu(acc[j][i][1] + b[1]));
+      w.y = pack_bf16x2(gelu(acc[j][i][2] + b[2]), gelu(acc[j][i][3] + b[3]));
+      *(uint2*)(out + (16 * i + r) * FA_PITCH + n0) = w;
+    }
+  }
+}
+
+// Last-layer epilogue: acc + b in FP32 to Y, row-major (Y[m ldy + n]) or
+// output-major (Y[n ldy + m]).
+template <int NT>
+__device__ __forceinline__ void fused_store_out(const f32x4 (&acc)[FP_TILES][8], const float* sb, int wave, int r,
+                                                int q, float* __restrict__ Y, long long ldy, long long m0,
+                                                int out_major) {
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int n0 = 16 * (wave + FP_WAVES * j) + 4 * q;
+    const f32x4 b = *(const f32x4*)(sb + n0);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const long long m = m0 + 16 * i + r;
+      const f32x4 v = acc[j][i] + b;
+      if (out_major) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) Y[(size_t)(n0 + e) * (size_t)ldy + (size_t)m] = v[e];
       } else {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) Y[(size_t)(m0 + mr + e) * (size_t)ldy + n] = acc[i][j][e] + bn;
+        *(f32x4*)(Y + (size_t)m * (size_t)ldy + n0) = v;
       }
     }
   }
 }
 
-template <int NB>
-__device__ __forceinline__ void fused_layer_n(int ntw, const bf16* W, int K, int N, int wave, int lane, int row0,
-                                              int nw0, const bf16* actA, const bf16* actB, bf16* const (&buf)[NB],
-                                              f32x4 (&acc)[4][4]) {
-  switch (ntw) {
-    case 4: fused_layer<4, NB>(W, K, N, wave, lane, row0, nw0, actA, actB, buf, acc); break;
-    case 3: fused_layer<3, NB>(W, K, N, wave, lane, row0, nw0, actA, actB, buf, acc); break;
-    case 2: fused_layer<2, NB>(W, K, N, wave, lane, row0, nw0, actA, actB, buf, acc); break;
-    default: fused_layer<1, NB>(W, K, N, wave, lane, row0, nw0, actA, actB, buf, acc); break;
-  }
-}
-
-// The whole network on a block of 128 rows, activations resident in LDS.
-//
-// Workgroup of 8 wavefronts: the block's layer input sits in LDS -- columns
-// 0..255 in region A (128 x FA_PITCH bf16), input columns 256..383 of the
-// first layer in region B (128 x FB_PITCH); wave w owns rows
-// 64 (w / 4) .. + 64 and output columns [(w % 4) pout / 4, (w % 4 + 1) pout
-// / 4), two waves per SIMD.  The inputs are staged with direct-to-LDS loads,
-// all in flight at once.  W streams through LDS (fused_layer): three static
-// slice buffers while region B holds layer input, five -- the two extra in
-// region B -- once it does not, so that the layers of width 256 keep four
-// slices in flight (the weight-slice latency from L2 bounds the kernel).
-// After a layer's k-loop the block overwrites region A with GELU(acc + b)
-// in bf16; the last layer adds the bias and stores FP32 to Y, row-major
-// (Y[m ldy + n]) or output-major (Y[n ldy + m]).  HBM sees only the input
-// features and the outputs: no activation round trips between layers.
-// Limits (checked by the host): pin <= FP_MAXK, pout <= FP_MAXN, all
-// multiples of 64; rows a multiple of 128.
-__global__ void __launch_bounds__(64 * FP_WAVES) k_mlp_fused(FusedNet fn, const bf16* __restrict__ X0, int ldx,
-                                                             float* __restrict__ Y, long long ldy, int out_major) {
-  extern __shared__ __align__(16) bf16 act[];  // region A then region B (dynamic LDS)
-  __shared__ __align__(16) bf16 w0[FP_MAXN * 32], w1[FP_MAXN * 32], w2[FP_MAXN * 32];
-  // every layer's bias, staged once: read from global memory at the
-  // epilogue, its loads sat behind the prefetched slices in the in-order
-  // vmcnt and the compiler drained them all
-  __shared__ float sbias[FP_MAXL * FP_MAXN];
-  bf16* actA = act;
-  bf16* actB = act + 128 * FA_PITCH;
+template <int D>
+__device__ __forceinline__ void fused_body(const FusedNet& fn, const bf16* __restrict__ X0, int ldx,
+                                           float* __restrict__ Y, long long ldy, int out_major, bf16* buf0,
+                                           bf16* buf1, float* sbias) {
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;  // wave index in a scalar register
   const int r = lane & 15, q = lane >> 4;
   const long long m0 = (long long)blockIdx.x * 128;
   {
-    // the block's LDS image, region by region, in 1 KB pieces of 64 16-byte
-    // chunks; the pad chunk at the end of each row re-reads the row's first
+    // the block's input, region by region, with direct global-to-LDS loads
+    // in 1 KB pieces of 64 16-byte chunks (lane l writes the piece's chunk
+    // l); the pad chunk at the end of each LDS row re-reads the row's first
     // chunk of that region (never used)
     auto stage = [&](bf16* dst, int pitch, int col0, int ncol) {
       const int cpr = pitch / 8;  // chunks per LDS row
@@ -570,56 +549,65 @@ __global__ void __launch_bounds__(64 * FP_WAVES) k_mlp_fused(FusedNet fn, const 
                                          16, 0, 0);
       }
     };
-    stage(actA, FA_PITCH, 0, min(ldx, 256));
-    if (ldx > 256) stage(actB, FB_PITCH, 256, ldx - 256);
+    stage(buf0, FA_PITCH, 0, min(ldx, 256));
+    if (ldx > 256) stage(buf1, FB_PITCH, 256, ldx - 256);
   }
-  const int row0 = 64 * (wave >> 2);
-  bf16* const b3[3] = {w0, w1, w2};
-  bf16* const b5[5] = {w0, w1, w2, actB, actB + FP_MAXN * 32};
-  // a layer whose input reaches into region B streams W through three
-  // buffers, any other through five
-  auto prefetch = [&](int l) {
-    if (fn.pin[l] > 256)
-      fused_prefetch<3>(fn.W[l], fn.pin[l], fn.pout[l], wave, lane, b3);
-    else
-      fused_prefetch<5>(fn.W[l], fn.pin[l], fn.pout[l], wave, lane, b5);
-  };
-  prefetch(0);
+  bf16x8 ring[4][FP_TILES];
+  WPtrs cur = wptrs(fn.W[0], fn.pin[0], fn.pout[0], wave, r, q);
+#pragma unroll
+  for (int u = 0; u < D; ++u)
+#pragma unroll
+    for (int j = 0; j < FP_TILES; ++j) ring[u][j] = *(const bf16x8*)(cur.w[j] + 32 * u);
   for (int l = 0; l < fn.nl; ++l)
     for (int n = tid; n < fn.pout[l]; n += 64 * FP_WAVES) sbias[l * FP_MAXN + n] = fn.b[l][n];
-  __builtin_amdgcn_s_waitcnt(0);  // the inputs (and the first slices) have landed
+  __builtin_amdgcn_s_waitcnt(0);  // the staged input has landed (and the first weight fragments)
   __syncthreads();
+  const bf16* in0 = buf0;
+  bf16* out = buf1;
   for (int l = 0; l < fn.nl; ++l) {
     const int K = fn.pin[l], N = fn.pout[l];
-    const int ntw = N / 64;                // 16-wide n-tiles per wave (1..4)
-    const int nw0 = (wave & 3) * (N / 4);  // this wave's first output column
-    const bool last = (l == fn.nl - 1);
-    f32x4 acc[4][4];
+    const int tiles = N / 16;
+    const int nt = (wave < tiles ? 1 : 0) + (wave + FP_WAVES < tiles ? 1 : 0);
+    const bool last = l == fn.nl - 1;
+    const WPtrs nxt = last ? cur : wptrs(fn.W[l + 1], fn.pin[l + 1], fn.pout[l + 1], wave, r, q);
+    f32x4 acc[FP_TILES][8];
     if (K > 256)
-      fused_layer_n<3>(ntw, fn.W[l], K, N, wave, lane, row0, nw0, actA, actB, b3, acc);
+      fused_kloop_n<D, true>(nt, ring, cur, nxt, K / 32, in0, buf1, r, q, acc);
     else
-      fused_layer_n<5>(ntw, fn.W[l], K, N, wave, lane, row0, nw0, actA, actB, b5, acc);
-    // this wave's bias values, while no load is in flight
-    float bn[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) bn[j] = j < ntw ? sbias[l * FP_MAXN + nw0 + 16 * j + r] : 0.f;
-    // the k-loop ended with a barrier: every wave has read this layer's
-    // input and the slice buffers are free, so the next layer's first
-    // slices go out now and arrive during the epilogue
-    if (!last) prefetch(l + 1);
-    switch (ntw) {
-      case 4: fused_epilogue<4>(acc, bn, row0, nw0, r, q, actA, last, Y, ldy, m0, out_major); break;
-      case 3: fused_epilogue<3>(acc, bn, row0, nw0, r, q, actA, last, Y, ldy, m0, out_major); break;
-      case 2: fused_epilogue<2>(acc, bn, row0, nw0, r, q, actA, last, Y, ldy, m0, out_major); break;
-      default: fused_epilogue<1>(acc, bn, row0, nw0, r, q, actA, last, Y, ldy, m0, out_major); break;
+      fused_kloop_n<D, false>(nt, ring, cur, nxt, K / 32, in0, buf1, r, q, acc);
+    const float* sb = sbias + l * FP_MAXN;
+    if (last) {
+      if (nt >= 2)
+        fused_store_out<2>(acc, sb, wave, r, q, Y, ldy, m0, out_major);
+      else if (nt == 1)
+        fused_store_out<1>(acc, sb, wave, r, q, Y, ldy, m0, out_major);
+      break;
     }
-    // the next layer's input is complete: this wave's LDS writes are done
-    // (lgkmcnt(0); vmcnt and expcnt left at their maximum) and the barrier
-    // meets the others -- not __syncthreads, whose vmcnt(0) would drain the
-    // prefetched slices
-    __builtin_amdgcn_s_waitcnt(0xC07F);
-    __builtin_amdgcn_s_barrier();
+    if (l == 0) lds_barrier();  // the first layer's input covers part of buf1: all reads done first
+    if (nt >= 2)
+      fused_store_act<2>(acc, sb, wave, r, q, out);
+    else if (nt == 1)
+      fused_store_act<1>(acc, sb, wave, r, q, out);
+    lds_barrier();  // the next layer's input is complete
+    in0 = out;
+    out = (out == buf1) ? buf0 : buf1;
+    cur = nxt;
   }
+}
+
+// Limits (checked by the host): pin[0] <= FP_MAXK, every other pin and every
+// pout <= FP_MAXN, all multiples of 64; rows a multiple of 128.  Weight
+// fragments run 4 k-steps ahead when every pin is a multiple of 128, else 2.
+__global__ void __launch_bounds__(64 * FP_WAVES) k_mlp_fused(FusedNet fn, const bf16* __restrict__ X0, int ldx,
+                                                             float* __restrict__ Y, long long ldy, int out_major) {
+  extern __shared__ __align__(16) bf16 act[];  // buf0 then buf1 (dynamic LDS)
+  __shared__ __align__(16) float sbias[FP_MAXL * FP_MAXN];
+  bool d4 = true;
+  for (int l = 0; l < fn.nl; ++l) d4 = d4 && (fn.pin[l] % 128 == 0);
+  if (d4)
+    fused_body<4>(fn, X0, ldx, Y, ldy, out_major, act, act + 128 * FA_PITCH, sbias);
+  else
+    fused_body<2>(fn, X0, ldx, Y, ldy, out_major, act, act + 128 * FA_PITCH, sbias);
 }
 
 // out[c][r] = in[r][c] for an R x C bf16 matrix (row pitches ldi, ldo).

@@ -265,7 +265,8 @@ static bool unfused_requested() {
 bool fused_shape_ok(const LayerView& v, int Mpad) {
   if (unfused_requested() || Mpad % 128 != 0 || v.nl > FP_MAXL) return false;
   for (int l = 0; l < v.nl; ++l)
-    if (v.pin[l] > FP_MAXK || v.pout[l] > FP_MAXN) return false;
+    if (v.pin[l] > (l == 0 ? FP_MAXK : FP_MAXN) || v.pout[l] > FP_MAXN || v.pin[l] % 64 || v.pout[l] % 64)
+      return false;
   return true;
 }
 

@@ -64,12 +64,16 @@ def _call_grad(lib, widths, params, x, t, w):
 
 
 @pytest.mark.gpu
-def test_mlp_forward_matches_oracle(gpu_lib):
-    print("[case] MLP forward 381-256x4-149, batch 200, vs FP64 oracle", flush=True)
-    widths = [381, 256, 256, 256, 256, 149]
+@pytest.mark.parametrize("widths,batch", [
+    ([381, 256, 256, 256, 256, 149], 200),  # the closure network
+    ([100, 64, 192, 149], 300),             # layers narrower than the waves' tiles; 2 k-steps ahead
+    ([320, 64, 256, 40], 130),              # first layer past column 256 with a narrow output
+])
+def test_mlp_forward_matches_oracle(gpu_lib, widths, batch):
+    print(f"[case] MLP forward {widths}, batch {batch}, vs FP64 oracle", flush=True)
     rng = np.random.default_rng(2)
     p = MO.init_params(widths, seed=3)
-    x = rng.normal(0, 1, (200, 381)).astype(np.float32)
+    x = rng.normal(0, 1, (batch, widths[0])).astype(np.float32)
     y = _call_eval(gpu_lib, widths, p, x).astype(np.float64)
     ye, _, _ = MO.forward(widths, p, x, bf16=True)
     y64, _, _ = MO.forward(widths, p, x, bf16=False)
