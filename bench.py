@@ -497,7 +497,7 @@ def ensemble_bench(args) -> dict:
                             "training state of")
     with ColumnEnsemble.from_oceans(train_oceans, config=cfg) as ens:
         del train_oceans
-        n = ens.mlp_collect(DT, args.train_steps, stride=4)
+        n = ens.mlp_collect(DT, args.train_steps, stride=args.ens_train_stride)
         t2 = time.perf_counter()
         schedule = parse_schedule(args.lr_schedule)
         tl, vl = train_schedule(ens, schedule, guard)
@@ -621,6 +621,8 @@ def main() -> int:
     ap.add_argument("--train-steps", type=int, default=2, help="plain steps recorded per training state")
     ap.add_argument("--train-states", type=int, default=6, help="independent state draws the network trains on")
     ap.add_argument("--train-stride", type=int, default=1, help="record every n-th column of each training step")
+    ap.add_argument("--ens-train-stride", type=int, default=1,
+                    help="the ensemble's training: record every n-th column of each training step")
     args = ap.parse_args()
 
     if args.copy_probe:

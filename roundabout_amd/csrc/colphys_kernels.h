@@ -78,6 +78,8 @@ struct SurrView {
   const unsigned char* guard;
   int lev_off;
   int pad;
+  const float* tmin;  // the range of each target over the training set: a
+  const float* tmax;  // prediction outside it is clamped to it
 };
 
 __global__ void k_prepare(StateView s, Params pr, int p0, int p1);

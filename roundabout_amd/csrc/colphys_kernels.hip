@@ -489,8 +489,13 @@ __device__ __forceinline__ void column_step(const StateView& s, const Params& pr
   // learned closure: this column takes the network's kv/kt and h_b unless
   // its guard tripped (then the plain PP81 + KPP path below runs for it)
   const bool use_s = SURR && sv.guard[p] == 0;
+  // (a network extrapolating far outside its training targets predicted
+  // kt of 1e18 m^2/s on one column of a large ensemble, whose Thomas pivots
+  // then cancelled to zero; the clamp keeps every prediction in the range
+  // the network was trained on)
   auto pred = [&](int n) -> double {
-    return (double)(sv.Yt[(size_t)n * (size_t)sv.ldy + (size_t)p] * sv.stdv[n] + sv.mean[n]);
+    const float v = sv.Yt[(size_t)n * (size_t)sv.ldy + (size_t)p] * sv.stdv[n] + sv.mean[n];
+    return (double)fmn(fmx(v, sv.tmin[n]), sv.tmax[n]);
   };
   const int B = blockDim.x, tid = threadIdx.x;
   // LDS slot of quantity q (c't, rT, rS, r of each passive, c's) at live

@@ -63,8 +63,8 @@ struct SurrState {
   float* Wn = nullptr;
   unsigned* idx = nullptr;
   // statistics
-  std::vector<float> fmean, fstd, fmin, fmax, tmean, tstd;
-  float* d_stats = nullptr;  // fmean | fstd | fmin | fmax | tmean | tstd
+  std::vector<float> fmean, fstd, fmin, fmax, tmean, tstd, tmin, tmax;
+  float* d_stats = nullptr;  // fmean | fstd | fmin | fmax | tmean | tstd | tmin | tmax
   // inference
   unsigned char* guard = nullptr;
   unsigned int* nfall = nullptr;
@@ -618,13 +618,13 @@ int surr_train(rdb_colphys* h, const rdb_colphys_mlp_config* cfg, float* train_l
   const int din = S->din, dout = S->dout;
   if (N > 0xffffffffLL) return perr(RDB_COLPHYS_ERR_BAD_SHAPE, "mlp_train: more than 2^32 training columns");
   // statistics over the whole collected set
-  std::vector<float> stats(4 * (size_t)din + 2 * (size_t)dout);
+  std::vector<float> stats(4 * (size_t)din + 4 * (size_t)dout);
   float* dst = nullptr;
   SG_TRY(hipMalloc((void**)&dst, 4 * stats.size()));
   hipLaunchKernelGGL(k_col_stats, dim3(din), dim3(256), 0, h->stream, S->Xraw, (const float*)nullptr, N, din, dst,
                      dst + din, dst + 2 * din, dst + 3 * din);
   hipLaunchKernelGGL(k_col_stats, dim3(dout), dim3(256), 0, h->stream, S->Traw, S->Wraw, N, dout, dst + 4 * din,
-                     dst + 4 * din + dout, (float*)nullptr, (float*)nullptr);
+                     dst + 4 * din + dout, dst + 4 * din + 2 * dout, dst + 4 * din + 3 * dout);
   SG_TRY(hipGetLastError());
   SG_TRY(hipMemcpyAsync(stats.data(), dst, 4 * stats.size(), hipMemcpyDeviceToHost, h->stream));
   SG_TRY(hipStreamSynchronize(h->stream));
@@ -635,7 +635,22 @@ int surr_train(rdb_colphys* h, const rdb_colphys_mlp_config* cfg, float* train_l
   S->fmin.assign(stats.begin() + 2 * din, stats.begin() + 3 * din);
   S->fmax.assign(stats.begin() + 3 * din, stats.begin() + 4 * din);
   S->tmean.assign(stats.begin() + 4 * din, stats.begin() + 4 * din + dout);
-  S->tstd.assign(stats.begin() + 4 * din + dout, stats.end());
+  S->tstd.assign(stats.begin() + 4 * din + dout, stats.begin() + 4 * din + 2 * dout);
+  S->tmin.assign(stats.begin() + 4 * din + 2 * dout, stats.begin() + 4 * din + 3 * dout);
+  S->tmax.assign(stats.begin() + 4 * din + 3 * dout, stats.end());
+  // a target no training row weights (an interface no collected column
+  // has) has an empty range: its predictions are left unclamped
+  bool patched = false;
+  for (int n = 0; n < dout; ++n)
+    if (!(S->tmin[(size_t)n] <= S->tmax[(size_t)n])) {
+      S->tmin[(size_t)n] = -3.4e38f;
+      S->tmax[(size_t)n] = 3.4e38f;
+      patched = true;
+    }
+  if (patched) {
+    SG_TRY(hipMemcpy(dst + 4 * din + 2 * dout, S->tmin.data(), 4 * (size_t)dout, hipMemcpyHostToDevice));
+    SG_TRY(hipMemcpy(dst + 4 * din + 3 * dout, S->tmax.data(), 4 * (size_t)dout, hipMemcpyHostToDevice));
+  }
   // network
   std::vector<int> widths{din};
   for (int l = 0; l < cfg->n_hidden; ++l) widths.push_back(cfg->hidden);
@@ -826,7 +841,8 @@ int surr_prepare(rdb_colphys* h) {
     // could overwrite while the copy is pending
     std::vector<SurrView> svs;
     for (rdb_colphys* m : ps)
-      svs.push_back(SurrView{net.Y + m->row_off, (long long)Mpad, tmean, tstd, S->guard + m->row_off, nzn - m->nz, 0});
+      svs.push_back(SurrView{net.Y + m->row_off, (long long)Mpad, tmean, tstd, S->guard + m->row_off, nzn - m->nz, 0,
+                             tstd + S->dout, tstd + 2 * S->dout});
     if (svs.size() != S->svs_up.size() ||
         std::memcmp(svs.data(), S->svs_up.data(), sizeof(SurrView) * svs.size()) != 0) {
       SG_TRY(hipMemcpyAsync(h->d_svs, svs.data(), sizeof(SurrView) * svs.size(), hipMemcpyHostToDevice, h->stream));
@@ -884,7 +900,7 @@ int surr_launch(rdb_colphys* h, double dt) {
   if (!h->members.empty()) return launch_step_all(h, dt, true);
   const float* tmean = S->d_stats + 4 * S->din;
   const float* tstd = tmean + S->dout;
-  const SurrView sv{net.Y, (long long)Mpad, tmean, tstd, S->guard, 0, 0};
+  const SurrView sv{net.Y, (long long)Mpad, tmean, tstd, S->guard, 0, 0, tstd + S->dout, tstd + 2 * S->dout};
   const bool two = (h->cfg.kt_bg != h->cfg.ks_bg);
   const int ntr = h->n_passive;
   dim3 g((unsigned)((h->n_wet + h->block - 1) / h->block)), b((unsigned)h->block);
@@ -996,14 +1012,14 @@ int surr_save(rdb_colphys* h, const char* path) {
   if (S->net.get_params(p.data())) return perr(RDB_COLPHYS_ERR_SETUP, "mlp_save: %s", S->net.err.c_str());
   FILE* f = std::fopen(path, "wb");
   if (!f) return perr(RDB_COLPHYS_ERR_IO, "mlp_save: cannot open %s", path);
-  // version 3: buoyancy-jump / signed-log level encoding of the features,
-  // logarithms in single precision
-  const char magic[8] = {'R', 'D', 'B', 'S', 'U', 'R', 'R', '4'};
+  // version 4: buoyancy-jump / signed-log level encoding of the features,
+  // logarithms in single precision; version 5 adds the targets' range
+  const char magic[8] = {'R', 'D', 'B', 'S', 'U', 'R', 'R', '5'};
   int hdr[4] = {S->nz, S->din, S->dout, (int)S->widths.size()};
   bool ok = std::fwrite(magic, 1, 8, f) == 8 && std::fwrite(hdr, 4, 4, f) == 4 &&
             std::fwrite(S->widths.data(), 4, S->widths.size(), f) == S->widths.size() &&
             std::fwrite(p.data(), 4, p.size(), f) == p.size();
-  for (const std::vector<float>* v : {&S->fmean, &S->fstd, &S->fmin, &S->fmax, &S->tmean, &S->tstd})
+  for (const std::vector<float>* v : {&S->fmean, &S->fstd, &S->fmin, &S->fmax, &S->tmean, &S->tstd, &S->tmin, &S->tmax})
     ok = ok && std::fwrite(v->data(), 4, v->size(), f) == v->size();
   ok = std::fclose(f) == 0 && ok;
   return ok ? RDB_COLPHYS_OK : perr(RDB_COLPHYS_ERR_IO, "mlp_save: write to %s failed", path);
@@ -1014,7 +1030,9 @@ int surr_load(rdb_colphys* h, const char* path) {
   if (!f) return perr(RDB_COLPHYS_ERR_IO, "mlp_load: cannot open %s", path);
   char magic[8];
   int hdr[4];
-  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "RDBSURR4", 8) == 0 && std::fread(hdr, 4, 4, f) == 4;
+  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "RDBSURR", 7) == 0 &&
+            (magic[7] == '4' || magic[7] == '5') && std::fread(hdr, 4, 4, f) == 4;
+  const bool has_trange = ok && magic[7] == '5';
   if (!ok || hdr[0] != net_levels(h) || hdr[3] < 2 || hdr[3] > 16) {
     std::fclose(f);
     return perr(RDB_COLPHYS_ERR_BAD_SHAPE, "mlp_load: %s is not a network for nz = %d", path, net_levels(h));
@@ -1034,6 +1052,11 @@ int surr_load(rdb_colphys* h, const char* path) {
   S->tstd.resize(S->dout);
   for (std::vector<float>* v : {&S->fmean, &S->fstd, &S->fmin, &S->fmax, &S->tmean, &S->tstd})
     ok = ok && std::fread(v->data(), 4, v->size(), f) == v->size();
+  // a version-4 file has no targets' range: predictions are not clamped
+  S->tmin.assign(S->dout, -3.4e38f);
+  S->tmax.assign(S->dout, 3.4e38f);
+  if (has_trange)
+    for (std::vector<float>* v : {&S->tmin, &S->tmax}) ok = ok && std::fread(v->data(), 4, v->size(), f) == v->size();
   std::fclose(f);
   if (!ok) return perr(RDB_COLPHYS_ERR_IO, "mlp_load: %s is truncated or malformed", path);
   e = S->net.set_params(p.data());
@@ -1041,7 +1064,7 @@ int surr_load(rdb_colphys* h, const char* path) {
   S->widths = widths;
   S->have_net = true;
   std::vector<float> stats;
-  for (const std::vector<float>* v : {&S->fmean, &S->fstd, &S->fmin, &S->fmax, &S->tmean, &S->tstd})
+  for (const std::vector<float>* v : {&S->fmean, &S->fstd, &S->fmin, &S->fmax, &S->tmean, &S->tstd, &S->tmin, &S->tmax})
     stats.insert(stats.end(), v->begin(), v->end());
   if (S->d_stats) (void)hipFree(S->d_stats);
   S->d_stats = nullptr;

@@ -131,23 +131,30 @@ def test_mlp_forward_output_major_matches_oracle(gpu_lib):
 
 # ------------------------------------------------------------------ the guard
 def _read_net(path):
+    """The network file: header, parameters, then the statistics
+    fmean | fstd | fmin | fmax | tmean | tstd | tmin | tmax (version 5)."""
     b = Path(path).read_bytes()
-    assert b[:8] == b"RDBSURR4"
+    assert b[:8] == b"RDBSURR5"
     nz, din, dout, nw = struct.unpack("<4i", b[8:24])
     widths = np.frombuffer(b, dtype="<i4", count=nw, offset=24)
     off = 24 + 4 * nw
     npub = int(sum(int(o) * int(i) + int(o) for i, o in zip(widths[:-1], widths[1:])))
     params = np.frombuffer(b, dtype="<f4", count=npub, offset=off)
     off += 4 * npub
-    stats = np.frombuffer(b, dtype="<f4", count=4 * din + 2 * dout, offset=off).copy()
+    stats = np.frombuffer(b, dtype="<f4", count=4 * din + 4 * dout, offset=off).copy()
+    assert off + 4 * stats.size == len(b)
     return {"head": b[:off], "nz": nz, "din": din, "dout": dout, "params": params, "stats": stats}
 
 
-def _write_net(path, net, fmin, fmax):
-    din = net["din"]
+def _write_net(path, net, fmin, fmax, tmin=None, tmax=None):
+    din, dout = net["din"], net["dout"]
     st = net["stats"].copy()
     st[2 * din:3 * din] = fmin
     st[3 * din:4 * din] = fmax
+    if tmin is not None:
+        st[4 * din + 2 * dout:4 * din + 3 * dout] = tmin
+    if tmax is not None:
+        st[4 * din + 3 * dout:] = tmax
     Path(path).write_bytes(net["head"] + st.astype("<f4").tobytes())
 
 
@@ -241,6 +248,75 @@ def test_guard_range_on_one_input_matches_oracle(gpu_lib, small_net, tmp_path):
     want = wet & ~GO.range_flags(ocean.Q_heat, lo, hi)
     assert np.array_equal(used, want), int(np.sum(used != want))
     assert fb == pytest.approx(0.3, abs=0.02)
+
+
+def test_training_records_the_targets_range(small_net):
+    """The file carries each target's range over the training set (weighted
+    rows only): finite, ordered, and around the mean."""
+    print("[case] network file: targets' range, 64x64x20 seed 31", flush=True)
+    _, net = small_net
+    din, dout = net["din"], net["dout"]
+    st = net["stats"]
+    tmean, tmin, tmax = st[4 * din:4 * din + dout], st[4 * din + 2 * dout:4 * din + 3 * dout], st[4 * din + 3 * dout:]
+    assert np.all(np.isfinite(tmin)) and np.all(np.isfinite(tmax)) and np.all(tmin <= tmax)
+    on = tmax < 1e38  # outputs no training row weights are left unclamped (the widest range)
+    assert on.sum() > dout // 2 and np.all(tmin[~on] < -1e38)
+    assert np.all(tmin[on] <= tmean[on] + 1e-3) and np.all(tmean[on] <= tmax[on] + 1e-3)
+
+
+def test_predictions_are_clamped_to_the_targets_range(gpu_lib, small_net, tmp_path):
+    """A targets' range of a single value pins every prediction: log10 kt =
+    -3 at every interface and a boundary-layer depth of 0 (no KPP profile to
+    merge), so each surrogate column's kt is 10^-3 between live layers."""
+    print("[case] surrogate with the targets' range pinned, 64x64x20 seed 31, 1 step", flush=True)
+    ocean, net = small_net
+    nz, dout = net["nz"], net["dout"]
+    tmin = np.full(dout, -3.0, dtype=np.float32)  # log10 kt and log10 kv
+    tmin[-1] = 0.0                                # boundary-layer depth
+    path = tmp_path / "pinned.bin"
+    _write_net(path, net, np.float32(-3.0e38), np.float32(3.0e38), tmin, tmin)
+    cfg = default_config()
+    with ColumnPhysics.from_ocean(ocean, config=cfg) as cp:
+        cp.mlp_load(path)
+        cp.set_guard(GUARD_RANGE)
+        cp.set_closure(CLOSURE_SURROGATE)
+        cp.step(DT, 1)
+        assert cp.stats()["fallback_fraction"] == 0.0
+        kt, kv, bl = cp.get_field("kt"), cp.get_field("kv"), cp.get_field("bl_depth")
+    want = min(max(float(np.exp2(np.float64(-3.0) * 3.321928094887362)), cfg.kt_bg), cfg.kd_max)
+    want_v = min(max(float(np.exp2(np.float64(-3.0) * 3.321928094887362)), cfg.kv_bg), cfg.kv_max)
+    wet = ocean.wet > 0.5
+    live = ocean.h > 1.5e-4
+    both = live[:, :, 1:] & live[:, :, :-1] & wet[:, :, None]  # interfaces 1 .. nz-1 between live layers
+    assert both.sum() > 1000
+    # (the device's exp2 and numpy's may differ in the last bit)
+    assert kt.shape[2] == nz + 1  # interfaces 0 .. nz
+    np.testing.assert_allclose(kt[:, :, 1:nz][both], want, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(kv[:, :, 1:nz][both], want_v, rtol=1e-15, atol=0)
+    assert np.all(bl[wet] == 0.0)
+
+
+def test_version4_network_file_loads_without_clamp(gpu_lib, small_net, tmp_path):
+    """A file written before the targets' range existed (version 4) still
+    loads; its predictions are not clamped, so it steps like the version-5
+    file whose range is wide open."""
+    print("[case] version-4 network file, 64x64x20 seed 31, 2 steps", flush=True)
+    ocean, net = small_net
+    din, dout = net["din"], net["dout"]
+    st = net["stats"].copy()
+    st[2 * din:4 * din] = np.concatenate([np.full(din, -3.0e38), np.full(din, 3.0e38)]).astype(np.float32)
+    p4, p5 = tmp_path / "v4.bin", tmp_path / "v5.bin"
+    p4.write_bytes(b"RDBSURR4" + net["head"][8:] + st[:4 * din + 2 * dout].astype("<f4").tobytes())
+    _write_net(p5, net, np.float32(-3.0e38), np.float32(3.0e38), np.float32(-3.0e38), np.float32(3.0e38))
+    out = []
+    for path in (p4, p5):
+        with ColumnPhysics.from_ocean(ocean, config=default_config()) as cp:
+            cp.mlp_load(path)
+            cp.set_guard(GUARD_SURFACE)
+            cp.set_closure(CLOSURE_SURROGATE)
+            cp.step(DT, 2)
+            out.append((cp.get_field("temperature"), cp.get_field("kt")))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
 def test_retraining_after_surrogate_steps(gpu_lib, tmp_path):
