@@ -519,7 +519,13 @@ int rdb_colphys_mlp_collect(rdb_colphys *h, double dt, int n_steps, int stride, 
 int rdb_colphys_mlp_train(rdb_colphys *h, const rdb_colphys_mlp_config *cfg, float *train_loss, float *val_loss,
                           int n_hist, int *n_epochs_done);
 
-/* Network + normalisation statistics to / from a file. */
+/* Network + normalisation statistics to / from a file: the features' mean,
+ * std and training range (the guard's range test) and the targets' mean,
+ * std and training range (a surrogate step clamps every prediction into
+ * it).  Files of the previous format, without the targets' range, load and
+ * are not clamped.  A later mlp_train call continues from the network and
+ * the Adam moments it holds (so successive calls can lower the learning
+ * rate). */
 int rdb_colphys_mlp_save(rdb_colphys *h, const char *path);
 int rdb_colphys_mlp_load(rdb_colphys *h, const char *path);
 
