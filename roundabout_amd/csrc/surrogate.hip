@@ -137,15 +137,19 @@ struct LevelVals {
 // never sit behind a branch, which made the compiler wait for every load in
 // flight; the loads and the EOS do not depend on liveness, and a vanished
 // level's values are never used)
-__device__ __forceinline__ LevelVals load_raw(const StateView& s, int p, int k) {
+// kfirst: the column's first live level; a bed filler below it is never
+// live, so its T, S, u, v are never used and are read from level kfirst
+// instead (lines the walk reads anyway: cache hits, not HBM traffic)
+__device__ __forceinline__ LevelVals load_raw(const StateView& s, int p, int k, int kfirst = 0) {
   const int P = s.P;
   const int kc = min(max(k, 0), s.nz - 1);
+  const int kv = max(kc, min(kfirst, s.nz - 1));
   LevelVals v;
   v.h = LV(s.h, kc);
-  v.T = LV(s.T, kc);
-  v.S = LV(s.S, kc);
-  v.U = LV(s.ub, kc);
-  v.V = LV(s.vb, kc);
+  v.T = LV(s.T, kv);
+  v.S = LV(s.S, kv);
+  v.U = LV(s.ub, kv);
+  v.V = LV(s.vb, kv);
   v.rho = 0.0;
   v.live = k >= 0 && k < s.nz;  // "exists"; finish_level turns it into "live"
   return v;
@@ -156,8 +160,9 @@ __device__ __forceinline__ LevelVals finish_level(LevelVals v, double rho0, doub
   v.rho = eos_wright<double>(v.T, v.S, v.h, rho0, p_ref);
   return v;
 }
-__device__ __forceinline__ LevelVals load_level(const StateView& s, int p, int k, double rho0, double p_ref) {
-  return finish_level(load_raw(s, p, k), rho0, p_ref);
+__device__ __forceinline__ LevelVals load_level(const StateView& s, int p, int k, double rho0, double p_ref,
+                                                int kfirst = 0) {
+  return finish_level(load_raw(s, p, k, kfirst), rho0, p_ref);
 }
 
 // sign(x) log10(1 + |x| / x0), in single precision: the argument is formed
@@ -380,7 +385,8 @@ __device__ __forceinline__ void features_block(int p0, const StateView& s, const
   if (p < n) {
     const int per = (nzn + FW - 1) / FW;
     const int j0 = part * per, j1 = min(nzn, j0 + per);
-    LevelVals above = load_level(s, p, j1 - lev_off, rho0, p_ref);  // the level above this part's highest
+    const int kfirst = s.kb[p];
+    LevelVals above = load_level(s, p, j1 - lev_off, rho0, p_ref, kfirst);  // the level above this part's highest
     bool unstable = false;
     double dpart = 0.0;  // this group's live thickness, summed from its top down
     // the next two levels' loads stay in flight while a level is computed:
@@ -390,7 +396,7 @@ __device__ __forceinline__ void features_block(int p0, const StateView& s, const
     // (the loads are unconditional -- past the group's lowest level they
     // read the next group's levels, which its own wave loads as well)
     auto step = [&](int j, const LevelVals& raw, LevelVals& ahead) {
-      ahead = load_raw(s, p, j - 2 - lev_off);
+      ahead = load_raw(s, p, j - 2 - lev_off, kfirst);
       __builtin_amdgcn_sched_barrier(0);
       if (j < j0) return;
       const LevelVals cur = finish_level(raw, rho0, p_ref);
@@ -404,8 +410,8 @@ __device__ __forceinline__ void features_block(int p0, const StateView& s, const
       if (cur.live) dpart += cur.h;
       above = cur;
     };
-    LevelVals sa = load_raw(s, p, j1 - 1 - lev_off);
-    LevelVals sb = load_raw(s, p, j1 - 2 - lev_off);
+    LevelVals sa = load_raw(s, p, j1 - 1 - lev_off, kfirst);
+    LevelVals sb = load_raw(s, p, j1 - 2 - lev_off, kfirst);
     LevelVals sc;
     for (int j = j1 - 1; j >= j0; j -= 3) {
       step(j, sa, sc);
