@@ -417,6 +417,7 @@ def surrogate_leg(ocean, cfg, args, plain_value: float) -> dict:
         "heat_rel_residual": abs((h1 - h0) - dep) / abs(h0),
         "salt_rel_residual": abs(s1 - s0) / abs(s0),
         "fallback_fraction_last_step": st["fallback_fraction"],
+        "graph_captured": bool(st["graph_replay"]),
         "guard": args.guard,
         "training": {"columns": n, "tflops": train_tflops, "mfma_frac": train_tflops / PEAK_BF16_TFLOPS,
                      "source": f"{args.train_steps} plain steps of each of {args.train_states} "
@@ -552,7 +553,8 @@ def ensemble_bench(args) -> dict:
         "config": {"workload": "batched ensemble: 16 z-level members, each seeded separately, from "
                                "256x256x50 to 1440x1080x75; one persistent graph-captured launch per step",
                    "members": [list(sz) for sz in sizes], "wet_columns": n_wet, "path": "plain",
-                   "block_threads": st_plain["block_threads"], "lds_levels": st_plain["lds_levels"]},
+                   "block_threads": st_plain["block_threads"], "lds_levels": st_plain["lds_levels"],
+                   "graph_captured": bool(st_plain["graph_replay"])},
         "roofline": {"bound": "hbm", "achieved": st_plain["algorithmic_bytes_per_step"] / (wall_plain / K) / 1e9,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": st_plain["algorithmic_bytes_per_step"] / (wall_plain / K) / 1e9 / PEAK_HBM_GBS,
@@ -564,6 +566,7 @@ def ensemble_bench(args) -> dict:
                       "meets_error_bar": bool(t_rms <= 0.01 and s_rms <= 0.002),
                       "heat_rel_residual": abs((h1 - h0) - dep) / abs(h0), "salt_rel_residual": abs(s1 - s0) / abs(s0),
                       "fallback_fraction_last_step": st_s["fallback_fraction"], "guard": args.guard,
+                      "graph_captured": bool(st_s["graph_replay"]),
                       "training": {"columns": n, "epochs": sum(e for e, _ in schedule), "val_loss_first": vl[0],
                                    "val_loss_last": vl[-1], "seconds": t_train},
                       "mlp": {"forward_ms": st_s["mlp_forward_ms"], "flop_per_step": st_s["mlp_flop_per_step"]}},
@@ -708,6 +711,7 @@ def main() -> int:
             "kernel": "k_column_step (fused, one lane per column, hipGraph replay)",
             "block_threads": stats["block_threads"],
             "lds_levels": stats["lds_levels"],
+            "graph_captured": bool(stats["graph_replay"]),
             "column_order": "deepest first" if stats["order_mode"] == 1 else "grid order",
             "live_levels_mean": stats["live_levels"] / max(stats["wet_columns"], 1),
         },

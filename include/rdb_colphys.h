@@ -198,7 +198,10 @@ typedef struct rdb_colphys_counters {
   int block_threads;           /* autotuned launch shape                   */
   int order_mode;              /* 0 natural order, 1 sorted by depth       */
   int lds_levels;              /* autotuned live levels kept in LDS        */
-  int pad1;
+  int graph_replay;            /* 1: the last rdb_colphys_step replayed its
+                                  steps from a captured hipGraph (the fused
+                                  step, the ensemble, the learned closure);
+                                  0: it launched the staged kernels          */
   double mlp_forward_ms;       /* surrogate closure, timing on: HIP-event
                                   time of the network's forward GEMMs in the
                                   last step                                 */

@@ -1264,6 +1264,7 @@ int rdb_colphys_step(rdb_colphys* h, double dt, int n_steps) {
     // through KV_ML_INVZ2.
     const bool invz2 = h->cpar.kv_ml_invz2 > 0.0 && h->cpar.hmix > 0.0;
     const int pc = (h->extras && invz2) ? h->n_all : h->n_wet;
+    h->last_graph = 0;
     if (h->timing) {
       st = ensure_events(h, 2);
       if (st) return st;
@@ -1303,6 +1304,7 @@ int rdb_colphys_step(rdb_colphys* h, double dt, int n_steps) {
       HIP_TRY(hipEventRecord(h->events[0], h->stream));
     }
     HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
+    h->last_graph = 1;
     if (h->timing) {
       HIP_TRY(hipEventRecord(h->events[1], h->stream));
       HIP_TRY(hipEventSynchronize(h->events[1]));
@@ -1592,6 +1594,7 @@ int rdb_colphys_stats(rdb_colphys* h, rdb_colphys_counters* out) {
   out->algorithmic_bytes_per_step = 8.0 * (14.0 * (double)live + 3.0 * (double)wet) + 44.0 * (double)wet;
   out->live_levels = live;
   out->lds_levels = h->lds_levels;
+  out->graph_replay = h->last_graph;
   if (h->closure != RDB_COLPHYS_CLOSURE_PLAIN) {
     const int st = colphys::surr_settle(h);
     if (st) return st;

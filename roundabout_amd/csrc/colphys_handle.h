@@ -75,6 +75,7 @@ struct rdb_colphys {
   // measurement
   bool timing = false;
   double last_ms = 0.0, sum_ms = 0.0;
+  int last_graph = 0;  // the last step replayed a captured hipGraph
   long long timed = 0, steps = 0;
   int block = 256;
   int lds_levels = 0;  // live levels of Thomas state kept in LDS per lane

@@ -976,6 +976,7 @@ int surr_run(rdb_colphys* h, double dt, int n) {
     SG_TRY(hipEventRecord(e0, h->stream));
   }
   SG_TRY(hipGraphLaunch(S->gexec, h->stream));
+  h->last_graph = 1;
   if (h->timing) {
     SG_TRY(hipEventRecord(e1, h->stream));
     SG_TRY(hipEventSynchronize(e1));

@@ -186,7 +186,7 @@ class Counters(ctypes.Structure):
         ("block_threads", ctypes.c_int),
         ("order_mode", ctypes.c_int),
         ("lds_levels", ctypes.c_int),
-        ("pad1", ctypes.c_int),
+        ("graph_replay", ctypes.c_int),
         ("mlp_forward_ms", ctypes.c_double),
         ("mlp_flop_per_step", ctypes.c_double),
     ]

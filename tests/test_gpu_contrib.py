@@ -154,6 +154,7 @@ def test_contributors_tile_crop_bitwise(orc, gpu_lib):
     ost.step(DT, 6)
     with ColumnPhysics.from_ocean(ocean, config=cfg) as cp:
         cp.step(DT, 6)
+        assert cp.stats()["graph_replay"] == 0  # the staged kernels, launched one by one
         _compare(cp, ost, cfg)
         mld = cp.get_field("mld")
         wet = ocean.wet > 0.5

@@ -91,6 +91,7 @@ def test_ensemble_guarded_surrogate(gpu_lib, tmp_path):
     cfg = default_config()
     with ColumnEnsemble.from_oceans(oceans, config=cfg) as ens:
         ens.step(DT, 24)
+        assert ens.stats()["graph_replay"] == 1  # the plain ensemble step replays its captured graph
         T_ref = [ens.member_get_field(m, "temperature") for m in range(len(oceans))]
     net = tmp_path / "ens.bin"
     with ColumnEnsemble.from_oceans([synth.redraw(o, 700 + i) for i, o in enumerate(oceans)], config=cfg) as ens:
@@ -107,6 +108,7 @@ def test_ensemble_guarded_surrogate(gpu_lib, tmp_path):
         ens.step(DT, 24)
         h1, s1 = ens.heat_salt_totals()
         fb = ens.stats()["fallback_fraction"]
+        assert ens.stats()["graph_replay"] == 1  # so does the guarded-surrogate step
         T = [ens.member_get_field(m, "temperature") for m in range(len(oceans))]
         kbl = [ens.member_get_field(m, "kbl") for m in range(len(oceans))]
     dep = 24 * DT / (cfg.rho0 * cfg.cp) * sum(float((o.Q_heat * o.wet).sum()) for o in oceans)

@@ -68,6 +68,7 @@ def test_fused_step_small_bitwise(orc, gpu_lib, nsteps):
     ost = run_oracle(orc, ocean, cfg, nsteps)
     with ColumnPhysics.from_ocean(ocean, config=cfg) as cp:
         cp.step(DT, nsteps)
+        assert cp.stats()["graph_replay"] == 1  # the fused step replays its captured graph
         compare_state(cp, ost, ocean.nz, ocean.n_passive)
 
 
