@@ -350,7 +350,7 @@ int launch_step(rdb_colphys* h, double dt, int p0, int p1) {
   dim3 g = grid1(p1 - p0, h->block), b(h->block);
   const int M = h->lds_levels;
   const size_t shmem = (size_t)h->block * (size_t)M * (size_t)(3 + std::min(ntr, 4) + (two ? 1 : 0)) * 8;
-  const colphys::SurrView nosv{nullptr, 0, nullptr, nullptr, nullptr};
+  const colphys::SurrView nosv{};  // the plain path reads none of it
 #define LAUNCH(R, N, TF)                                                                                        \
   hipLaunchKernelGGL((colphys::k_column_step<R, N, TF, false>), g, b, shmem, h->stream, h->s, h->pr, dt, p0, p1, \
                      h->d_guard, M, nosv)

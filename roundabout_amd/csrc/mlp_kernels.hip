@@ -635,7 +635,7 @@ __device__ __forceinline__ float block_sum(const float* __restrict__ part, int n
   red[threadIdx.x] = a;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
   const float v = red[0];

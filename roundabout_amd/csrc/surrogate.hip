@@ -298,7 +298,7 @@ __global__ void k_col_stats(const float* X, const float* W, long long n, int d, 
   smx[threadIdx.x] = hi;
   __syncthreads();
   for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-    if (threadIdx.x < st) {
+    if ((int)threadIdx.x < st) {
       s1[threadIdx.x] += s1[threadIdx.x + st];
       s2[threadIdx.x] += s2[threadIdx.x + st];
       sw[threadIdx.x] += sw[threadIdx.x + st];
