@@ -31,15 +31,20 @@ __device__ __forceinline__ bf16 f2bf(float f) { return __builtin_bit_cast(bf16, 
 
 // GELU, tanh form: 0.5 x (1 + tanh(u)), u = sqrt(2/pi) (x + 0.044715 x^3),
 // evaluated as x * sigma(2 u) with sigma(z) = 1 / (1 + e^-z) (the identity
-// 0.5 (1 + tanh(u)) = sigma(2 u)): one hardware exp and one hardware
-// reciprocal (1 ulp; the result is rounded to BF16).  Large |u| saturates
-// correctly (e^-2u -> inf gives rcp(inf) = 0, -> 0 gives x).
+// 0.5 (1 + tanh(u)) = sigma(2 u)).  e^-2u = 2^(x (k1 + k2 x^2)) with the
+// constants -2 log2(e) sqrt(2/pi) and its 0.044715 multiple folded: one
+// multiply, one fma and one multiply form the exponent (the unfolded form
+// took six operations, every one a separate rounding without contraction),
+// then one hardware exp2 and one hardware reciprocal (1 ulp; the result is
+// rounded to BF16).  Large |u| saturates correctly (2^+big -> inf gives
+// rcp(inf) = 0, 2^-big -> 0 gives x).
 __device__ __forceinline__ float gelu_sigma(float x, float* u_prime) {
-  const float c = 0.7978845608028654f;
+  constexpr double c = 0.7978845608028654;
+  constexpr float k1 = (float)(-2.0 * 1.4426950408889634 * c);
+  constexpr float k2 = (float)(-2.0 * 1.4426950408889634 * c * 0.044715);
   const float x2 = x * x;
-  const float u = c * (x + 0.044715f * x2 * x);
-  if (u_prime) *u_prime = c * (1.0f + 3.0f * 0.044715f * x2);
-  return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+  if (u_prime) *u_prime = (float)c * (1.0f + 3.0f * 0.044715f * x2);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(x * __builtin_fmaf(k2, x2, k1)));
 }
 __device__ __forceinline__ float gelu(float x) { return x * gelu_sigma(x, nullptr); }
 // d/dx [x sigma(2u)] = sigma + 2 x sigma (1 - sigma) u'
