@@ -438,11 +438,16 @@ __device__ __forceinline__ void fused_read_act(bf16x8 (&a)[8], const bf16* a0, c
 template <int NT, int D, bool HASB>
 __device__ __forceinline__ void fused_kloop(bf16x8 (&ring)[4][FP_TILES], const WPtrs& cur, const WPtrs& nxt,
                                             int nsteps, const bf16* in0, const bf16* in1, int r, int q,
-                                            f32x4 (&acc)[FP_TILES][8]) {
+                                            const float* sb, int wave, f32x4 (&acc)[FP_TILES][8]) {
+  // the accumulators start at the bias (a result tile holds features
+  // n0 .. n0 + 3 of one row per lane; sb: the layer's biases in LDS): the
+  // epilogue adds nothing
 #pragma unroll
-  for (int j = 0; j < FP_TILES; ++j)
+  for (int j = 0; j < FP_TILES; ++j) {
+    const f32x4 b = j < NT ? *(const f32x4*)(sb + 16 * (wave + FP_WAVES * j) + 4 * q) : f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int i = 0; i < 8; ++i) acc[j][i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < 8; ++i) acc[j][i] = b;
+  }
   const bf16* a0 = in0 + r * FA_PITCH + 8 * q;
   const bf16* a1 = in1 + r * FB_PITCH + 8 * q;
   bf16x8 a[2][8];
@@ -474,52 +479,49 @@ __device__ __forceinline__ void fused_kloop(bf16x8 (&ring)[4][FP_TILES], const W
 template <int D, bool HASB>
 __device__ __forceinline__ void fused_kloop_n(int nt, bf16x8 (&ring)[4][FP_TILES], const WPtrs& cur, const WPtrs& nxt,
                                               int nsteps, const bf16* in0, const bf16* in1, int r, int q,
-                                              f32x4 (&acc)[FP_TILES][8]) {
+                                              const float* sb, int wave, f32x4 (&acc)[FP_TILES][8]) {
   if (nt >= 2)
-    fused_kloop<2, D, HASB>(ring, cur, nxt, nsteps, in0, in1, r, q, acc);
+    fused_kloop<2, D, HASB>(ring, cur, nxt, nsteps, in0, in1, r, q, sb, wave, acc);
   else if (nt == 1)
-    fused_kloop<1, D, HASB>(ring, cur, nxt, nsteps, in0, in1, r, q, acc);
+    fused_kloop<1, D, HASB>(ring, cur, nxt, nsteps, in0, in1, r, q, sb, wave, acc);
   else
-    fused_kloop<0, D, HASB>(ring, cur, nxt, nsteps, in0, in1, r, q, acc);
+    fused_kloop<0, D, HASB>(ring, cur, nxt, nsteps, in0, in1, r, q, sb, wave, acc);
 }
 
 __device__ __forceinline__ unsigned pack_bf16x2(float lo, float hi) {
   return (unsigned)f2bf(lo) | ((unsigned)f2bf(hi) << 16);
 }
 
-// Hidden-layer epilogue: GELU(acc + b) in bf16 to the next layer's LDS
-// input, four features of one row per 8-byte write.
+// Hidden-layer epilogue: GELU(acc) (acc started at the bias) in bf16 to the
+// next layer's LDS input, four features of one row per 8-byte write.
 template <int NT>
-__device__ __forceinline__ void fused_store_act(const f32x4 (&acc)[FP_TILES][8], const float* sb, int wave, int r,
-                                                int q, bf16* out) {
+__device__ __forceinline__ void fused_store_act(const f32x4 (&acc)[FP_TILES][8], int wave, int r, int q,
+                                                bf16* out) {
 #pragma unroll
   for (int j = 0; j < NT; ++j) {
     const int n0 = 16 * (wave + FP_WAVES * j) + 4 * q;
-    const f32x4 b = *(const f32x4*)(sb + n0);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       uint2 w;
-      w.x = pack_bf16x2(gelu(acc[j][i][0] + b[0]), gelu(acc[j][i][1] + b[1]));
-      w.y = pack_bf16x2(gelu(acc[j][i][2] + b[2]), gelu(acc[j][i][3] + b[3]));
+      w.x = pack_bf16x2(gelu(acc[j][i][0]), gelu(acc[j][i][1]));
+      w.y = pack_bf16x2(gelu(acc[j][i][2]), gelu(acc[j][i][3]));
       *(uint2*)(out + (16 * i + r) * FA_PITCH + n0) = w;
     }
   }
 }
 
-// Last-layer epilogue: acc + b in FP32 to Y, row-major (Y[m ldy + n]) or
-// output-major (Y[n ldy + m]).
+// Last-layer epilogue: acc (bias included) in FP32 to Y, row-major
+// (Y[m ldy + n]) or output-major (Y[n ldy + m]).
 template <int NT>
-__device__ __forceinline__ void fused_store_out(const f32x4 (&acc)[FP_TILES][8], const float* sb, int wave, int r,
-                                                int q, float* __restrict__ Y, long long ldy, long long m0,
-                                                int out_major) {
+__device__ __forceinline__ void fused_store_out(const f32x4 (&acc)[FP_TILES][8], int wave, int r, int q,
+                                                float* __restrict__ Y, long long ldy, long long m0, int out_major) {
 #pragma unroll
   for (int j = 0; j < NT; ++j) {
     const int n0 = 16 * (wave + FP_WAVES * j) + 4 * q;
-    const f32x4 b = *(const f32x4*)(sb + n0);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const long long m = m0 + 16 * i + r;
-      const f32x4 v = acc[j][i] + b;
+      const f32x4 v = acc[j][i];
       if (out_major) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) Y[(size_t)(n0 + e) * (size_t)ldy + (size_t)m] = v[e];
@@ -575,24 +577,24 @@ __device__ __forceinline__ void fused_body(const FusedNet& fn, const bf16* __res
     const int nt = (wave < tiles ? 1 : 0) + (wave + FP_WAVES < tiles ? 1 : 0);
     const bool last = l == fn.nl - 1;
     const WPtrs nxt = last ? cur : wptrs(fn.W[l + 1], fn.pin[l + 1], fn.pout[l + 1], wave, r, q);
+    const float* sb = sbias + l * FP_MAXN;
     f32x4 acc[FP_TILES][8];
     if (K > 256)
-      fused_kloop_n<D, true>(nt, ring, cur, nxt, K / 32, in0, buf1, r, q, acc);
+      fused_kloop_n<D, true>(nt, ring, cur, nxt, K / 32, in0, buf1, r, q, sb, wave, acc);
     else
-      fused_kloop_n<D, false>(nt, ring, cur, nxt, K / 32, in0, buf1, r, q, acc);
-    const float* sb = sbias + l * FP_MAXN;
+      fused_kloop_n<D, false>(nt, ring, cur, nxt, K / 32, in0, buf1, r, q, sb, wave, acc);
     if (last) {
       if (nt >= 2)
-        fused_store_out<2>(acc, sb, wave, r, q, Y, ldy, m0, out_major);
+        fused_store_out<2>(acc, wave, r, q, Y, ldy, m0, out_major);
       else if (nt == 1)
-        fused_store_out<1>(acc, sb, wave, r, q, Y, ldy, m0, out_major);
+        fused_store_out<1>(acc, wave, r, q, Y, ldy, m0, out_major);
       break;
     }
     if (l == 0) lds_barrier();  // the first layer's input covers part of buf1: all reads done first
     if (nt >= 2)
-      fused_store_act<2>(acc, sb, wave, r, q, out);
+      fused_store_act<2>(acc, wave, r, q, out);
     else if (nt == 1)
-      fused_store_act<1>(acc, sb, wave, r, q, out);
+      fused_store_act<1>(acc, wave, r, q, out);
     lds_barrier();  // the next layer's input is complete
     in0 = out;
     out = (out == buf1) ? buf0 : buf1;
