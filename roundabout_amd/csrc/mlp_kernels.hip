@@ -488,8 +488,12 @@ __device__ __forceinline__ void fused_kloop_n(int nt, bf16x8 (&ring)[4][FP_TILES
     fused_kloop<0, D, HASB>(ring, cur, nxt, nsteps, in0, in1, r, q, sb, wave, acc);
 }
 
+// two floats to two bf16 (round to nearest even) in one word: one
+// v_cvt_pk_bf16_f32 (two one-lane conversions and the shift and or took four)
 __device__ __forceinline__ unsigned pack_bf16x2(float lo, float hi) {
-  return (unsigned)f2bf(lo) | ((unsigned)f2bf(hi) << 16);
+  typedef __attribute__((ext_vector_type(2))) float f32x2;
+  typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2_t;
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(f32x2{lo, hi}, bf16x2_t));
 }
 
 // Hidden-layer epilogue: GELU(acc) (acc started at the bias) in bf16 to the
