@@ -700,8 +700,11 @@ __device__ __forceinline__ void column_step(const StateView& s, const Params& pr
     if (use_s) {
       // predicted log10 kt (outputs 0..nz-2) and log10 kv (nz-1..2nz-3)
       const int i_net = kf + sv.lev_off;  // the network's interface
-      ktv = exp2(pred(i_net - 1) * 3.321928094887362);
-      kvv = exp2(pred(nz + sv.lev_off - 1 + i_net - 1) * 3.321928094887362);
+      // 10^pred in single precision (one hardware exp2; a relative error of
+      // a few 1e-8, far inside the network's own): the FP64 exp2 of two
+      // predictions per interface had cost as much as the rest of the level
+      ktv = (double)__builtin_amdgcn_exp2f((float)pred(i_net - 1) * 3.32192809f);
+      kvv = (double)__builtin_amdgcn_exp2f((float)pred(nz + sv.lev_off - 1 + i_net - 1) * 3.32192809f);
     } else {
       R kv_r, kt_r;
       pp81_iface<R>(pr, ub_lo, vb_lo, ub_hi, vb_hi, R(h_lo), R(h_hi), rho_hi - rho_lo, kv_r, kt_r);

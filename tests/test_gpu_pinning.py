@@ -283,16 +283,17 @@ def test_predictions_are_clamped_to_the_targets_range(gpu_lib, small_net, tmp_pa
         cp.step(DT, 1)
         assert cp.stats()["fallback_fraction"] == 0.0
         kt, kv, bl = cp.get_field("kt"), cp.get_field("kv"), cp.get_field("bl_depth")
-    want = min(max(float(np.exp2(np.float64(-3.0) * 3.321928094887362)), cfg.kt_bg), cfg.kd_max)
-    want_v = min(max(float(np.exp2(np.float64(-3.0) * 3.321928094887362)), cfg.kv_bg), cfg.kv_max)
+    ten_m3 = float(np.exp2(np.float32(-3.0) * np.float32(3.32192809)))  # 10^-3 in single precision
+    want = min(max(ten_m3, cfg.kt_bg), cfg.kd_max)
+    want_v = min(max(ten_m3, cfg.kv_bg), cfg.kv_max)
     wet = ocean.wet > 0.5
     live = ocean.h > 1.5e-4
     both = live[:, :, 1:] & live[:, :, :-1] & wet[:, :, None]  # interfaces 1 .. nz-1 between live layers
     assert both.sum() > 1000
-    # (the device's exp2 and numpy's may differ in the last bit)
+    # (the hardware exp2 and numpy's may differ in the last bits of single precision)
     assert kt.shape[2] == nz + 1  # interfaces 0 .. nz
-    np.testing.assert_allclose(kt[:, :, 1:nz][both], want, rtol=1e-15, atol=0)
-    np.testing.assert_allclose(kv[:, :, 1:nz][both], want_v, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(kt[:, :, 1:nz][both], want, rtol=4e-7, atol=0)
+    np.testing.assert_allclose(kv[:, :, 1:nz][both], want_v, rtol=4e-7, atol=0)
     assert np.all(bl[wet] == 0.0)
 
 
